@@ -1,0 +1,218 @@
+"""Benchmark of the 3D neighbourhood filters on one MI355X.
+
+Default workload (uint16 1024x1024x512, Poisson(100) background plus 500
+Gaussian blobs of amplitude 2000 and sigma 4, seed 2): one step runs the
+separable suite as the reference's single-axis drop-in launches,
+  gaussian_blur_separable_3d  sigma=(3,3,1.5) -> N=(25,25,13), u16 -> f32 -> f32 -> f32
+  mean/minimum/maximum_separable_3d  radius 3 -> N=7 on each axis, u16 -> u16
+i.e. 12 launches and four 3D filters.  value = 4 * V / step time, in
+Gvoxel/s (voxels of 3D-filtered output per second).  Inputs are resident in
+HBM before timing starts.
+
+Also printed in the same JSON line:
+  roofline      the dominant kernel's algorithmic bytes per launch (one
+                read of src + one write of dst) / its mean launch time
+                (HIP events on the launch stream), against 8 TB/s
+  cpu_baseline  the oracle (C restatement of the reference kernels, OpenMP)
+                timed on this host's cores on a bounded sub-volume
+  launches      per-launch mean times and % of HBM peak
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "clij-opencl-kernels_amd"), os.path.join(ROOT, "oracle")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0
+CLK_USHORT, CLK_FLOAT, CLK_UCHAR = 3, 4, 1
+
+
+def make_c2_volume(torch, D, H, W, seed=2, nblobs=500, amp=2000.0, sigma=4.0, lam=100.0):
+    """uint16 volume in int16 storage: Poisson(lam) + Gaussian blobs, rounded
+    and saturated.  Generated on the GPU from a seeded generator."""
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    vol = torch.poisson(torch.full((D, H, W), lam, device="cuda"), generator=g)
+    cen = torch.rand((nblobs, 3), generator=g, device="cuda") * torch.tensor(
+        [D, H, W], device="cuda", dtype=torch.float32)
+    cen = cen.cpu().numpy()
+    r = int(4 * sigma)
+    for cz, cy, cx in cen:
+        z0, z1 = max(0, int(cz) - r), min(D, int(cz) + r + 1)
+        y0, y1 = max(0, int(cy) - r), min(H, int(cy) + r + 1)
+        x0, x1 = max(0, int(cx) - r), min(W, int(cx) + r + 1)
+        zz = torch.arange(z0, z1, device="cuda", dtype=torch.float32)[:, None, None] - float(cz)
+        yy = torch.arange(y0, y1, device="cuda", dtype=torch.float32)[None, :, None] - float(cy)
+        xx = torch.arange(x0, x1, device="cuda", dtype=torch.float32)[None, None, :] - float(cx)
+        vol[z0:z1, y0:y1, x0:x1] += amp * torch.exp(-(zz * zz + yy * yy + xx * xx) / (2 * sigma * sigma))
+    vol = vol.round_().clamp_(0, 65535)
+    assert float(vol.max()) < 32768
+    out = vol.to(torch.int16)
+    del vol
+    return out
+
+
+def suite_c2():
+    """(label, kernel, dim, N, s, src kind, dst kind) of one step."""
+    L = []
+    for dim, (N, s) in enumerate(((25, 3.0), (25, 3.0), (13, 1.5))):
+        L.append(("gaussian_blur_separable_3d dim=%d" % dim, "gaussian_blur_separable_3d", dim, N, s,
+                  "u16" if dim == 0 else "f32", "f32"))
+    for k in ("mean_separable_3d", "minimum_separable_3d", "maximum_separable_3d"):
+        for dim in range(3):
+            L.append(("%s dim=%d" % (k, dim), k, dim, 7, 0.0, "u16", "u16"))
+    return L
+
+
+def cpu_baseline(nthreads, sample=(128, 256, 256)):
+    """The oracle on a bounded sub-volume of the same workload, all host
+    cores: the same four 3D filters as three single-axis passes each."""
+    import clk_oracle as O
+    rng = np.random.default_rng(2)
+    D, H, W = sample
+    src = (rng.poisson(100.0, size=sample)).astype(np.uint16)
+    t0 = time.perf_counter()
+    g = O.separable(0, src, np.float32, 0, 25, 3.0, nthreads)
+    g = O.separable(0, g, np.float32, 1, 25, 3.0, nthreads)
+    g = O.separable(0, g, np.float32, 2, 13, 1.5, nthreads)
+    for op in (1, 2, 3):
+        a = O.separable(op, src, np.uint16, 0, 7, 0.0, nthreads)
+        a = O.separable(op, a, np.uint16, 1, 7, 0.0, nthreads)
+        a = O.separable(op, a, np.uint16, 2, 7, 0.0, nthreads)
+    dt = time.perf_counter() - t0
+    V = D * H * W
+    return {"value": round(4 * V / dt / 1e9, 4), "unit": "Gvoxel/s", "cores": int(nthreads),
+            "kind": "port",
+            "sample": "uint16 %dx%dx%d Poisson(100), the same 12 single-axis launches, "
+                      "oracle/clk_oracle.c with OpenMP, %.1f s" % (W, H, D, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size", default="1024x1024x512", help="WxHxD")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only", default=None, help="run only launches whose label contains this")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the filters shard by z-slab or by volume, and this "
+              "build has no multi-GPU path (see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import clkmi
+
+    W, H, D = (int(v) for v in args.size.split("x"))
+    V = W * H * D
+    torch.cuda.init()
+    src = make_c2_volume(torch, D, H, W)
+    f_a = torch.empty((D, H, W), device="cuda", dtype=torch.float32)
+    f_b = torch.empty((D, H, W), device="cuda", dtype=torch.float32)
+    u_a = torch.empty((D, H, W), device="cuda", dtype=torch.int16)
+    u_b = torch.empty((D, H, W), device="cuda", dtype=torch.int16)
+
+    def im(t, code):
+        return clkmi.Image(t.data_ptr(), code, W, H, D)
+
+    suite = [s for s in suite_c2() if args.only is None or args.only in s[0]]
+    # buffer plan per launch: gaussian u16->fa->fb->fa ; box ops u16->ua->ub->ua
+    plan = []
+    for label, k, dim, N, s, sk, dk in suite:
+        if k.startswith("gaussian"):
+            sbuf = {0: (src, CLK_USHORT), 1: (f_a, CLK_FLOAT), 2: (f_b, CLK_FLOAT)}[dim]
+            dbuf = {0: (f_a, CLK_FLOAT), 1: (f_b, CLK_FLOAT), 2: (f_a, CLK_FLOAT)}[dim]
+        else:
+            sbuf = {0: (src, CLK_USHORT), 1: (u_a, CLK_USHORT), 2: (u_b, CLK_USHORT)}[dim]
+            dbuf = {0: (u_a, CLK_USHORT), 1: (u_b, CLK_USHORT), 2: (u_a, CLK_USHORT)}[dim]
+        bpp = {"u16": 2, "f32": 4}
+        plan.append((label, getattr(clkmi, k), im(dbuf[0], dbuf[1]), im(sbuf[0], sbuf[1]), dim, N, s,
+                     V * (bpp[sk] + bpp[dk])))
+
+    stream = torch.cuda.current_stream()
+
+    def step(events=None):
+        for i, (label, fn, d, s_, dim, N, sg, _) in enumerate(plan):
+            if events is not None:
+                events[i][0].record(stream)
+            fn(d, s_, dim, N, sg, stream=stream)
+            if events is not None:
+                events[i][1].record(stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    # timed region: exactly K steps between synchronisations
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    ev = [[[torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)]
+           for _ in plan] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t_start.record(stream)
+    for k in range(args.steps):
+        step(ev[k])
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    total_ms = t_start.elapsed_time(t_end)
+    ms_per_step = total_ms / args.steps
+
+    launches = []
+    for i, (label, fn, d, s_, dim, N, sg, nbytes) in enumerate(plan):
+        ms = float(np.mean([ev[k][i][0].elapsed_time(ev[k][i][1]) for k in range(args.steps)]))
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        launches.append({"launch": label, "ms": round(ms, 4), "bytes": nbytes,
+                         "GBps": round(gbs, 1), "pct_hbm": round(100 * gbs / HBM_PEAK_GBS, 1)})
+    dom = max(launches, key=lambda r: r["ms"])
+    nfilters = len({p[0].split(" dim")[0] for p in plan})
+    value = nfilters * V / (ms_per_step * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu = cpu_baseline(nthreads)
+
+    out = {
+        "metric": "Gvoxel/s + % HBM peak, 3D separable/median filters; p50/p99 latency served",
+        "value": round(value, 3),
+        "unit": "Gvoxel/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: uint16 Poisson(100) + 500 Gaussian blobs (A=2000, sigma=4), seed 2",
+        "config": {
+            "workload": "separable suite on uint16 %dx%dx%d: gaussian sigma=(3,3,1.5)->f32, "
+                        "mean/min/max radius 3 ->u16, 12 single-axis drop-in launches" % (W, H, D),
+            "volume_xyz": [W, H, D],
+            "pixel": "uint16",
+            "filters_per_step": nfilters,
+        },
+        "roofline": {
+            "kernel": dom["launch"],
+            "bound": "hbm",
+            "achieved": dom["GBps"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(dom["GBps"] / HBM_PEAK_GBS, 4),
+            "traffic": None,
+        },
+        "cpu_baseline": cpu,
+        "launches": launches,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,81 @@
+// Device-side helpers shared by the gfx950 kernels: pixel traits, the
+// saturating float->pixel conversion (CONVERT_dst_PIXEL_TYPE), the widening
+// read (READ_src_IMAGE(...).x cast to float) and a correctly rounded divide
+// by a launch-uniform divisor.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace clk {
+
+enum Op : int { OP_GAUSS = 0, OP_MEAN = 1, OP_MIN = 2, OP_MAX = 3 };
+
+template <class T> struct Pix;
+template <> struct Pix<int8_t>   { static constexpr bool kInt = true;  static constexpr float lo = -128.f;   static constexpr float hi = 127.f; };
+template <> struct Pix<uint8_t>  { static constexpr bool kInt = true;  static constexpr float lo = 0.f;      static constexpr float hi = 255.f; };
+template <> struct Pix<int16_t>  { static constexpr bool kInt = true;  static constexpr float lo = -32768.f; static constexpr float hi = 32767.f; };
+template <> struct Pix<uint16_t> { static constexpr bool kInt = true;  static constexpr float lo = 0.f;      static constexpr float hi = 65535.f; };
+template <> struct Pix<float>    { static constexpr bool kInt = false; static constexpr float lo = 0.f;      static constexpr float hi = 0.f; };
+
+// CONVERT_dst_PIXEL_TYPE (reference README.md:35-47): identity for float;
+// for an integer type saturate to [min,max], truncate toward zero, NaN -> 0
+// (OpenCL 1.2 convert_<T>_sat with its default round-toward-zero).
+template <class T>
+__device__ __forceinline__ T convert_sat(float f) {
+  if constexpr (!Pix<T>::kInt) {
+    return f;
+  } else {
+    f = (f != f) ? 0.f : f;
+    f = fminf(fmaxf(f, Pix<T>::lo), Pix<T>::hi);
+    return static_cast<T>(static_cast<int>(f));
+  }
+}
+
+// Correctly rounded a / b for a launch-uniform divisor b >= 1 with
+// r = RN(1/b) computed on the host: q = RN(a*r) is faithful, the residual
+// a - b*q is exact under FMA, and RN(q + residual*r) is RN(a/b) (Markstein).
+// Inputs whose residual could underflow, and non-finite values, take the
+// IEEE divide.
+__device__ __forceinline__ float div_uniform(float a, float b, float r) {
+  const float q = a * r;
+  const float e = __builtin_fmaf(-q, b, a);
+  const float q1 = __builtin_fmaf(e, r, q);
+  const float aa = __builtin_fabsf(a);
+  if (__builtin_expect(!(aa >= 0x1p-96f && aa <= 0x1p+120f), 0)) {
+    return a / b;
+  }
+  return q1;
+}
+
+// Load COLS consecutive pixels of type T, widened to float.  `p` must be
+// aligned to sizeof(T)*COLS when COLS > 1.
+template <class T, int COLS> struct alignas(sizeof(T) * COLS) PixVec { T v[COLS]; };
+
+template <class T, int COLS>
+__device__ __forceinline__ void load_widen(const T* __restrict__ p, float (&out)[COLS]) {
+  if constexpr (COLS == 1) {
+    out[0] = static_cast<float>(p[0]);
+  } else {
+    const PixVec<T, COLS> v = *reinterpret_cast<const PixVec<T, COLS>*>(p);
+#pragma unroll
+    for (int i = 0; i < COLS; ++i) out[i] = static_cast<float>(v.v[i]);
+  }
+}
+
+template <class T, int COLS>
+__device__ __forceinline__ void store_vec(T* __restrict__ p, const T (&in)[COLS]) {
+  if constexpr (COLS == 1) {
+    p[0] = in[0];
+  } else {
+    PixVec<T, COLS> v;
+#pragma unroll
+    for (int i = 0; i < COLS; ++i) v.v[i] = in[i];
+    *reinterpret_cast<PixVec<T, COLS>*>(p) = v;
+  }
+}
+
+__device__ __forceinline__ int64_t clamp64(int64_t v, int64_t lo, int64_t hi) {
+  return v < lo ? lo : (v > hi ? hi : v);
+}
+
+}  // namespace clk

@@ -1,0 +1,62 @@
+// Internal interface between the C-ABI layer (clkmi_api.cpp) and the kernel
+// launchers (sep_kernels.hip, median_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/clkmi.h"
+
+namespace clk {
+
+constexpr int kMaxTaps = CLK_GAUSSIAN_MAX_TAPS;
+
+// Parameters of one single-axis separable pass.  The volume is viewed as
+// [outer][L][inner]: `inner` elements are contiguous, the filtered axis has
+// length L and stride `inner`.  dim 0 -> (inner=1, L=W, outer=H*D),
+// dim 1 -> (W, H, D), dim 2 -> (W*H, D, 1), any other dim -> (V, 1, 1) so
+// that every clamped tap reads the centre voxel.
+struct SepParams {
+  const void* src;
+  void* dst;
+  int64_t inner;
+  int64_t L;
+  int64_t outer;
+  int64_t ncolgroups;  // strided pass: column groups of COLS per slab
+  int64_t ntasks;      // x pass: rows * segments
+  int32_t nseg;        // segments along the axis (per column group / row)
+  int32_t seg;         // outputs per segment along the axis (multiple of K)
+  int32_t c;           // half width, >= 0
+  int32_t ntaps;       // 2c+1
+  float div;           // Gaussian: sum of weights; mean: tap count
+  float rdiv;          // RN(1/div)
+  int32_t lds_stride;  // x pass: floats of LDS per wave
+  int32_t vec_ok;      // x pass: rows are aligned for vector stores
+  float w[kMaxTaps];   // Gaussian weights in tap order v = -c..c
+};
+
+struct Status {
+  int32_t code = 0;
+  std::string msg;
+};
+
+// Reference-semantics Gaussian weight exp(x) for one tap, computed on the
+// host once per launch.
+float expf_weight(float x);
+
+// Single-axis separable launch; op is Op (clk_device.h).  Arguments are
+// already validated by the caller (types, sizes, no overlap).
+int32_t launch_separable(int op, hipStream_t st, const clk_image& dst,
+                         const clk_image& src, int32_t dim, int32_t N, float s,
+                         std::string* err);
+
+// median_box_3d launch; arguments validated by the caller.
+int32_t launch_median(hipStream_t st, const clk_image& dst, const clk_image& src,
+                      int32_t Nx, int32_t Ny, int32_t Nz, std::string* err);
+
+// Converting copy dst <- CONVERT(src) (used by the host chain for axes that
+// are not filtered).
+int32_t launch_convert_copy(hipStream_t st, const clk_image& dst,
+                            const clk_image& src, std::string* err);
+
+}  // namespace clk

@@ -1,0 +1,331 @@
+// C-ABI of libclkmi: argument validation, error reporting, by-name dispatch,
+// the 3-pass host chain, and device-memory/stream helpers.  Kernel launches
+// live in sep_launch.hip and median_kernels.hip.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <cmath>
+#include <string>
+
+#include "../../include/clkmi.h"
+#include "clk_device.h"
+#include "clk_internal.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int32_t fail(int32_t code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int32_t ok() { return CLK_SUCCESS; }
+
+int64_t voxels(const clk_image& im) { return im.width * im.height * im.depth; }
+
+int32_t check_image(const clk_image* im, const char* what) {
+  if (!im) return fail(CLK_INVALID_MEM_OBJECT, std::string(what) + ": image is NULL");
+  if (clk_dtype_size(im->dtype) == 0)
+    return fail(CLK_INVALID_VALUE, std::string(what) + ": invalid pixel type " + std::to_string(im->dtype));
+  if (im->width < 0 || im->height < 0 || im->depth < 0)
+    return fail(CLK_INVALID_VALUE, std::string(what) + ": negative size");
+  if (im->reserved != 0) return fail(CLK_INVALID_VALUE, std::string(what) + ": reserved field must be 0");
+  if (!im->ptr && voxels(*im) > 0) return fail(CLK_INVALID_MEM_OBJECT, std::string(what) + ": NULL device pointer");
+  return CLK_SUCCESS;
+}
+
+// Both images valid, same size, and their byte ranges disjoint (the kernels
+// read neighbours of src, so they cannot run in place).
+int32_t check_pair(const char* kernel, const clk_image* dst, const clk_image* src) {
+  int32_t rc = check_image(dst, "dst");
+  if (rc) return rc;
+  rc = check_image(src, "src");
+  if (rc) return rc;
+  if (dst->width != src->width || dst->height != src->height || dst->depth != src->depth)
+    return fail(CLK_INVALID_VALUE, std::string(kernel) + ": src and dst sizes differ");
+  const uintptr_t s0 = (uintptr_t)src->ptr, d0 = (uintptr_t)dst->ptr;
+  const uintptr_t s1 = s0 + (uintptr_t)(voxels(*src) * clk_dtype_size(src->dtype));
+  const uintptr_t d1 = d0 + (uintptr_t)(voxels(*dst) * clk_dtype_size(dst->dtype));
+  if (voxels(*dst) > 0 && s0 < d1 && d0 < s1)
+    return fail(CLK_MEM_COPY_OVERLAP, std::string(kernel) + ": src and dst overlap");
+  return CLK_SUCCESS;
+}
+
+int32_t separable(int op, const char* name, void* stream, const clk_image* dst, const clk_image* src,
+                  int32_t dim, int32_t N, float s) {
+  int32_t rc = check_pair(name, dst, src);
+  if (rc) return rc;
+  std::string err;
+  rc = clk::launch_separable(op, (hipStream_t)stream, *dst, *src, dim, N, s, &err);
+  if (rc) return fail(rc, std::string(name) + ": " + err);
+  return ok();
+}
+
+// ---- host chain -------------------------------------------------------------
+
+// Kernel size of a Gaussian sigma: int(8*sigma), made odd.  The reference
+// hosts choose N from sigma; this is the CLIJ convention.
+int32_t sigma_to_n(float sigma) {
+  int32_t n = (int32_t)(sigma * 8.0f);
+  if (n % 2 == 0) n += 1;
+  return n;
+}
+
+struct PassSpec {
+  bool active;
+  int32_t N;
+  float s;
+};
+
+// Runs the active passes in axis order x, y, z.  Intermediate images are
+// float32 and stream-ordered allocations; an inactive axis is skipped (the
+// next pass reads the previous image directly, which gives the same values
+// as a float copy); with no active axis the result is a converting copy.
+int32_t chain3(int op, const char* name, void* stream, const clk_image* dst, const clk_image* src,
+               const PassSpec (&ps)[3]) {
+  int32_t rc = check_pair(name, dst, src);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  std::string err;
+  int nact = 0;
+  for (int a = 0; a < 3; ++a) nact += ps[a].active;
+  if (nact == 0) {
+    rc = clk::launch_convert_copy(st, *dst, *src, &err);
+    return rc ? fail(rc, std::string(name) + ": " + err) : ok();
+  }
+  const int64_t V = voxels(*dst);
+  void* tmp[2] = {nullptr, nullptr};
+  const int ntmp = nact - 1 < 2 ? nact - 1 : 2;
+  for (int i = 0; i < ntmp; ++i) {
+    if (V > 0 && hipMallocAsync(&tmp[i], (size_t)V * sizeof(float), st) != hipSuccess) {
+      for (int j = 0; j < i; ++j) (void)hipFreeAsync(tmp[j], st);
+      return fail(CLK_MEM_OBJECT_ALLOCATION_FAILURE, std::string(name) + ": temporary allocation failed");
+    }
+  }
+  clk_image cur = *src;
+  int done = 0, ti = 0;
+  rc = 0;
+  for (int a = 0; a < 3 && rc == 0; ++a) {
+    if (!ps[a].active) continue;
+    ++done;
+    clk_image out;
+    if (done == nact) {
+      out = *dst;
+    } else {
+      out = *dst;
+      out.dtype = CLK_FLOAT;
+      out.ptr = tmp[ti];
+      ti ^= 1;
+    }
+    rc = clk::launch_separable(op, st, out, cur, a, ps[a].N, ps[a].s, &err);
+    cur = out;
+  }
+  for (int i = 0; i < ntmp; ++i)
+    if (tmp[i]) (void)hipFreeAsync(tmp[i], st);
+  return rc ? fail(rc, std::string(name) + ": " + err) : ok();
+}
+
+int32_t box_chain(int op, const char* name, void* stream, const clk_image* dst, const clk_image* src,
+                  int32_t rx, int32_t ry, int32_t rz) {
+  if (rx < 0 || ry < 0 || rz < 0) return fail(CLK_INVALID_VALUE, std::string(name) + ": negative radius");
+  const PassSpec ps[3] = {{rx > 0, 2 * rx + 1, 0.f}, {ry > 0, 2 * ry + 1, 0.f}, {rz > 0, 2 * rz + 1, 0.f}};
+  return chain3(op, name, stream, dst, src, ps);
+}
+
+// ---- by-name dispatch ---------------------------------------------------------
+
+struct NamedArgs {
+  const clk_image* dst = nullptr;
+  const clk_image* src = nullptr;
+  bool has[5] = {false, false, false, false, false};  // dim/Nx, N/Ny, s/Nz
+  int32_t iv[3] = {0, 0, 0};
+  float fv = 0.f;
+};
+
+bool strip_program_name(const char* in, std::string* out) {
+  std::string k(in);
+  const std::string suf = "_x.cl";
+  if (k.size() > suf.size() && k.compare(k.size() - suf.size(), suf.size(), suf) == 0)
+    k = k.substr(0, k.size() - suf.size());
+  *out = k;
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t clk_dtype_size(int32_t dtype) {
+  switch (dtype) {
+    case CLK_CHAR: case CLK_UCHAR: return 1;
+    case CLK_SHORT: case CLK_USHORT: return 2;
+    case CLK_FLOAT: return 4;
+  }
+  return 0;
+}
+
+const char* clk_last_error_string(void) { return g_last_error.c_str(); }
+
+int32_t clk_version(void) { return 100; }
+
+int32_t clk_gaussian_blur_separable_3d(void* stream, const clk_image* dst, const clk_image* src,
+                                       int32_t dim, int32_t N, float s) {
+  return separable(clk::OP_GAUSS, "gaussian_blur_separable_3d", stream, dst, src, dim, N, s);
+}
+
+int32_t clk_mean_separable_3d(void* stream, const clk_image* dst, const clk_image* src, int32_t dim,
+                              int32_t N, float s) {
+  return separable(clk::OP_MEAN, "mean_separable_3d", stream, dst, src, dim, N, s);
+}
+
+int32_t clk_minimum_separable_3d(void* stream, const clk_image* dst, const clk_image* src,
+                                 int32_t dim, int32_t N, float s) {
+  return separable(clk::OP_MIN, "minimum_separable_3d", stream, dst, src, dim, N, s);
+}
+
+int32_t clk_maximum_separable_3d(void* stream, const clk_image* dst, const clk_image* src,
+                                 int32_t dim, int32_t N, float s) {
+  return separable(clk::OP_MAX, "maximum_separable_3d", stream, dst, src, dim, N, s);
+}
+
+int32_t clk_median_box_3d(void* stream, const clk_image* dst, const clk_image* src, int32_t Nx,
+                          int32_t Ny, int32_t Nz) {
+  int32_t rc = check_pair("median_box_3d", dst, src);
+  if (rc) return rc;
+  std::string err;
+  rc = clk::launch_median((hipStream_t)stream, *dst, *src, Nx, Ny, Nz, &err);
+  if (rc) return fail(rc, err);
+  return ok();
+}
+
+int32_t clk_execute(void* stream, const char* kernel_name, const clk_arg* args, int32_t nargs) {
+  if (!kernel_name) return fail(CLK_INVALID_KERNEL_NAME, "kernel name is NULL");
+  if (nargs < 0 || (nargs > 0 && !args)) return fail(CLK_INVALID_KERNEL_ARGS, "invalid argument list");
+  std::string k;
+  strip_program_name(kernel_name, &k);
+  int op = -1;
+  if (k == "gaussian_blur_separable_3d") op = clk::OP_GAUSS;
+  else if (k == "mean_separable_3d") op = clk::OP_MEAN;
+  else if (k == "minimum_separable_3d") op = clk::OP_MIN;
+  else if (k == "maximum_separable_3d") op = clk::OP_MAX;
+  else if (k == "median_box_3d") op = 4;
+  if (op < 0) return fail(CLK_INVALID_KERNEL_NAME, "unknown kernel: " + k);
+  // parameter names in declaration order (gaussian_blur_separable_3d_x.cl:8-11,
+  // median_box_3d_x.cl:25-29)
+  static const char* sep_names[3] = {"dim", "N", "s"};
+  static const char* med_names[3] = {"Nx", "Ny", "Nz"};
+  const char** names = (op == 4) ? med_names : sep_names;
+  NamedArgs na;
+  for (int32_t i = 0; i < nargs; ++i) {
+    const clk_arg& a = args[i];
+    if (!a.name) return fail(CLK_INVALID_KERNEL_ARGS, k + ": argument without a name");
+    const std::string nm(a.name);
+    if (nm == "dst" || nm == "src") {
+      if (a.kind != CLK_ARG_IMAGE) return fail(CLK_INVALID_KERNEL_ARGS, k + ": " + nm + " must be an image");
+      (nm == "dst" ? na.dst : na.src) = a.image;
+      continue;
+    }
+    int slot = -1;
+    for (int j = 0; j < 3; ++j)
+      if (nm == names[j]) slot = j;
+    if (slot < 0) return fail(CLK_INVALID_KERNEL_ARGS, k + ": unknown parameter " + nm);
+    if (a.kind == CLK_ARG_IMAGE) return fail(CLK_INVALID_KERNEL_ARGS, k + ": " + nm + " must be a number");
+    const bool is_float_param = (op != 4 && slot == 2);
+    if (is_float_param) na.fv = (a.kind == CLK_ARG_FLOAT) ? a.f : (float)a.i;
+    else na.iv[slot] = (a.kind == CLK_ARG_INT) ? a.i : (int32_t)a.f;
+    na.has[slot] = true;
+  }
+  if (!na.dst || !na.src) return fail(CLK_INVALID_KERNEL_ARGS, k + ": dst and src are required");
+  if (op == 4) {
+    for (int j = 0; j < 3; ++j)
+      if (!na.has[j]) return fail(CLK_INVALID_KERNEL_ARGS, k + ": missing parameter " + names[j]);
+    return clk_median_box_3d(stream, na.dst, na.src, na.iv[0], na.iv[1], na.iv[2]);
+  }
+  if (!na.has[0] || !na.has[1])
+    return fail(CLK_INVALID_KERNEL_ARGS, k + ": missing parameter dim or N");
+  if (op == clk::OP_GAUSS && !na.has[2]) return fail(CLK_INVALID_KERNEL_ARGS, k + ": missing parameter s");
+  return separable(op, k.c_str(), stream, na.dst, na.src, na.iv[0], na.iv[1], na.fv);
+}
+
+int32_t clk_gaussian_blur_3d(void* stream, const clk_image* dst, const clk_image* src, float sx,
+                             float sy, float sz) {
+  const PassSpec ps[3] = {{sx > 0, sigma_to_n(sx), sx}, {sy > 0, sigma_to_n(sy), sy},
+                          {sz > 0, sigma_to_n(sz), sz}};
+  return chain3(clk::OP_GAUSS, "gaussian_blur_3d", stream, dst, src, ps);
+}
+
+int32_t clk_mean_box_3d(void* stream, const clk_image* dst, const clk_image* src, int32_t rx,
+                        int32_t ry, int32_t rz) {
+  return box_chain(clk::OP_MEAN, "mean_box_3d", stream, dst, src, rx, ry, rz);
+}
+
+int32_t clk_minimum_box_3d(void* stream, const clk_image* dst, const clk_image* src, int32_t rx,
+                           int32_t ry, int32_t rz) {
+  return box_chain(clk::OP_MIN, "minimum_box_3d", stream, dst, src, rx, ry, rz);
+}
+
+int32_t clk_maximum_box_3d(void* stream, const clk_image* dst, const clk_image* src, int32_t rx,
+                           int32_t ry, int32_t rz) {
+  return box_chain(clk::OP_MAX, "maximum_box_3d", stream, dst, src, rx, ry, rz);
+}
+
+int32_t clk_init(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(CLK_OUT_OF_RESOURCES, "no HIP device");
+  return ok();
+}
+
+int32_t clk_malloc(void** ptr, int64_t bytes) {
+  if (!ptr || bytes < 0) return fail(CLK_INVALID_VALUE, "clk_malloc: invalid arguments");
+  if (hipMalloc(ptr, (size_t)(bytes > 0 ? bytes : 1)) != hipSuccess)
+    return fail(CLK_MEM_OBJECT_ALLOCATION_FAILURE, "clk_malloc: hipMalloc failed");
+  return ok();
+}
+
+int32_t clk_free(void* ptr) {
+  if (ptr && hipFree(ptr) != hipSuccess) return fail(CLK_INVALID_MEM_OBJECT, "clk_free: hipFree failed");
+  return ok();
+}
+
+int32_t clk_memcpy_h2d(void* dst_dev, const void* src_host, int64_t bytes, void* stream) {
+  if (bytes < 0 || (bytes > 0 && (!dst_dev || !src_host))) return fail(CLK_INVALID_VALUE, "clk_memcpy_h2d: invalid arguments");
+  if (bytes == 0) return ok();
+  if (hipMemcpyAsync(dst_dev, src_host, (size_t)bytes, hipMemcpyHostToDevice, (hipStream_t)stream) != hipSuccess ||
+      hipStreamSynchronize((hipStream_t)stream) != hipSuccess)
+    return fail(CLK_OUT_OF_RESOURCES, "clk_memcpy_h2d failed");
+  return ok();
+}
+
+int32_t clk_memcpy_d2h(void* dst_host, const void* src_dev, int64_t bytes, void* stream) {
+  if (bytes < 0 || (bytes > 0 && (!dst_host || !src_dev))) return fail(CLK_INVALID_VALUE, "clk_memcpy_d2h: invalid arguments");
+  if (bytes == 0) return ok();
+  if (hipMemcpyAsync(dst_host, src_dev, (size_t)bytes, hipMemcpyDeviceToHost, (hipStream_t)stream) != hipSuccess ||
+      hipStreamSynchronize((hipStream_t)stream) != hipSuccess)
+    return fail(CLK_OUT_OF_RESOURCES, "clk_memcpy_d2h failed");
+  return ok();
+}
+
+int32_t clk_stream_create(void** stream) {
+  if (!stream) return fail(CLK_INVALID_VALUE, "clk_stream_create: NULL");
+  hipStream_t s;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
+    return fail(CLK_OUT_OF_RESOURCES, "clk_stream_create failed");
+  *stream = (void*)s;
+  return ok();
+}
+
+int32_t clk_stream_destroy(void* stream) {
+  if (stream && hipStreamDestroy((hipStream_t)stream) != hipSuccess)
+    return fail(CLK_INVALID_VALUE, "clk_stream_destroy failed");
+  return ok();
+}
+
+int32_t clk_stream_sync(void* stream) {
+  const hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+  if (e != hipSuccess) return fail(CLK_OUT_OF_RESOURCES, std::string("clk_stream_sync: ") + hipGetErrorString(e));
+  return ok();
+}
+
+}  // extern "C"

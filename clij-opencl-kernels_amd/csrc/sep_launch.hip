@@ -1,0 +1,155 @@
+// Host side of one single-axis separable launch: maps (dim, N, s) of the
+// reference kernel signature onto SepParams and picks the pass kernel.
+// Also holds the small converting-copy and fill kernels.
+#include <math.h>
+#include <string.h>
+
+#include "clk_device.h"
+#include "clk_internal.h"
+#include "sep_impl.h"
+
+namespace clk {
+
+// exp() of one Gaussian tap argument.  The reference evaluates the OpenCL
+// builtin exp per voxel and tap (gaussian_blur_separable_3d_x.cl:29); here it
+// is evaluated once per launch on the host, in double precision and rounded
+// to float, so the weight is the correctly rounded value in all but
+// vanishingly rare cases (OpenCL 1.2 allows 3 ulp).
+float expf_weight(float x) { return (float)exp((double)x); }
+
+template <class TS, class TD>
+__global__ __launch_bounds__(256) void convert_copy_kernel(const TS* __restrict__ src,
+                                                           TD* __restrict__ dst, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
+    dst[i] = convert_sat<TD>(static_cast<float>(src[i]));
+}
+
+template <class TD>
+__global__ __launch_bounds__(256) void fill_kernel(TD* __restrict__ dst, int64_t n, TD value) {
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) dst[i] = value;
+}
+
+static uint32_t grid_for(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  return (uint32_t)b;
+}
+
+template <class TS, class TD>
+static int32_t copy_typed(hipStream_t st, const clk_image& dst, const clk_image& src, int64_t n) {
+  hipLaunchKernelGGL((convert_copy_kernel<TS, TD>), dim3(grid_for(n)), dim3(256), 0, st,
+                     static_cast<const TS*>(src.ptr), static_cast<TD*>(dst.ptr), n);
+  return 0;
+}
+
+template <class TS>
+static int32_t copy_dst(hipStream_t st, const clk_image& dst, const clk_image& src, int64_t n) {
+  switch (dst.dtype) {
+    case CLK_CHAR:   return copy_typed<TS, int8_t>(st, dst, src, n);
+    case CLK_UCHAR:  return copy_typed<TS, uint8_t>(st, dst, src, n);
+    case CLK_SHORT:  return copy_typed<TS, int16_t>(st, dst, src, n);
+    case CLK_USHORT: return copy_typed<TS, uint16_t>(st, dst, src, n);
+    case CLK_FLOAT:  return copy_typed<TS, float>(st, dst, src, n);
+  }
+  return CLK_INVALID_VALUE;
+}
+
+int32_t launch_convert_copy(hipStream_t st, const clk_image& dst, const clk_image& src,
+                            std::string* err) {
+  const int64_t n = dst.width * dst.height * dst.depth;
+  if (n == 0) return 0;
+  int32_t rc = CLK_INVALID_VALUE;
+  switch (src.dtype) {
+    case CLK_CHAR:   rc = copy_dst<int8_t>(st, dst, src, n); break;
+    case CLK_UCHAR:  rc = copy_dst<uint8_t>(st, dst, src, n); break;
+    case CLK_SHORT:  rc = copy_dst<int16_t>(st, dst, src, n); break;
+    case CLK_USHORT: rc = copy_dst<uint16_t>(st, dst, src, n); break;
+    case CLK_FLOAT:  rc = copy_dst<float>(st, dst, src, n); break;
+  }
+  if (rc != 0) { *err = "invalid pixel type"; return rc; }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) { *err = std::string("kernel launch failed: ") + hipGetErrorString(e); return CLK_OUT_OF_RESOURCES; }
+  return 0;
+}
+
+// Every voxel gets CONVERT_dst(NaN): what the reference computes for mean and
+// Gaussian when N < 0 leaves the tap loop empty (0/0).
+static int32_t fill_nan(hipStream_t st, const clk_image& dst, std::string* err) {
+  const int64_t n = dst.width * dst.height * dst.depth;
+  if (n == 0) return 0;
+  const uint32_t g = grid_for(n);
+  switch (dst.dtype) {
+    case CLK_CHAR:   hipLaunchKernelGGL((fill_kernel<int8_t>), dim3(g), dim3(256), 0, st, (int8_t*)dst.ptr, n, (int8_t)0); break;
+    case CLK_UCHAR:  hipLaunchKernelGGL((fill_kernel<uint8_t>), dim3(g), dim3(256), 0, st, (uint8_t*)dst.ptr, n, (uint8_t)0); break;
+    case CLK_SHORT:  hipLaunchKernelGGL((fill_kernel<int16_t>), dim3(g), dim3(256), 0, st, (int16_t*)dst.ptr, n, (int16_t)0); break;
+    case CLK_USHORT: hipLaunchKernelGGL((fill_kernel<uint16_t>), dim3(g), dim3(256), 0, st, (uint16_t*)dst.ptr, n, (uint16_t)0); break;
+    case CLK_FLOAT:  hipLaunchKernelGGL((fill_kernel<float>), dim3(g), dim3(256), 0, st, (float*)dst.ptr, n, __builtin_nanf("")); break;
+    default: *err = "invalid dst pixel type"; return CLK_INVALID_VALUE;
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) { *err = std::string("kernel launch failed: ") + hipGetErrorString(e); return CLK_OUT_OF_RESOURCES; }
+  return 0;
+}
+
+int32_t launch_separable(int op, hipStream_t st, const clk_image& dst, const clk_image& src,
+                         int32_t dim, int32_t N, float s, std::string* err) {
+  const int64_t W = dst.width, H = dst.height, D = dst.depth;
+  if (W * H * D == 0) return 0;
+  int c = (N - 1) / 2;  // C division truncates toward zero, as the kernels' (N-1)/2
+  if (c < 0) {
+    // empty tap loop: min/max keep the centre (== a one-tap window);
+    // mean and Gaussian divide 0 by 0.
+    if (op == OP_MIN || op == OP_MAX) c = 0;
+    else return fill_nan(st, dst, err);
+  }
+  if (op == OP_GAUSS && 2 * (int64_t)c + 1 > kMaxTaps) {
+    *err = "gaussian_blur_separable_3d: N exceeds CLK_GAUSSIAN_MAX_TAPS";
+    return CLK_INVALID_VALUE;
+  }
+  SepParams* pp = new (std::nothrow) SepParams;
+  if (!pp) { *err = "host allocation failed"; return CLK_OUT_OF_RESOURCES; }
+  SepParams& p = *pp;
+  memset(&p, 0, sizeof(p));
+  p.src = src.ptr;
+  p.dst = dst.ptr;
+  p.c = c;
+  p.ntaps = 2 * c + 1;
+  int axis_is_x = 0;
+  switch (dim) {
+    case 0: p.inner = 1; p.L = W; p.outer = H * D; axis_is_x = 1; break;
+    case 1: p.inner = W; p.L = H; p.outer = D; break;
+    case 2: p.inner = W * H; p.L = D; p.outer = 1; break;
+    default: p.inner = W * H * D; p.L = 1; p.outer = 1; break;  // dir = (0,0,0)
+  }
+  if (op == OP_GAUSS) {
+    // gaussian_blur_separable_3d_x.cl:24,28-31 in float: n = -2*s*s,
+    // h = exp((v*v)/n), hsum accumulated in tap order.
+    const float n = -2 * s * s;
+    float hsum = 0.f;
+    for (int v = -c; v <= c; ++v) {
+      const float h = expf_weight((float)(v * v) / n);
+      p.w[v + c] = h;
+      hsum += h;
+    }
+    p.div = hsum;
+    p.rdiv = 1.0f / hsum;
+  } else if (op == OP_MEAN) {
+    p.div = (float)(2 * c + 1);  // mean_separable_3d_x.cl:26 counts the taps in float
+    p.rdiv = 1.0f / p.div;
+  }
+  int32_t rc;
+  switch (op) {
+    case OP_GAUSS: rc = launch_sep_gauss(st, p, axis_is_x, src.dtype, dst.dtype, err); break;
+    case OP_MEAN:  rc = launch_sep_mean(st, p, axis_is_x, src.dtype, dst.dtype, err); break;
+    case OP_MIN:   rc = launch_sep_min(st, p, axis_is_x, src.dtype, dst.dtype, err); break;
+    case OP_MAX:   rc = launch_sep_max(st, p, axis_is_x, src.dtype, dst.dtype, err); break;
+    default: *err = "invalid op"; rc = CLK_INVALID_VALUE; break;
+  }
+  delete pp;
+  return rc;
+}
+
+}  // namespace clk

@@ -1,0 +1,178 @@
+/*
+ * clkmi.h -- C-ABI of the MI355X (gfx950) implementation of the 3D
+ * neighbourhood-filter kernels of clEsperanto/clij-opencl-kernels.
+ *
+ * Each typed entry point replaces one OpenCL enqueue of the kernel of the
+ * same name: same named parameters, same meaning, same pixel types
+ * (char/uchar/short/ushort/float), clamp-to-edge border and saturating
+ * CONVERT_dst_PIXEL_TYPE.  The launch domain is the dst image, exactly as the
+ * reference host enqueues global = (W,H,D) of dst.
+ *
+ * Conventions
+ *   - Images are dense device buffers, x fastest:
+ *       offset(x,y,z) = x + y*width + z*width*height      (elements)
+ *   - Every entry point is asynchronous on `stream` (a hipStream_t, NULL =
+ *     the null stream) and returns 0 or a negative OpenCL-style error code.
+ *   - No C++ exception crosses this boundary.  clk_last_error_string() gives
+ *     the message of the last failing call on the calling thread.
+ *   - Calls are thread-safe.  The library borrows the caller's pointers until
+ *     the stream reaches the enqueued work; it never frees caller memory.
+ *   - src and dst must not overlap (the kernels read neighbours of src);
+ *     overlap returns CLK_MEM_COPY_OVERLAP.
+ */
+#ifndef CLKMI_H
+#define CLKMI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Pixel types: the values IMAGE_*_PIXEL_TYPE expands to
+ * (reference README.md:64-73). */
+typedef enum clk_dtype {
+  CLK_CHAR   = 0, /* int8   */
+  CLK_UCHAR  = 1, /* uint8  */
+  CLK_SHORT  = 2, /* int16  */
+  CLK_USHORT = 3, /* uint16 */
+  CLK_FLOAT  = 4  /* float32 */
+} clk_dtype;
+
+/* Error codes: the OpenCL cl_int values a CLIJ host already handles. */
+#define CLK_SUCCESS                          0
+#define CLK_OUT_OF_RESOURCES                (-5)
+#define CLK_MEM_OBJECT_ALLOCATION_FAILURE   (-4)
+#define CLK_MEM_COPY_OVERLAP                (-8)
+#define CLK_INVALID_VALUE                  (-30)
+#define CLK_INVALID_MEM_OBJECT             (-38)
+#define CLK_INVALID_KERNEL_NAME            (-46)
+#define CLK_INVALID_KERNEL_ARGS            (-52)
+#define CLK_INVALID_WORK_DIMENSION         (-53)
+
+/* Image descriptor: replaces a cl_mem buffer plus the compile-time
+ * GET_IMAGE_WIDTH/HEIGHT/DEPTH constants (reference README.md:75-87). */
+typedef struct clk_image {
+  void*   ptr;     /* device pointer to width*height*depth pixels */
+  int32_t dtype;   /* clk_dtype */
+  int32_t reserved;/* must be 0 */
+  int64_t width;   /* x */
+  int64_t height;  /* y */
+  int64_t depth;   /* z */
+} clk_image;
+
+/* Largest Gaussian tap count N accepted by
+ * clk_gaussian_blur_separable_3d (weights travel as kernel arguments). */
+#define CLK_GAUSSIAN_MAX_TAPS 511
+
+/* ---- the five kernels of the path ------------------------------------- */
+
+/* gaussian_blur_separable_3d(dst, src, dim, N, s)
+ * replaces kernels/gaussian_blur_separable_3d_x.cl:6-12.
+ * 1D Gaussian along axis `dim` (0=x,1=y,2=z; any other value makes every tap
+ * read the centre voxel, as the reference's dir vector does), c=(N-1)/2,
+ * weights exp(v*v/(-2*s*s)), normalised by their sum. */
+int32_t clk_gaussian_blur_separable_3d(void* stream, const clk_image* dst,
+                                       const clk_image* src, int32_t dim,
+                                       int32_t N, float s);
+
+/* mean_separable_3d(dst, src, dim, N, s)
+ * replaces kernels/mean_separable_3d_x.cl:3-10; s is accepted and ignored
+ * as in the reference. */
+int32_t clk_mean_separable_3d(void* stream, const clk_image* dst,
+                              const clk_image* src, int32_t dim, int32_t N,
+                              float s);
+
+/* minimum_separable_3d(dst, src, dim, N, s)
+ * replaces kernels/minimum_separable_3d_x.cl:3-10. */
+int32_t clk_minimum_separable_3d(void* stream, const clk_image* dst,
+                                 const clk_image* src, int32_t dim, int32_t N,
+                                 float s);
+
+/* maximum_separable_3d(dst, src, dim, N, s)
+ * replaces kernels/maximum_separable_3d_x.cl:3-10. */
+int32_t clk_maximum_separable_3d(void* stream, const clk_image* dst,
+                                 const clk_image* src, int32_t dim, int32_t N,
+                                 float s);
+
+/* median_box_3d(dst, src, Nx, Ny, Nz)
+ * replaces kernels/median_box_3d_x.cl:23-30.  The host define
+ * MAX_ARRAY_SIZE (median_box_3d_x.cl:36) is not needed: any box works. */
+int32_t clk_median_box_3d(void* stream, const clk_image* dst,
+                          const clk_image* src, int32_t Nx, int32_t Ny,
+                          int32_t Nz);
+
+/* ---- by-name dispatch (CLIJ execute(kernelName, parameterMap)) --------- */
+
+typedef enum clk_arg_kind {
+  CLK_ARG_IMAGE = 0,
+  CLK_ARG_INT   = 1,
+  CLK_ARG_FLOAT = 2
+} clk_arg_kind;
+
+typedef struct clk_arg {
+  const char*      name;  /* parameter name as declared in the .cl kernel */
+  int32_t          kind;  /* clk_arg_kind */
+  int32_t          i;     /* CLK_ARG_INT */
+  float            f;     /* CLK_ARG_FLOAT */
+  int32_t          pad;
+  const clk_image* image; /* CLK_ARG_IMAGE */
+} clk_arg;
+
+/* Launch a kernel by its reference name ("gaussian_blur_separable_3d", or the
+ * program file name "gaussian_blur_separable_3d_x.cl") with arguments bound
+ * by parameter name, as a CLIJ host binds them (reference README.md:26).
+ * An int argument is accepted for a float parameter and vice versa, as the
+ * reference hosts convert Java/Python numbers.  Unknown names or missing
+ * parameters return CLK_INVALID_KERNEL_ARGS; unknown kernels
+ * CLK_INVALID_KERNEL_NAME. */
+int32_t clk_execute(void* stream, const char* kernel_name, const clk_arg* args,
+                    int32_t nargs);
+
+/* ---- host chain: full 3D filters (three single-axis passes) ------------- */
+
+/* The reference leaves the 3-pass chain to its host.  These run the passes
+ * x, y, z with float32 temporaries (allocated and freed by the library,
+ * stream-ordered); an axis with sigma <= 0 (Gaussian) or radius 0 is a
+ * converting copy.  N for a Gaussian axis is int(8*sigma) rounded up to
+ * odd; for the box filters N = 2*radius+1. */
+int32_t clk_gaussian_blur_3d(void* stream, const clk_image* dst,
+                             const clk_image* src, float sigma_x,
+                             float sigma_y, float sigma_z);
+int32_t clk_mean_box_3d(void* stream, const clk_image* dst,
+                        const clk_image* src, int32_t rx, int32_t ry,
+                        int32_t rz);
+int32_t clk_minimum_box_3d(void* stream, const clk_image* dst,
+                           const clk_image* src, int32_t rx, int32_t ry,
+                           int32_t rz);
+int32_t clk_maximum_box_3d(void* stream, const clk_image* dst,
+                           const clk_image* src, int32_t rx, int32_t ry,
+                           int32_t rz);
+
+/* ---- device memory and streams (for hosts without their own HIP) ------- */
+
+int32_t clk_init(void);
+int32_t clk_malloc(void** ptr, int64_t bytes);
+int32_t clk_free(void* ptr);
+int32_t clk_memcpy_h2d(void* dst_dev, const void* src_host, int64_t bytes,
+                       void* stream);
+int32_t clk_memcpy_d2h(void* dst_host, const void* src_dev, int64_t bytes,
+                       void* stream);
+int32_t clk_stream_create(void** stream);
+int32_t clk_stream_destroy(void* stream);
+int32_t clk_stream_sync(void* stream);
+
+/* Bytes per pixel of a clk_dtype, or 0 for an invalid value. */
+int32_t clk_dtype_size(int32_t dtype);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* clk_last_error_string(void);
+
+/* Library version as major*10000 + minor*100 + patch. */
+int32_t clk_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CLKMI_H */

@@ -1,0 +1,238 @@
+/*
+ * clk_oracle.c -- CPU restatement of the five 3D neighbourhood kernels of
+ * clEsperanto/clij-opencl-kernels, voxel by voxel, in the reference's own
+ * operation order.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/,
+ * __graft_entry__.smoke() and the cpu_baseline leg of bench.py may load it.
+ * The product library (libclkmi) never links, calls or falls back to it.
+ *
+ * Each function cites the reference lines it restates.  Choices the
+ * reference leaves to parts that are not in it (the clij preamble and the
+ * OpenCL runtime) are made here explicitly and documented in place:
+ *   - READ_src_IMAGE with CLK_ADDRESS_CLAMP_TO_EDGE: each coordinate clamped
+ *     to [0, size-1]; the pixel is widened exactly and taken as float.
+ *   - CONVERT_dst_PIXEL_TYPE: float identity; integers saturate, truncate
+ *     toward zero, NaN -> 0 (OpenCL 1.2 convert_<T>_sat, default rtz).
+ *   - exp(): (float)exp((double)x), the correctly rounded value in all but
+ *     vanishingly rare cases (OpenCL 1.2 allows 3 ulp).
+ *   - res += h * x: contracted to one fused multiply-add, as OpenCL C's
+ *     default FP_CONTRACT ON gives on every FMA-capable device.  This file is
+ *     compiled with -ffp-contract=off so that fmaf() below is the only fusion.
+ *   - OpenCL min(x,y) = y < x ? y : x; max(x,y) = x < y ? y : x.
+ *
+ * Pixel types: 0 char, 1 uchar, 2 short, 3 ushort, 4 float (clkmi.h).
+ * Layout: offset(x,y,z) = x + y*W + z*W*H.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+enum { T_CHAR = 0, T_UCHAR = 1, T_SHORT = 2, T_USHORT = 3, T_FLOAT = 4 };
+
+typedef struct {
+  const void* p;
+  int t;
+  int64_t w, h, d;
+} img_t;
+
+static int64_t clampi(int64_t v, int64_t lo, int64_t hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* READ_src_IMAGE(src, sampler, pos).x cast to float, clamp-to-edge sampler
+ * (gaussian_blur_separable_3d_x.cl:4,30). */
+static float read_px(const img_t* im, int64_t x, int64_t y, int64_t z) {
+  x = clampi(x, 0, im->w - 1);
+  y = clampi(y, 0, im->h - 1);
+  z = clampi(z, 0, im->d - 1);
+  const int64_t i = x + y * im->w + z * im->w * im->h;
+  switch (im->t) {
+    case T_CHAR: return (float)((const int8_t*)im->p)[i];
+    case T_UCHAR: return (float)((const uint8_t*)im->p)[i];
+    case T_SHORT: return (float)((const int16_t*)im->p)[i];
+    case T_USHORT: return (float)((const uint16_t*)im->p)[i];
+    default: return ((const float*)im->p)[i];
+  }
+}
+
+static float sat_rtz(float f, float lo, float hi) {
+  if (f != f) return 0.f;
+  if (f >= hi) return hi;
+  if (f <= lo) return lo;
+  return (float)(int32_t)f; /* C conversion truncates toward zero */
+}
+
+/* CONVERT_dst_PIXEL_TYPE (README.md:35-47), returned as the float holding
+ * the converted value. */
+float clko_convert(float f, int t) {
+  switch (t) {
+    case T_CHAR: return sat_rtz(f, -128.f, 127.f);
+    case T_UCHAR: return sat_rtz(f, 0.f, 255.f);
+    case T_SHORT: return sat_rtz(f, -32768.f, 32767.f);
+    case T_USHORT: return sat_rtz(f, 0.f, 65535.f);
+    default: return f;
+  }
+}
+
+/* WRITE_dst_IMAGE(dst, coord, value) with an already converted value. */
+static void write_px(void* p, int t, int64_t i, float v) {
+  switch (t) {
+    case T_CHAR: ((int8_t*)p)[i] = (int8_t)v; break;
+    case T_UCHAR: ((uint8_t*)p)[i] = (uint8_t)v; break;
+    case T_SHORT: ((int16_t*)p)[i] = (int16_t)v; break;
+    case T_USHORT: ((uint16_t*)p)[i] = (uint16_t)v; break;
+    default: ((float*)p)[i] = v; break;
+  }
+}
+
+float clko_expf(float x) { return (float)exp((double)x); }
+
+/* gaussian_blur_separable_3d_x.cl:14-34 for one work-item (i,j,k). */
+static float gaussian_voxel(const img_t* src, int64_t i, int64_t j, int64_t k, int dim, int N, float s) {
+  const int64_t di = (dim == 0), dj = (dim == 1), dk = (dim == 2); /* :19 */
+  const int c = (N - 1) / 2;                                        /* :22 */
+  const float n = -2 * s * s;                                       /* :24 */
+  float res = 0;                                                    /* :26 */
+  float hsum = 0;                                                   /* :27 */
+  for (int v = -c; v <= c; v++) {                                   /* :28 */
+    const float h = clko_expf((float)(v * v) / n);                  /* :29 */
+    res = fmaf(h, read_px(src, i + v * di, j + v * dj, k + v * dk), res); /* :30 */
+    hsum += h;                                                      /* :31 */
+  }
+  res /= hsum; /* :33 */
+  return res;
+}
+
+/* mean_separable_3d_x.cl:12-29 for one work-item. */
+static float mean_voxel(const img_t* src, int64_t i, int64_t j, int64_t k, int dim, int N) {
+  const int64_t di = (dim == 0), dj = (dim == 1), dk = (dim == 2); /* :17 */
+  const int c = (N - 1) / 2;                                        /* :20 */
+  float res = 0;                                                    /* :22 */
+  float count = 0;                                                  /* :23 */
+  for (int v = -c; v <= c; v++) {                                   /* :24 */
+    res = res + read_px(src, i + v * di, j + v * dj, k + v * dk);   /* :25 */
+    count = count + 1;                                              /* :26 */
+  }
+  res /= count; /* :28 */
+  return res;
+}
+
+/* minimum_separable_3d_x.cl:12-23 / maximum_separable_3d_x.cl:12-23. */
+static float minmax_voxel(const img_t* src, int64_t i, int64_t j, int64_t k, int dim, int N, int is_max) {
+  const int64_t di = (dim == 0), dj = (dim == 1), dk = (dim == 2); /* :14 */
+  const int c = (N - 1) / 2;                                        /* :17 */
+  float res = read_px(src, i, j, k);                                /* :19 */
+  for (int v = -c; v <= c; v++) {                                   /* :20 */
+    const float x = read_px(src, i + v * di, j + v * dj, k + v * dk);
+    if (is_max) res = (res < x) ? x : res; /* :21 max(res, x) */
+    else res = (x < res) ? x : res;        /* :21 min(res, x) */
+  }
+  return res;
+}
+
+/* op: 0 Gaussian, 1 mean, 2 minimum, 3 maximum.  NDRange = dst size; src
+ * and dst must have the same size.  Returns 0, or -30 for bad arguments. */
+int clko_separable(int op, void* dst, int dst_t, const void* src, int src_t, int64_t W, int64_t H,
+                   int64_t D, int dim, int N, float s, int nthreads) {
+  if (op < 0 || op > 3 || src_t < 0 || src_t > 4 || dst_t < 0 || dst_t > 4) return -30;
+  const img_t im = {src, src_t, W, H, D};
+  if (nthreads < 1) nthreads = 1;
+  int64_t k, j;
+#pragma omp parallel for collapse(2) num_threads(nthreads) schedule(static)
+  for (k = 0; k < D; k++) {
+    for (j = 0; j < H; j++) {
+      for (int64_t i = 0; i < W; i++) {
+        float r;
+        switch (op) {
+          case 0: r = gaussian_voxel(&im, i, j, k, dim, N, s); break;
+          case 1: r = mean_voxel(&im, i, j, k, dim, N); break;
+          case 2: r = minmax_voxel(&im, i, j, k, dim, N, 0); break;
+          default: r = minmax_voxel(&im, i, j, k, dim, N, 1); break;
+        }
+        write_px(dst, dst_t, i + j * W + k * W * H, clko_convert(r, dst_t)); /* WRITE(CONVERT(res)) */
+      }
+    }
+  }
+  return 0;
+}
+
+/* (dst_t)(float)value: the implicit conversion of median_box_3d_x.cl:55.
+ * Out of range it is undefined in the reference; here it saturates. */
+static float to_dst_type(float f, int t) { return clko_convert(f, t); }
+
+/* sort() and median() of median_box_3d_x.cl:3-20: stable insertion sort
+ * with `<`, then element array_size/2. */
+static float median_of(float* a, int n) {
+  for (int i = 0; i < n; i++) {
+    int jj;
+    const float temp = a[i];
+    for (jj = i - 1; jj >= 0 && temp < a[jj]; jj--) a[jj + 1] = a[jj];
+    a[jj + 1] = temp;
+  }
+  return a[n / 2];
+}
+
+/* median_box_3d_x.cl:32-64.  Returns -30 when a negative N leaves the box
+ * empty (the reference then reads an uninitialised array element). */
+int clko_median_box_3d(void* dst, int dst_t, const void* src, int src_t, int64_t W, int64_t H, int64_t D,
+                       int Nx, int Ny, int Nz, int nthreads) {
+  if (src_t < 0 || src_t > 4 || dst_t < 0 || dst_t > 4) return -30;
+  const int ex = (Nx - 1) / 2, ey = (Ny - 1) / 2, ez = (Nz - 1) / 2; /* :39 */
+  if (ex < 0 || ey < 0 || ez < 0) return -30;
+  const int n = (2 * ex + 1) * (2 * ey + 1) * (2 * ez + 1);
+  const img_t im = {src, src_t, W, H, D};
+  if (nthreads < 1) nthreads = 1;
+  int rc = 0;
+#pragma omp parallel num_threads(nthreads)
+  {
+    float* arr = (float*)malloc(sizeof(float) * (size_t)n);
+    if (!arr) {
+#pragma omp atomic write
+      rc = -5;
+    }
+    int64_t k, j;
+#pragma omp for collapse(2) schedule(static)
+    for (k = 0; k < D; k++) {
+      for (j = 0; j < H; j++) {
+        if (!arr) continue;
+        for (int64_t i = 0; i < W; i++) {
+          int count = 0;
+          for (int x = -ex; x <= ex; x++)       /* :47 */
+            for (int y = -ey; y <= ey; y++)     /* :48 */
+              for (int z = -ez; z <= ez; z++) { /* :49 */
+                const float value_res = read_px(&im, i + x, j + y, k + z); /* :54 */
+                arr[count++] = to_dst_type(value_res, dst_t);               /* :55 */
+              }
+          const float res = median_of(arr, count); /* :63 */
+          write_px(dst, dst_t, i + j * W + k * W * H, res); /* :64, no CONVERT */
+        }
+      }
+    }
+    free(arr);
+  }
+  return rc;
+}
+
+/* Correctly rounded a/b through RN(1/b) and one FMA residual step, the
+ * formula the GPU kernels use for a launch-uniform divisor; exported so the
+ * tests can check it against the IEEE divide on many inputs. */
+float clko_div_uniform(float a, float b, float r) {
+  const float q = a * r;
+  const float e = fmaf(-q, b, a);
+  const float q1 = fmaf(e, r, q);
+  const float aa = fabsf(a);
+  if (!(aa >= 0x1p-96f && aa <= 0x1p+120f)) return a / b;
+  return q1;
+}
+
+int clko_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

@@ -1,0 +1,64 @@
+"""The C-ABI library loads and exports every symbol include/clkmi.h declares,
+and its host-side validation answers with OpenCL codes -- no GPU needed."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "clkmi.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^(?:int32_t|const char\*)\s+(clk_\w+)\s*\(", text, re.M)))
+
+
+def test_header_declares_the_five_kernels():
+    names = declared_functions()
+    for k in ("gaussian_blur_separable_3d", "mean_separable_3d", "minimum_separable_3d",
+              "maximum_separable_3d", "median_box_3d"):
+        assert "clk_" + k in names
+
+
+def test_library_exports_every_declared_symbol():
+    from clkmi import _lib
+    L = _lib.lib()
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(declared_functions()) == set(_lib.EXPORTED)
+
+
+def test_dtype_sizes_and_version():
+    from clkmi import _lib
+    L = _lib.lib()
+    assert [L.clk_dtype_size(i) for i in range(6)] == [1, 1, 2, 2, 4, 0]
+    assert L.clk_version() >= 100
+
+
+def test_host_validation_without_device():
+    import clkmi
+    from clkmi import _lib
+    L = _lib.lib()
+    img = _lib.ClkImage(0x1000, 3, 0, 8, 8, 8)
+    bad_type = _lib.ClkImage(0x100000, 7, 0, 8, 8, 8)
+    rc = L.clk_mean_separable_3d(None, ctypes.byref(img), None, 0, 3, 0.0)
+    assert rc == -38
+    rc = L.clk_mean_separable_3d(None, ctypes.byref(bad_type), ctypes.byref(img), 0, 3, 0.0)
+    assert rc == -30
+    assert b"pixel type" in L.clk_last_error_string()
+    rc = L.clk_maximum_separable_3d(None, ctypes.byref(img), ctypes.byref(img), 0, 3, 0.0)
+    assert rc == -8
+    with pytest.raises(clkmi.ClkError) as e:
+        clkmi.execute("sobel_3d", {"dst": clkmi.Image(0x1000, 1, 2, 2, 2)})
+    assert e.value.code == -46
+
+
+def test_unbuilt_library_fails_loudly(tmp_path, monkeypatch):
+    from clkmi import _lib
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "missing.so"))
+    with pytest.raises(RuntimeError, match="not built"):
+        _lib.lib()

@@ -1,0 +1,236 @@
+"""GPU parity: every result of libclkmi on cuda:0 must equal the oracle's
+(bit-exact for integer outputs; float outputs equal by value, i.e. +0 == -0
+and NaN == NaN) on the same seeded inputs."""
+import zlib
+
+import numpy as np
+import pytest
+
+import clk_oracle as O
+from helpers import DTYPES, DTYPE_NAMES, empty_device, from_device, random_volume, same_bits_or_value, to_device
+
+pytestmark = pytest.mark.gpu
+
+SEP_OPS = ["gaussian_blur_separable_3d", "mean_separable_3d", "minimum_separable_3d",
+           "maximum_separable_3d"]
+
+
+def gpu_separable(name, src, dst_t, dim, N, s):
+    import clkmi
+    ts, si = to_device(src)
+    td, di = empty_device(src.shape, dst_t)
+    getattr(clkmi, name)(di, si, dim, N, s)
+    return from_device(td, src.shape, dst_t)
+
+
+def gpu_median(src, dst_t, Nx, Ny, Nz):
+    import clkmi
+    ts, si = to_device(src)
+    td, di = empty_device(src.shape, dst_t)
+    clkmi.median_box_3d(di, si, Nx, Ny, Nz)
+    return from_device(td, src.shape, dst_t)
+
+
+def assert_parity(got, want, ctx):
+    if not same_bits_or_value(got, want):
+        bad = np.argwhere(~((got == want) | (np.isnan(got.astype(np.float64)) & np.isnan(want.astype(np.float64)))))
+        i = tuple(bad[0])
+        raise AssertionError("%s: %d mismatches, first at (z,y,x)=%s got %r want %r"
+                             % (ctx, len(bad), i, got[i], want[i]))
+
+
+@pytest.mark.parametrize("name", SEP_OPS)
+@pytest.mark.parametrize("src_t", DTYPES, ids=lambda t: DTYPE_NAMES[t])
+@pytest.mark.parametrize("dst_t", DTYPES, ids=lambda t: DTYPE_NAMES[t])
+def test_separable_all_type_pairs(gpu, name, src_t, dst_t):
+    src = random_volume((9, 13, 21), src_t, seed=zlib.crc32(("%s%s%s" % (name, np.dtype(src_t), np.dtype(dst_t))).encode()) & 0xffff)
+    for dim in (0, 1, 2, 5):
+        got = gpu_separable(name, src, dst_t, dim, 7, 1.3)
+        want = O.separable(name, src, dst_t, dim, 7, 1.3)
+        assert_parity(got, want, "%s dim=%d" % (name, dim))
+
+
+@pytest.mark.parametrize("name", SEP_OPS)
+@pytest.mark.parametrize("N", [-3, 0, 1, 2, 3, 5, 8, 9, 16, 17, 25, 33, 64, 131])
+def test_separable_tap_counts(gpu, name, N):
+    src = random_volume((19, 23, 37), np.uint16, seed=N + 500)
+    srcf = random_volume((19, 23, 37), np.float32, seed=N + 600)
+    for dim in (0, 1, 2):
+        for s_, dt in ((src, np.float32), (src, np.uint16), (srcf, np.float32)):
+            got = gpu_separable(name, s_, dt, dim, N, 2.2)
+            want = O.separable(name, s_, dt, dim, N, 2.2)
+            assert_parity(got, want, "%s N=%d dim=%d %s->%s" % (name, N, dim, s_.dtype, np.dtype(dt)))
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (1, 1, 5), (3, 1, 1), (2, 3, 1), (1, 600, 3),
+                                   (4, 4, 1030), (3, 2, 513), (70, 5, 8), (2, 300, 4)])
+def test_separable_odd_shapes(gpu, shape):
+    src = random_volume(shape, np.int16, seed=sum(shape))
+    for name in SEP_OPS:
+        for dim in (0, 1, 2):
+            for N in (3, 11):
+                got = gpu_separable(name, src, np.float32, dim, N, 1.7)
+                want = O.separable(name, src, np.float32, dim, N, 1.7)
+                assert_parity(got, want, "%s %s dim=%d N=%d" % (name, shape, dim, N))
+
+
+def test_separable_large_halo_along_x(gpu):
+    # c > 1024 along x takes the strided kernel with one row per lane.
+    src = random_volume((2, 3, 2100), np.uint8, seed=3)
+    for name in ("mean_separable_3d", "maximum_separable_3d"):
+        got = gpu_separable(name, src, np.uint8, 0, 2301, 0.0)
+        want = O.separable(name, src, np.uint8, 0, 2301, 0.0)
+        assert_parity(got, want, name)
+
+
+def test_float_special_values(gpu):
+    src = random_volume((5, 6, 40), np.float32, seed=8)
+    src[0, 0, 3] = np.inf
+    src[1, 2, 10] = -np.inf
+    src[2, 3, 20] = np.nan
+    src[3, 4, 5] = -0.0
+    src[4, 5, 6] = 1e-42
+    for name in SEP_OPS:
+        for dim in (0, 1, 2):
+            got = gpu_separable(name, src, np.float32, dim, 5, 1.0)
+            want = O.separable(name, src, np.float32, dim, 5, 1.0)
+            assert_parity(got, want, "%s dim=%d" % (name, dim))
+            got = gpu_separable(name, src, np.int16, dim, 5, 1.0)
+            want = O.separable(name, src, np.int16, dim, 5, 1.0)
+            assert_parity(got, want, "%s dim=%d int16" % (name, dim))
+
+
+@pytest.mark.parametrize("src_t", DTYPES, ids=lambda t: DTYPE_NAMES[t])
+@pytest.mark.parametrize("dst_t", DTYPES, ids=lambda t: DTYPE_NAMES[t])
+def test_median_all_type_pairs(gpu, src_t, dst_t):
+    src = random_volume((7, 9, 13), src_t, seed=77)
+    for Ns in ((3, 3, 3), (5, 5, 5), (3, 1, 5), (1, 1, 1), (4, 2, 3)):
+        got = gpu_median(src, dst_t, *Ns)
+        want = O.median_box_3d(src, dst_t, *Ns)
+        assert_parity(got, want, "median %s" % (Ns,))
+
+
+def test_median_salt_and_pepper(gpu):
+    rng = np.random.default_rng(3)
+    src = rng.integers(40, 200, size=(16, 40, 50), endpoint=True).astype(np.uint8)
+    m = rng.random(src.shape) < 0.05
+    src[m] = np.where(rng.random(m.sum()) < 0.5, 0, 255)
+    for r in (1, 2):
+        n = 2 * r + 1
+        got = gpu_median(src, np.uint8, n, n, n)
+        want = O.median_box_3d(src, np.uint8, n, n, n)
+        assert_parity(got, want, "median r=%d" % r)
+
+
+def test_execute_by_name_matches_typed_call(gpu):
+    import clkmi
+    src = random_volume((6, 7, 8), np.uint16, seed=1)
+    ts, si = to_device(src)
+    for name in SEP_OPS:
+        td, di = empty_device(src.shape, np.float32)
+        clkmi.execute(name + "_x.cl", {"dst": di, "src": si, "dim": 1, "N": 5, "s": 1.0})
+        got = from_device(td, src.shape, np.float32)
+        assert_parity(got, O.separable(name, src, np.float32, 1, 5, 1.0), name)
+    td, di = empty_device(src.shape, np.uint16)
+    clkmi.execute("median_box_3d", {"src": si, "dst": di, "Nx": 3, "Ny": 3, "Nz": 3})
+    assert_parity(from_device(td, src.shape, np.uint16), O.median_box_3d(src, np.uint16, 3, 3, 3), "median")
+
+
+def test_host_chain_matches_oracle_chain(gpu):
+    import clkmi
+    src = random_volume((20, 31, 45), np.uint16, seed=12, hi=3000)
+    ts, si = to_device(src)
+    td, di = empty_device(src.shape, np.float32)
+    clkmi.gaussian_blur_3d(si, di, 2.0, 2.0, 1.0)
+    want = O.gaussian_blur_3d(src, np.float32, 2.0, 2.0, 1.0)
+    assert_parity(from_device(td, src.shape, np.float32), want, "gaussian chain")
+    td, di = empty_device(src.shape, np.float32)
+    clkmi.gaussian_blur_3d(si, di, 0.0, 1.5, 0.0)
+    want = O.gaussian_blur_3d(src, np.float32, 0.0, 1.5, 0.0)
+    assert_parity(from_device(td, src.shape, np.float32), want, "gaussian chain, y only")
+    for fn, op in (("mean_box_3d", O.OP_MEAN), ("minimum_box_3d", O.OP_MIN),
+                   ("maximum_box_3d", O.OP_MAX)):
+        td, di = empty_device(src.shape, np.uint16)
+        getattr(clkmi, fn)(si, di, 3, 2, 1)
+        want = O.box_3d(op, src, np.uint16, 3, 2, 1)
+        assert_parity(from_device(td, src.shape, np.uint16), want, fn)
+
+
+def test_errors_are_opencl_codes(gpu):
+    import clkmi
+    src = random_volume((4, 5, 6), np.uint8, seed=0)
+    ts, si = to_device(src)
+    td, di = empty_device((4, 5, 7), np.uint8)
+    with pytest.raises(clkmi.ClkError) as e:
+        clkmi.mean_separable_3d(di, si, 0, 3)
+    assert e.value.code == -30
+    with pytest.raises(clkmi.ClkError) as e:
+        clkmi.mean_separable_3d(si, si, 0, 3)
+    assert e.value.code == -8
+    bad = clkmi.Image(si.ptr, 9, 6, 5, 4)
+    with pytest.raises(clkmi.ClkError) as e:
+        clkmi.mean_separable_3d(bad, si, 0, 3)
+    assert e.value.code == -30
+    with pytest.raises(clkmi.ClkError) as e:
+        clkmi.execute("no_such_kernel", {"dst": si, "src": si})
+    assert e.value.code == -46
+    td, di = empty_device((4, 5, 6), np.uint8)
+    with pytest.raises(clkmi.ClkError) as e:
+        clkmi.execute("mean_separable_3d", {"dst": di, "src": si, "dim": 0})
+    assert e.value.code == -52
+    with pytest.raises(clkmi.ClkError) as e:
+        clkmi.gaussian_blur_separable_3d(di, si, 0, 2001, 100.0)
+    assert e.value.code == -30
+    with pytest.raises(clkmi.ClkError) as e:
+        clkmi.median_box_3d(di, si, 3, -1, 3)
+    assert e.value.code == -30
+
+
+def _sample_lines(vol_t, dim, idx_a, idx_b):
+    """Lines along axis `dim` of a (z, y, x) torch volume at the given
+    coordinates of the other two axes, as a numpy (n, 1, L) array laid out
+    so the filtered axis is x."""
+    import torch
+    if dim == 0:
+        lines = vol_t[idx_a, idx_b, :]          # (z, y) -> x lines
+    elif dim == 1:
+        lines = vol_t[idx_a, :, idx_b]          # (z, x) -> y lines
+    else:
+        lines = vol_t[:, idx_a, idx_b].T        # (y, x) -> z lines
+    return lines.contiguous().cpu().numpy()[:, None, :]
+
+
+@pytest.mark.parametrize("name,N,s,dst", [
+    ("gaussian_blur_separable_3d", 25, 3.0, np.float32),
+    ("mean_separable_3d", 7, 0.0, np.uint16),
+    ("minimum_separable_3d", 7, 0.0, np.uint16),
+    ("maximum_separable_3d", 7, 0.0, np.uint16),
+])
+def test_full_size_sampled_lines(gpu, name, N, s, dst):
+    """1024x1024x512 uint16: each single-axis launch checked exactly on 512
+    random full lines against the oracle."""
+    import torch
+    import clkmi
+    D, H, W = 512, 1024, 1024
+    g = torch.Generator(device="cuda").manual_seed(2)
+    # uint16 data held in int16 storage (values < 4096 read the same)
+    src = torch.randint(0, 4096, (D, H, W), generator=g, device="cuda", dtype=torch.int16)
+    si = clkmi.Image(src.data_ptr(), 3, W, H, D)
+    tdt = torch.float32 if dst is np.float32 else torch.int16
+    out = torch.empty((D, H, W), device="cuda", dtype=tdt)
+    do = clkmi.Image(out.data_ptr(), 4 if dst is np.float32 else 3, W, H, D)
+    rng = np.random.default_rng(0)
+    for dim in (0, 1, 2):
+        getattr(clkmi, name)(do, si, dim, N, s)
+        torch.cuda.synchronize()
+        sizes = {0: (D, H), 1: (D, W), 2: (H, W)}[dim]
+        a = torch.from_numpy(rng.integers(0, sizes[0], 512)).cuda()
+        b = torch.from_numpy(rng.integers(0, sizes[1], 512)).cuda()
+        src_lines = _sample_lines(src, dim, a, b).view(np.uint16)
+        got = _sample_lines(out, dim, a, b)
+        if dst is not np.float32:
+            got = got.view(np.uint16)
+        want = O.separable(name, src_lines, dst, 0, N, s)
+        assert_parity(got, want, "%s dim=%d" % (name, dim))
+    del src, out
+    torch.cuda.empty_cache()

@@ -1,0 +1,274 @@
+"""Pins the CPU restatement (oracle/clk_oracle.c) against facts that do not
+come from it: scipy's rank filters and correlations, exact integer
+arithmetic, known answers worked out by hand, and IEEE division.  A dropped
+tap, a shifted window, a wrong border, a wrong conversion or a reordered sum
+fails one of these."""
+import math
+
+import numpy as np
+import pytest
+from scipy import ndimage
+
+import clk_oracle as O
+from helpers import DTYPES, random_volume
+
+SHAPES = [(11, 13, 17), (5, 1, 9), (1, 1, 23)]
+
+
+# ---- CONVERT_dst_PIXEL_TYPE -------------------------------------------------
+# Known answers of OpenCL 1.2 convert_<T>_sat with round-toward-zero.
+CONVERT_CASES = [
+    # value, char, uchar, short, ushort
+    (-1e9, -128, 0, -32768, 0),
+    (-129.0, -128, 0, -129, 0),
+    (-128.5, -128, 0, -128, 0),
+    (-0.9, 0, 0, 0, 0),
+    (-0.0, 0, 0, 0, 0),
+    (0.5, 0, 0, 0, 0),
+    (1.99, 1, 1, 1, 1),
+    (127.7, 127, 127, 127, 127),
+    (254.9, 127, 254, 254, 254),
+    (255.5, 127, 255, 255, 255),
+    (256.0, 127, 255, 256, 256),
+    (32767.9, 127, 255, 32767, 32767),
+    (65535.9, 127, 255, 32767, 65535),
+    (1e9, 127, 255, 32767, 65535),
+    (float("inf"), 127, 255, 32767, 65535),
+    (float("-inf"), -128, 0, -32768, 0),
+    (float("nan"), 0, 0, 0, 0),
+]
+
+
+@pytest.mark.parametrize("case", CONVERT_CASES)
+def test_convert_known_answers(case):
+    v = np.float32(case[0])
+    for dt, want in zip([np.int8, np.uint8, np.int16, np.uint16], case[1:]):
+        got = O.convert(np.array([v]), dt)[0]
+        assert int(got) == want, (case[0], dt, got, want)
+    f = O.convert(np.array([v], dtype=np.float32), np.float32)[0]
+    assert (np.isnan(f) and np.isnan(v)) or f == v
+
+
+def test_expf_matches_libm_within_half_ulp():
+    xs = np.float32(-np.arange(0, 2000) / 37.0)
+    for x in xs:
+        got = O.expf(x)
+        want = math.exp(float(x))
+        assert abs(float(got) - want) <= 0.5 * float(np.spacing(np.float32(want))) + 1e-45
+
+
+# ---- minimum / maximum: scipy rank filters, exactly ---------------------------
+
+@pytest.mark.parametrize("dt", DTYPES)
+@pytest.mark.parametrize("dim", [0, 1, 2])
+@pytest.mark.parametrize("N", [1, 3, 4, 7, 15])
+def test_minmax_pass_equals_scipy(dt, dim, N):
+    src = random_volume((11, 13, 17), dt, seed=N * 10 + dim)
+    c = (N - 1) // 2
+    axis = 2 - dim
+    got_min = O.minimum_separable_3d(src, dt, dim, N)
+    got_max = O.maximum_separable_3d(src, dt, dim, N)
+    want_min = ndimage.minimum_filter1d(src, 2 * c + 1, axis=axis, mode="nearest")
+    want_max = ndimage.maximum_filter1d(src, 2 * c + 1, axis=axis, mode="nearest")
+    assert np.array_equal(got_min, want_min)
+    assert np.array_equal(got_max, want_max)
+
+
+@pytest.mark.parametrize("r", [(1, 1, 1), (3, 3, 3), (2, 0, 5)])
+def test_minmax_chain_equals_3d_rank_filter(r):
+    src = random_volume((12, 15, 18), np.uint16, seed=7)
+    size = (2 * r[2] + 1, 2 * r[1] + 1, 2 * r[0] + 1)
+    got = O.box_3d(O.OP_MAX, src, np.uint16, *r)
+    assert np.array_equal(got, ndimage.maximum_filter(src, size=size, mode="nearest"))
+    got = O.box_3d(O.OP_MIN, src, np.uint16, *r)
+    assert np.array_equal(got, ndimage.minimum_filter(src, size=size, mode="nearest"))
+
+
+def test_minmax_saturating_conversion():
+    src = random_volume((6, 7, 9), np.int16, seed=3)
+    got = O.maximum_separable_3d(src, np.uint8, 1, 5)
+    want = np.clip(ndimage.maximum_filter1d(src, 5, axis=1, mode="nearest"), 0, 255).astype(np.uint8)
+    assert np.array_equal(got, want)
+
+
+# ---- mean: exact integer arithmetic --------------------------------------------
+
+def _window_sum_int(src, dim, N):
+    c = (N - 1) // 2
+    axis = 2 - dim
+    x = src.astype(np.int64)
+    pad = [(0, 0)] * 3
+    pad[axis] = (c, c)
+    xp = np.pad(x, pad, mode="edge")
+    acc = np.zeros_like(x)
+    L = x.shape[axis]
+    for v in range(2 * c + 1):
+        sl = [slice(None)] * 3
+        sl[axis] = slice(v, v + L)
+        acc += xp[tuple(sl)]
+    return acc, 2 * c + 1
+
+
+@pytest.mark.parametrize("src_t", [np.int8, np.uint8, np.int16, np.uint16])
+@pytest.mark.parametrize("dst_t", DTYPES)
+@pytest.mark.parametrize("dim", [0, 1, 2])
+def test_mean_integer_source_exact(src_t, dst_t, dim):
+    N = 7
+    src = random_volume((9, 10, 11), src_t, seed=dim + 100)
+    got = O.mean_separable_3d(src, dst_t, dim, N)
+    S, n = _window_sum_int(src, dim, N)
+    if np.dtype(dst_t).kind == "f":
+        want = (S.astype(np.float32) / np.float32(n)).astype(np.float32)  # IEEE divide of exact values
+    else:
+        q = np.trunc(S / n)  # |S| < 2^24: the float quotient truncates like integer division
+        info = np.iinfo(dst_t)
+        want = np.clip(q, info.min, info.max).astype(dst_t)
+    assert np.array_equal(got, want)
+
+
+def test_mean_float_sum_is_sequential():
+    # Window (2^24, 1, 1): summed left to right each +1 is lost to rounding
+    # (2^24 + 1 rounds to 2^24), so the mean is RN(2^24/3) = 5592405.5; any
+    # other order keeps 2 and gives 5592406.
+    src = np.array([[[16777216.0, 1.0, 1.0]]], dtype=np.float32)
+    got = O.mean_separable_3d(src, np.float32, 0, 3)
+    assert got[0, 0, 1] == np.float32(5592405.5)
+
+
+def test_mean_float_source_close_to_fp64():
+    src = random_volume((8, 9, 10), np.float32, seed=5)
+    for dim in (0, 1, 2):
+        got = O.mean_separable_3d(src, np.float32, dim, 9).astype(np.float64)
+        want = ndimage.uniform_filter1d(src.astype(np.float64), 9, axis=2 - dim, mode="nearest")
+        assert np.max(np.abs(got - want)) <= 1e-5 * np.max(np.abs(src))
+
+
+# ---- Gaussian: FP64 reference, impulse response, hand-worked taps ----------------
+
+@pytest.mark.parametrize("src_t", DTYPES)
+@pytest.mark.parametrize("dim", [0, 1, 2])
+@pytest.mark.parametrize("N,s", [(9, 1.0), (17, 2.0), (5, 0.7), (25, 3.0)])
+def test_gaussian_float_dst_close_to_fp64(src_t, dim, N, s):
+    src = random_volume((9, 14, 27), src_t, seed=N)
+    got = O.gaussian_blur_separable_3d(src, np.float32, dim, N, s).astype(np.float64)
+    want = O.gaussian_pass_fp64(src, dim, N, s)
+    scale = max(1.0, float(np.max(np.abs(src.astype(np.float64)))))
+    assert np.max(np.abs(got - want)) <= 1e-5 * scale
+
+
+@pytest.mark.parametrize("dst_t", [np.uint8, np.uint16, np.int16])
+def test_gaussian_integer_dst_matches_fp64_away_from_ties(dst_t):
+    src = random_volume((10, 12, 14), np.uint8, seed=11)
+    got = O.gaussian_blur_separable_3d(src, dst_t, 1, 9, 1.5).astype(np.int64)
+    want = O.gaussian_pass_fp64(src, 1, 9, 1.5)
+    info = np.iinfo(dst_t)
+    wt = np.clip(np.trunc(want), info.min, info.max).astype(np.int64)
+    frac = np.abs(want - np.round(want))
+    clear = frac > 1e-3  # not within float rounding of an integer boundary
+    assert clear.mean() > 0.9
+    assert np.array_equal(got[clear], wt[clear])
+    assert np.max(np.abs(got - wt)) <= 1
+
+
+def test_gaussian_impulse_gives_normalised_weights():
+    N, s = 13, 2.0
+    src = np.zeros((1, 1, 41), dtype=np.float32)
+    src[0, 0, 20] = 1.0
+    got = O.gaussian_blur_separable_3d(src, np.float32, 0, N, s)[0, 0]
+    v = np.arange(-6, 7)
+    w = np.exp(-(v * v) / (2 * s * s))
+    w /= w.sum()
+    np.testing.assert_allclose(got[14:27], w[::-1], rtol=2e-7, atol=1e-9)
+    assert np.all(got[:14] == 0) and np.all(got[27:] == 0)
+
+
+def test_gaussian_three_taps_by_hand():
+    # N=3, s=1: weights e^-0.5, 1, e^-0.5 (float32), hsum = (h + 1) + h.
+    h = np.float32(math.exp(-0.5))
+    hsum = np.float32(np.float32(h + np.float32(1.0)) + h)
+    src = np.array([[[10.0, 20.0, 40.0]]], dtype=np.float32)
+    got = O.gaussian_blur_separable_3d(src, np.float32, 0, 3, 1.0)[0, 0]
+    # centre voxel: fma chain h*10 -> +1*20 -> +h*40, exact in float64 then rounded per step
+    r = np.float32(np.float64(h) * 10.0)
+    r = np.float32(np.float64(r) + 20.0)
+    r = np.float32(np.float64(h) * 40.0 + np.float64(r))
+    assert got[1] == np.float32(r / hsum)
+    # clamped left border: taps read (10, 10, 20)
+    r0 = np.float32(np.float64(h) * 10.0)
+    r0 = np.float32(np.float64(r0) + 10.0)
+    r0 = np.float32(np.float64(h) * 20.0 + np.float64(r0))
+    assert got[0] == np.float32(r0 / hsum)
+
+
+def test_gaussian_constant_volume():
+    src = np.full((5, 6, 7), 123.0, dtype=np.float32)
+    for dim in (0, 1, 2):
+        got = O.gaussian_blur_separable_3d(src, np.float32, dim, 11, 2.0)
+        np.testing.assert_allclose(got, 123.0, rtol=1e-6)
+
+
+# ---- degenerate parameters ---------------------------------------------------------
+
+@pytest.mark.parametrize("op", [O.OP_GAUSS, O.OP_MEAN, O.OP_MIN, O.OP_MAX])
+def test_n1_is_converting_copy(op):
+    src = random_volume((4, 5, 6), np.int16, seed=2)
+    got = O.separable(op, src, np.uint8, 1, 1, 1.0)
+    assert np.array_equal(got, np.clip(src, 0, 255).astype(np.uint8))
+
+
+def test_out_of_range_dim_reads_centre_only():
+    src = random_volume((4, 5, 6), np.uint16, seed=4)
+    for op in (O.OP_MEAN, O.OP_MIN, O.OP_MAX):
+        assert np.array_equal(O.separable(op, src, np.uint16, 7, 9, 0.0), src)
+
+
+def test_negative_n_gives_converted_nan_or_centre():
+    src = random_volume((3, 4, 5), np.uint8, seed=9)
+    assert np.all(O.mean_separable_3d(src, np.uint8, 0, -3) == 0)
+    assert np.all(np.isnan(O.gaussian_blur_separable_3d(src, np.float32, 0, -1, 1.0)))
+    assert np.array_equal(O.minimum_separable_3d(src, np.uint8, 0, -5), src)
+
+
+def test_sigma_zero_gives_nan():
+    src = random_volume((3, 4, 5), np.uint8, seed=9)
+    assert np.all(np.isnan(O.gaussian_blur_separable_3d(src, np.float32, 2, 5, 0.0)))
+
+
+# ---- median: scipy's median filter, exactly ------------------------------------------
+
+@pytest.mark.parametrize("dt", [np.uint8, np.int8, np.uint16, np.int16, np.float32])
+@pytest.mark.parametrize("Ns", [(3, 3, 3), (5, 5, 5), (3, 1, 5), (1, 1, 1), (4, 3, 2)])
+def test_median_equals_scipy(dt, Ns):
+    src = random_volume((7, 9, 11), dt, seed=sum(Ns))
+    got = O.median_box_3d(src, dt, *Ns)
+    size = tuple(2 * ((n - 1) // 2) + 1 for n in Ns[::-1])
+    assert np.array_equal(got, ndimage.median_filter(src, size=size, mode="nearest"))
+
+
+def test_median_converts_to_dst_before_ranking():
+    src = random_volume((5, 6, 7), np.int16, seed=1)
+    got = O.median_box_3d(src, np.uint8, 3, 3, 3)
+    conv = np.clip(src, 0, 255).astype(np.uint8)
+    assert np.array_equal(got, ndimage.median_filter(conv, size=3, mode="nearest"))
+
+
+def test_median_rejects_empty_box():
+    with pytest.raises(ValueError):
+        O.median_box_3d(np.zeros((2, 2, 2), np.uint8), np.uint8, -1, 3, 3)
+
+
+# ---- the GPU's divide formula equals the IEEE divide -----------------------------------
+
+def test_div_uniform_equals_ieee_divide():
+    rng = np.random.default_rng(0)
+    L = O.lib()
+    bs = np.concatenate([np.arange(1, 200, dtype=np.float32),
+                         rng.uniform(1, 60, 200).astype(np.float32)])
+    for b in bs:
+        r = np.float32(1.0) / np.float32(b)
+        a_s = np.concatenate([rng.uniform(-70000, 70000, 40), rng.integers(-2**24, 2**24, 40),
+                              [0.0, -0.0, 1e-40, -3e-39, 1e38, float("inf"), float("nan")]])
+        for a in a_s.astype(np.float32):
+            got = np.float32(L.clko_div_uniform(float(a), float(b), float(r)))
+            want = np.float32(a) / np.float32(b)
+            assert (np.isnan(got) and np.isnan(want)) or got.tobytes() == want.tobytes(), (a, b)
