@@ -74,6 +74,64 @@ __device__ __forceinline__ void store_vec(T* __restrict__ p, const T (&in)[COLS]
   }
 }
 
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Streaming (non-temporal) store of N pixels: outputs are not read again by
+// the kernel, so they should not displace inputs from the caches.  `p` must
+// be aligned to the vector size.
+template <class T, int N>
+__device__ __forceinline__ void store_nt(T* __restrict__ p, const T (&in)[N]) {
+  constexpr int B = N * (int)sizeof(T);
+  if constexpr (B % 4 != 0 || (B > 8 && B % 16 != 0)) {
+    store_vec<T, N>(p, in);
+  } else {
+    PixVec<T, N> v;
+#pragma unroll
+    for (int i = 0; i < N; ++i) v.v[i] = in[i];
+    uint32_t w[B / 4];
+    __builtin_memcpy(w, &v, B);
+    if constexpr (B == 4) {
+      __builtin_nontemporal_store(w[0], reinterpret_cast<uint32_t*>(p));
+    } else if constexpr (B == 8) {
+      const u32x2 t = {w[0], w[1]};
+      __builtin_nontemporal_store(t, reinterpret_cast<u32x2*>(p));
+    } else {
+#pragma unroll
+      for (int k = 0; k < B / 16; ++k) {
+        const u32x4 t = {w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]};
+        __builtin_nontemporal_store(t, reinterpret_cast<u32x4*>(p) + k);
+      }
+    }
+  }
+}
+
+// Streaming load of N pixels (read once), same alignment rule.
+template <class T, int N>
+__device__ __forceinline__ PixVec<T, N> load_nt(const T* __restrict__ p) {
+  constexpr int B = N * (int)sizeof(T);
+  PixVec<T, N> v;
+  if constexpr (B % 4 != 0 || (B > 8 && B % 16 != 0)) {
+    v = *reinterpret_cast<const PixVec<T, N>*>(p);
+  } else {
+    uint32_t w[B / 4];
+    if constexpr (B == 4) {
+      w[0] = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(p));
+    } else if constexpr (B == 8) {
+      const u32x2 t = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(p));
+      w[0] = t.x; w[1] = t.y;
+    } else {
+#pragma unroll
+      for (int k = 0; k < B / 16; ++k) {
+        const u32x4 t = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p) + k);
+        w[4 * k] = t.x; w[4 * k + 1] = t.y; w[4 * k + 2] = t.z; w[4 * k + 3] = t.w;
+      }
+    }
+    __builtin_memcpy(&v, w, B);
+  }
+  return v;
+}
+
 __device__ __forceinline__ int64_t clamp64(int64_t v, int64_t lo, int64_t hi) {
   return v < lo ? lo : (v > hi ? hi : v);
 }

@@ -23,15 +23,18 @@ struct SepParams {
   int64_t L;
   int64_t outer;
   int64_t ncolgroups;  // strided pass: column groups of COLS per slab
+  int64_t ncols_total; // strided pass: outer * ncolgroups
   int64_t ntasks;      // x pass: rows * segments
+  uint32_t nblk;       // strided pass: blocks covering ncols_total
   int32_t nseg;        // segments along the axis (per column group / row)
   int32_t seg;         // outputs per segment along the axis (multiple of K)
   int32_t c;           // half width, >= 0
   int32_t ntaps;       // 2c+1
   float div;           // Gaussian: sum of weights; mean: tap count
   float rdiv;          // RN(1/div)
-  int32_t lds_stride;  // x pass: floats of LDS per wave
-  int32_t vec_ok;      // x pass: rows are aligned for vector stores
+  int32_t lds_stride;  // x pass: 4-byte words of LDS per wave
+  int32_t vec_ld;      // x pass: rows are aligned for vector loads
+  int32_t vec_st;      // x pass: rows are aligned for vector stores
   float w[kMaxTaps];   // Gaussian weights in tap order v = -c..c
 };
 

@@ -14,47 +14,90 @@
 //    The window is streamed through a ring of K registers: tap v of output q
 //    reads ring slot (q+v) mod K, and after tap v slot (v mod K) is refilled
 //    with input v+K.  The tap loop is unrolled by K so every ring index is a
-//    compile-time register; each input is loaded once per K outputs, and the
-//    refill is issued K taps before its use (a software pipeline).
+//    compile-time register, and each refill is issued K taps before its use
+//    (a software pipeline).  When the window fits the ring (N <= K) the ring
+//    is carried into the next block of K outputs, so along a segment every
+//    input is loaded exactly once.
 //  * y/z passes (and an out-of-range dim): lanes cover consecutive x
 //    columns, COLS columns per lane with one vector load, so every load of a
-//    wave is a contiguous run; the axis position is wave-uniform and its
-//    clamp is scalar work.
-//  * x pass: a wave stages its row segment (64*K outputs + 2c halo) in LDS
-//    once, converted to float, with one pad word every K elements so that the
-//    lane-strided ring reads are free of bank conflicts.
+//    wave is one contiguous run; the axis position is wave-uniform, and
+//    blocks of K outputs away from the volume border use unclamped
+//    addresses.
+//  * x pass: waves walk row segments of 512 outputs.  A segment is fetched
+//    with one vector load per lane (plus c halo values on each side) while
+//    the previous segment is being filtered, staged in LDS as 32-bit values
+//    with one pad word every 8 so that the lane-strided ring reads are free
+//    of bank conflicts.
+//  * min/max of integer pixels run on 32-bit integers (exact, and no NaN
+//    handling); Gaussian/mean divide by the launch-uniform normaliser with a
+//    reciprocal and one FMA correction, which is the IEEE quotient.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <stdlib.h>
+
+#include <mutex>
 #include <string>
+#include <type_traits>
+#include <map>
+#include <utility>
 
 #include "clk_device.h"
 #include "clk_internal.h"
 
 namespace clk {
 
-template <int OP, bool kFloatSrc>
-__device__ __forceinline__ float tap_apply(float acc, float x, float h) {
+template <int OP, class TS>
+using vt_t = std::conditional_t<((OP == OP_MIN || OP == OP_MAX) && Pix<TS>::kInt), int32_t, float>;
+
+template <bool B> struct BoolC { static constexpr bool value = B; };
+
+template <class TS, class VT, int COLS>
+__device__ __forceinline__ void load_vt(const TS* __restrict__ p, VT (&out)[COLS]) {
+  if constexpr (COLS == 1) {
+    out[0] = static_cast<VT>(p[0]);
+  } else {
+    const PixVec<TS, COLS> v = *reinterpret_cast<const PixVec<TS, COLS>*>(p);
+#pragma unroll
+    for (int i = 0; i < COLS; ++i) out[i] = static_cast<VT>(v.v[i]);
+  }
+}
+
+template <int OP, bool kFloatSrc, class VT>
+__device__ __forceinline__ VT tap_apply(VT acc, VT x, float h) {
   if constexpr (OP == OP_GAUSS) {
     return __builtin_fmaf(h, x, acc);
   } else if constexpr (OP == OP_MEAN) {
     return acc + x;
   } else if constexpr (OP == OP_MIN) {
-    // OpenCL min(x, y) = y < x ? y : x.  For integer sources fminf is the
-    // same value (no NaN, no signed zero).
+    // OpenCL min(x, y) = y < x ? y : x; on integers this is the plain min.
     if constexpr (kFloatSrc) return (x < acc) ? x : acc;
-    else return fminf(acc, x);
+    else return x < acc ? x : acc;
   } else {
     // OpenCL max(x, y) = x < y ? y : x.
     if constexpr (kFloatSrc) return (acc < x) ? x : acc;
-    else return fmaxf(acc, x);
+    else return acc < x ? x : acc;
   }
 }
 
-template <int OP, class TD>
-__device__ __forceinline__ TD finish(float acc, const SepParams& p) {
+// Saturating int32 -> pixel conversion for integer-valued min/max results.
+template <class TD>
+__device__ __forceinline__ TD clamp_int(int32_t v) {
+  if constexpr (!Pix<TD>::kInt) {
+    return static_cast<float>(v);
+  } else {
+    constexpr int32_t lo = static_cast<int32_t>(Pix<TD>::lo), hi = static_cast<int32_t>(Pix<TD>::hi);
+    return static_cast<TD>(v < lo ? lo : (v > hi ? hi : v));
+  }
+}
+
+template <int OP, class TS, class TD, class VT>
+__device__ __forceinline__ TD finish(VT acc, const SepParams& p) {
   if constexpr (OP == OP_GAUSS || OP == OP_MEAN) {
     return convert_sat<TD>(div_uniform(acc, p.div, p.rdiv));
+  } else if constexpr (std::is_same<VT, int32_t>::value) {
+    return clamp_int<TD>(acc);
   } else {
     return convert_sat<TD>(acc);
   }
@@ -63,41 +106,52 @@ __device__ __forceinline__ TD finish(float acc, const SepParams& p) {
 // ---- y / z (strided) pass ---------------------------------------------------
 template <int OP, class TS, class TD, int K, int COLS>
 __global__ __launch_bounds__(256) void sep_strided_kernel(const SepParams p) {
+  using VT = vt_t<OP, TS>;
   constexpr bool kFloatSrc = !Pix<TS>::kInt;
-  const uint32_t nblk = (uint32_t)((p.outer * p.ncolgroups + 255) / 256);
-  const uint32_t sid = blockIdx.x / nblk;
-  const uint32_t b = blockIdx.x - sid * nblk;
+  const uint32_t sid = blockIdx.x / p.nblk;
+  const uint32_t b = blockIdx.x - sid * p.nblk;
   const int64_t g = (int64_t)b * 256 + threadIdx.x;
-  if (g >= p.outer * p.ncolgroups) return;
+  if (g >= p.ncols_total) return;
   const int64_t o = g / p.ncolgroups;
   const int64_t cg = g - o * p.ncolgroups;
-  const int64_t inner = p.inner;
-  const int64_t base = o * (p.L * inner) + cg * COLS;
-  const TS* __restrict__ sp = static_cast<const TS*>(p.src) + base;
-  TD* __restrict__ dp = static_cast<TD*>(p.dst) + base;
+  const int64_t eoff = o * (p.L * p.inner) + cg * COLS;
+  const char* const ls = static_cast<const char*>(p.src) + eoff * (int64_t)sizeof(TS);
+  char* const ld = static_cast<char*>(p.dst) + eoff * (int64_t)sizeof(TD);
+  const int64_t ss = p.inner * (int64_t)sizeof(TS);
+  const int64_t ds = p.inner * (int64_t)sizeof(TD);
   const int64_t Lm1 = p.L - 1;
   const int c = p.c;
   const int ntaps = p.ntaps;
+  const bool carry = ntaps <= K;
   const int64_t a0 = (int64_t)sid * p.seg;
   const int64_t a1 = (a0 + p.seg < p.L) ? a0 + p.seg : p.L;
 
-  for (int64_t p0 = a0; p0 < a1; p0 += K) {
-    float win[K][COLS];
-    float acc[K][COLS];
-#pragma unroll
-    for (int t = 0; t < K; ++t)
-      load_widen<TS, COLS>(sp + clamp64(p0 - c + t, 0, Lm1) * inner, win[t]);
+  VT win[K][COLS];
+  VT acc[K][COLS];
+  auto ldc = [&](int64_t pos, VT (&dst)[COLS]) {
+    load_vt<TS, VT, COLS>(reinterpret_cast<const TS*>(ls + clamp64(pos, 0, Lm1) * ss), dst);
+  };
+
+  // One block of K outputs starting at p0.  Loads in it read positions
+  // p0-c .. p0+c+K; CLAMP=false when all of them are inside the axis.
+  auto block = [&](int64_t p0, auto clampTag) {
+    constexpr bool CLAMP = decltype(clampTag)::value;
+    int64_t nxt = p0 - c + K;  // position of the next ring refill
+    auto refill = [&](VT (&dst)[COLS]) {
+      if constexpr (CLAMP) ldc(nxt, dst);
+      else load_vt<TS, VT, COLS>(reinterpret_cast<const TS*>(ls + nxt * ss), dst);
+      ++nxt;
+    };
 #pragma unroll
     for (int q = 0; q < K; ++q) {
       if constexpr (OP == OP_GAUSS || OP == OP_MEAN) {
 #pragma unroll
         for (int j = 0; j < COLS; ++j) acc[q][j] = 0.f;
       } else if constexpr (kFloatSrc) {
-        // centre value first, exactly as the reference initialises res
-        load_widen<TS, COLS>(sp + clamp64(p0 + q, 0, Lm1) * inner, acc[q]);
+        ldc(p0 + q, acc[q]);  // centre first, as the reference initialises res
       } else {
 #pragma unroll
-        for (int j = 0; j < COLS; ++j) acc[q][j] = win[q][j];
+        for (int j = 0; j < COLS; ++j) acc[q][j] = win[q][j];  // any window value (exact)
       }
     }
     int v = 0;
@@ -109,8 +163,8 @@ __global__ __launch_bounds__(256) void sep_strided_kernel(const SepParams p) {
         for (int q = 0; q < K; ++q)
 #pragma unroll
           for (int j = 0; j < COLS; ++j)
-            acc[q][j] = tap_apply<OP, kFloatSrc>(acc[q][j], win[(q + r) % K][j], h);
-        load_widen<TS, COLS>(sp + clamp64(p0 - c + v + r + K, 0, Lm1) * inner, win[r]);
+            acc[q][j] = tap_apply<OP, kFloatSrc, VT>(acc[q][j], win[(q + r) % K][j], h);
+        refill(win[r]);
       }
     }
 #pragma unroll
@@ -121,9 +175,8 @@ __global__ __launch_bounds__(256) void sep_strided_kernel(const SepParams p) {
         for (int q = 0; q < K; ++q)
 #pragma unroll
           for (int j = 0; j < COLS; ++j)
-            acc[q][j] = tap_apply<OP, kFloatSrc>(acc[q][j], win[(q + r) % K][j], h);
-        if (v + r + 1 < ntaps)
-          load_widen<TS, COLS>(sp + clamp64(p0 - c + v + r + K, 0, Lm1) * inner, win[r]);
+            acc[q][j] = tap_apply<OP, kFloatSrc, VT>(acc[q][j], win[(q + r) % K][j], h);
+        refill(win[r]);
       }
     }
 #pragma unroll
@@ -131,141 +184,234 @@ __global__ __launch_bounds__(256) void sep_strided_kernel(const SepParams p) {
       if (p0 + q < a1) {
         TD out[COLS];
 #pragma unroll
-        for (int j = 0; j < COLS; ++j) out[j] = finish<OP, TD>(acc[q][j], p);
-        store_vec<TD, COLS>(dp + (p0 + q) * inner, out);
+        for (int j = 0; j < COLS; ++j) out[j] = finish<OP, TS, TD, VT>(acc[q][j], p);
+        store_nt<TD, COLS>(reinterpret_cast<TD*>(ld + (p0 + q) * ds), out);
+      }
+    }
+  };
+
+#pragma unroll
+  for (int t = 0; t < K; ++t) ldc(a0 - c + t, win[t]);
+  for (int64_t p0 = a0; p0 < a1; p0 += K) {
+    if (p0 - c >= 0 && p0 + c + K <= Lm1) block(p0, BoolC<false>{});
+    else block(p0, BoolC<true>{});
+    const int64_t pn = p0 + K;
+    if (pn < a1) {
+      // After the taps slot s holds input s+K (s < ntaps); with a carried
+      // ring only the slots the taps did not refill need loading.
+#pragma unroll
+      for (int s = 0; s < K; ++s)
+        if (!carry || s >= ntaps) ldc(pn - c + s, win[s]);
+    }
+  }
+}
+
+// ---- x pass (contiguous axis, LDS-staged row segments) ----------------------
+constexpr int kXK = 8;            // outputs per lane per sub-segment
+constexpr int kXSeg = 64 * kXK;   // outputs per sub-segment (one wave-wide vector load)
+constexpr int kMaxCX = 1024;      // larger half widths along x use the strided kernel
+
+__host__ __device__ __forceinline__ int lds_pad(int e) { return e + e / kXK; }
+
+// One wave filters SUB*512 consecutive outputs of one row: SUB vector loads
+// per lane (all in flight together) plus c halo pixels per side, staged in
+// LDS, then SUB blocks of 8 outputs per lane through the register ring.  The
+// grid is one wave per task, in address order (no persistent loop).
+template <int OP, class TS, class TD, int SUB>
+__global__ __launch_bounds__(256) void sep_x_kernel(const SepParams p) {
+  using VT = vt_t<OP, TS>;
+  constexpr bool kFloatSrc = !Pix<TS>::kInt;
+  constexpr int K = kXK;
+  constexpr int TSEG = kXSeg * SUB;
+  extern __shared__ uint32_t lds_raw[];
+  const int wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  VT* const buf = reinterpret_cast<VT*>(lds_raw) + wv * p.lds_stride;
+  const int64_t W = p.L;
+  const int c = p.c;
+  const int ntaps = p.ntaps;
+  const uint32_t task = blockIdx.x * 4u + (uint32_t)wv;
+  const bool valid = task < (uint32_t)p.ntasks;
+  const uint32_t row = task / (uint32_t)p.nseg;
+  const int64_t x0 = (int64_t)(task - row * (uint32_t)p.nseg) * TSEG;
+  const TS* __restrict__ rp = static_cast<const TS*>(p.src) + (int64_t)row * W;
+
+  if (valid) {
+    PixVec<TS, K> v[SUB];
+#pragma unroll
+    for (int m = 0; m < SUB; ++m) {
+      const int64_t xi = x0 + kXSeg * m + K * lane;
+      if (p.vec_ld && xi + K <= W) {
+        v[m] = load_nt<TS, K>(rp + xi);
+      } else {
+#pragma unroll
+        for (int j = 0; j < K; ++j) v[m].v[j] = rp[xi + j < W ? xi + j : W - 1];
+      }
+    }
+    for (int e = lane; e < c; e += 64) {
+      buf[lds_pad(e)] = static_cast<VT>(rp[clamp64(x0 - c + e, 0, W - 1)]);
+      const int64_t xr = x0 + TSEG + e;
+      buf[lds_pad(c + TSEG + e)] = static_cast<VT>(rp[xr < W ? xr : W - 1]);
+    }
+#pragma unroll
+    for (int m = 0; m < SUB; ++m)
+#pragma unroll
+      for (int j = 0; j < K; ++j)
+        buf[lds_pad(c + kXSeg * m + K * lane + j)] = static_cast<VT>(v[m].v[j]);
+  }
+  __syncthreads();
+  if (!valid) return;
+#pragma unroll
+  for (int m = 0; m < SUB; ++m) {
+    // lds_pad(512*m + K*lane + t) = 576*m + (K+1)*lane + lds_pad(t)
+    const VT* lb = buf + (kXSeg + kXSeg / K) * m + (K + 1) * lane;
+    VT win[K];
+    VT acc[K];
+#pragma unroll
+    for (int t = 0; t < K; ++t) win[t] = lb[lds_pad(t)];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      if constexpr (OP == OP_GAUSS || OP == OP_MEAN) acc[q] = 0.f;
+      else if constexpr (kFloatSrc) acc[q] = lb[lds_pad(q + c)];
+      else acc[q] = win[q];
+    }
+    int vv = 0;
+    for (; vv + K <= ntaps; vv += K) {
+      const VT* lv = lb + (K + 1) * (vv / K);  // lds_pad(vv + r + K) = (K+1)*(vv/K) + r + K + 1
+#pragma unroll
+      for (int r = 0; r < K; ++r) {
+        const float h = (OP == OP_GAUSS) ? p.w[vv + r] : 0.f;
+#pragma unroll
+        for (int q = 0; q < K; ++q) acc[q] = tap_apply<OP, kFloatSrc, VT>(acc[q], win[(q + r) % K], h);
+        win[r] = lv[r + K + 1];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < K - 1; ++r) {
+      if (vv + r < ntaps) {
+        const float h = (OP == OP_GAUSS) ? p.w[vv + r] : 0.f;
+#pragma unroll
+        for (int q = 0; q < K; ++q) acc[q] = tap_apply<OP, kFloatSrc, VT>(acc[q], win[(q + r) % K], h);
+        if (vv + r + 1 < ntaps) win[r] = lb[lds_pad(vv + r + K)];
+      }
+    }
+    const int64_t xs = x0 + kXSeg * m + (int64_t)K * lane;
+    if (xs < W) {
+      TD* __restrict__ dp = static_cast<TD*>(p.dst) + (int64_t)row * W + xs;
+      TD out[K];
+#pragma unroll
+      for (int q = 0; q < K; ++q) out[q] = finish<OP, TS, TD, VT>(acc[q], p);
+      if (p.vec_st && xs + K <= W) {
+        store_nt<TD, K>(dp, out);
+      } else {
+#pragma unroll
+        for (int q = 0; q < K; ++q)
+          if (xs + q < W) dp[q] = out[q];
       }
     }
   }
 }
 
-// ---- x pass (contiguous axis, LDS-staged rows) ------------------------------
-template <int K>
-__host__ __device__ __forceinline__ int lds_pad(int e) { return e + e / K; }
+// ---- host-side launch of one (op, src type, dst type) ----------------------
 
-template <int OP, class TS, class TD, int K>
-__global__ __launch_bounds__(256) void sep_x_kernel(const SepParams p) {
-  constexpr bool kFloatSrc = !Pix<TS>::kInt;
-  extern __shared__ float lds[];
-  const int wv = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
-  const int64_t task = (int64_t)blockIdx.x * 4 + wv;
-  const bool active = task < p.ntasks;
-  const int64_t row = task / p.nseg;
-  const int64_t seg = task - row * p.nseg;
-  const int64_t W = p.L;
-  const int c = p.c;
-  const int ntaps = p.ntaps;
-  const int span = 64 * K + 2 * c;
-  float* buf = lds + wv * p.lds_stride;
-  const int64_t x0 = seg * (64 * K);
-  if (active) {
-    const TS* __restrict__ rp = static_cast<const TS*>(p.src) + row * W;
-    for (int e = lane; e < span; e += 64) {
-      const int64_t x = clamp64(x0 - c + e, 0, W - 1);
-      buf[lds_pad<K>(e)] = static_cast<float>(rp[x]);
-    }
+// Resident blocks of a kernel on the current device, cached per kernel.
+inline int resident_blocks(const void* kernel, size_t shmem) {
+  static std::mutex mu;
+  static std::map<std::pair<const void*, size_t>, int> cache;
+  static int cus = 0;
+  std::lock_guard<std::mutex> lk(mu);
+  if (cus == 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
   }
-  __syncthreads();
-  if (!active) return;
-  const float* lb = buf + (K + 1) * lane;  // lds_pad(K*lane + t) = (K+1)*lane + lds_pad(t)
-
-  float win[K];
-  float acc[K];
-#pragma unroll
-  for (int t = 0; t < K; ++t) win[t] = lb[lds_pad<K>(t)];
-#pragma unroll
-  for (int q = 0; q < K; ++q) {
-    if constexpr (OP == OP_GAUSS || OP == OP_MEAN) acc[q] = 0.f;
-    else if constexpr (kFloatSrc) acc[q] = lb[lds_pad<K>(q + c)];
-    else acc[q] = win[q];
-  }
-  int v = 0;
-  for (; v + K <= ntaps; v += K) {
-    const float* lv = lb + (K + 1) * (v / K);  // lds_pad(v + r + K) = (K+1)*(v/K) + r + K + 1
-#pragma unroll
-    for (int r = 0; r < K; ++r) {
-      const float h = (OP == OP_GAUSS) ? p.w[v + r] : 0.f;
-#pragma unroll
-      for (int q = 0; q < K; ++q) acc[q] = tap_apply<OP, kFloatSrc>(acc[q], win[(q + r) % K], h);
-      win[r] = lv[r + K + 1];
-    }
-  }
-#pragma unroll
-  for (int r = 0; r < K - 1; ++r) {
-    if (v + r < ntaps) {
-      const float h = (OP == OP_GAUSS) ? p.w[v + r] : 0.f;
-#pragma unroll
-      for (int q = 0; q < K; ++q) acc[q] = tap_apply<OP, kFloatSrc>(acc[q], win[(q + r) % K], h);
-      if (v + r + 1 < ntaps) win[r] = lb[lds_pad<K>(v + r + K)];
-    }
-  }
-  const int64_t xs = x0 + (int64_t)K * lane;
-  TD* __restrict__ dp = static_cast<TD*>(p.dst) + row * W + xs;
-  TD out[K];
-#pragma unroll
-  for (int q = 0; q < K; ++q) out[q] = finish<OP, TD>(acc[q], p);
-  if (p.vec_ok && xs + K <= W) {
-    store_vec<TD, K>(dp, out);
-  } else {
-#pragma unroll
-    for (int q = 0; q < K; ++q)
-      if (xs + q < W) dp[q] = out[q];
-  }
+  const auto key = std::make_pair(kernel, shmem);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int bpc = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kernel, 256, shmem) != hipSuccess || bpc <= 0)
+    bpc = 1;
+  cache[key] = bpc * cus;
+  return bpc * cus;
 }
 
-// ---- host-side launch of one (op, src type, dst type) ----------------------
-constexpr int kStridedK = 8;
-constexpr int kXK = 8;
-constexpr int kMaxCX = 1024;  // larger half widths along x use the strided kernel
+// Blocks of 256 threads per resident slot that a launch aims for; a few
+// rounds of equal-sized blocks keep every CU busy to the end.  CLK_ROUNDS
+// and CLK_STRIDED_VARIANT override the defaults (used when tuning).
+inline int env_int(const char* name, int dflt) {
+  const char* v = getenv(name);
+  return (v && *v) ? atoi(v) : dflt;
+}
+inline int strided_rounds() { static const int r = env_int("CLK_ROUNDS", 2); return r > 0 ? r : 2; }
+inline int strided_variant() { static const int v = env_int("CLK_STRIDED_VARIANT", 0); return v; }
+
+template <int OP, class TS, class TD, int K, int COLS>
+int32_t launch_strided(hipStream_t st, SepParams& p, std::string* err) {
+  p.ncolgroups = p.inner / COLS;
+  p.ncols_total = p.outer * p.ncolgroups;
+  const int64_t nblk = (p.ncols_total + 255) / 256;
+  const auto kern = sep_strided_kernel<OP, TS, TD, K, COLS>;
+  const int64_t target = (int64_t)resident_blocks((const void*)kern, 0) * strided_rounds();
+  int64_t nseg = (target + nblk - 1) / nblk;
+  const int64_t maxseg = (p.L + K - 1) / K;
+  if (nseg > maxseg) nseg = maxseg;
+  if (nseg < 1) nseg = 1;
+  int64_t seg = (p.L + nseg - 1) / nseg;
+  seg = (seg + K - 1) / K * K;
+  nseg = (p.L + seg - 1) / seg;
+  const int64_t blocks = nblk * nseg;
+  if (blocks > 0x7fffffffLL) {
+    *err = "volume too large for the strided pass grid";
+    return CLK_INVALID_VALUE;
+  }
+  p.seg = (int32_t)seg;
+  p.nseg = (int32_t)nseg;
+  p.nblk = (uint32_t)nblk;
+  hipLaunchKernelGGL(kern, dim3((uint32_t)blocks), dim3(256), 0, st, p);
+  return 0;
+}
 
 template <int OP, class TS, class TD>
 int32_t launch_sep_typed(hipStream_t st, SepParams& p, int axis_is_x, std::string* err) {
   const int64_t V = p.outer * p.L * p.inner;
   if (V == 0) return 0;
+  int32_t rc = 0;
   if (axis_is_x && p.c <= kMaxCX) {
-    constexpr int K = kXK;
     const int64_t W = p.L;
-    const int64_t rows = p.outer;
-    p.nseg = (int32_t)((W + 64 * K - 1) / (64 * K));
-    p.ntasks = rows * p.nseg;
-    const int span = 64 * K + 2 * p.c;
-    // one spare word: the last ring refill of the last lane reads one
-    // element past the window (its value is never used)
-    p.lds_stride = lds_pad<K>(span) + 1;
-    p.vec_ok = (W % K == 0) && ((uintptr_t)p.dst % (K * sizeof(TD)) == 0);
-    const int64_t blocks = (p.ntasks + 3) / 4;
-    const size_t shmem = (size_t)4 * p.lds_stride * sizeof(float);
-    if (blocks > 0x7fffffffLL) { *err = "volume too large for the x pass grid"; return CLK_INVALID_VALUE; }
-    hipLaunchKernelGGL((sep_x_kernel<OP, TS, TD, K>), dim3((uint32_t)blocks), dim3(256), shmem, st, p);
-  } else {
-    constexpr int K = kStridedK;
-    const bool vec4 = (p.inner % 4 == 0) && ((uintptr_t)p.src % (4 * sizeof(TS)) == 0) &&
-                      ((uintptr_t)p.dst % (4 * sizeof(TD)) == 0);
-    const int cols = vec4 ? 4 : 1;
-    p.ncolgroups = p.inner / cols;
-    const int64_t threads_axis = p.outer * p.ncolgroups;
-    // enough segments along the axis to give the chip ~1M threads
-    const int64_t target = int64_t(1) << 20;
-    int64_t nseg = (target + threads_axis - 1) / threads_axis;
-    const int64_t maxseg = (p.L + K - 1) / K;
-    if (nseg > maxseg) nseg = maxseg;
-    if (nseg < 1) nseg = 1;
-    int64_t seg = (p.L + nseg - 1) / nseg;
-    seg = (seg + K - 1) / K * K;
-    nseg = (p.L + seg - 1) / seg;
-    p.seg = (int32_t)seg;
-    p.nseg = (int32_t)nseg;
-    const int64_t nblk = (threads_axis + 255) / 256;
-    const int64_t blocks = nblk * nseg;
-    if (blocks > 0x7fffffffLL || nblk > 0xffffffffLL) {
-      *err = "volume too large for the strided pass grid";
+    // larger tasks for long rows and narrow halos (more loads in flight per
+    // wave, less per-task work); CLK_XSUB overrides when tuning
+    static const int sub_env = env_int("CLK_XSUB", 0);
+    int sub = sub_env;
+    if (sub != 1 && sub != 2 && sub != 4) sub = (W >= 4 * kXSeg && p.c <= 256) ? 2 : 1;
+    const int tseg = kXSeg * sub;
+    p.nseg = (int32_t)((W + tseg - 1) / tseg);
+    p.ntasks = p.outer * p.nseg;
+    if (p.ntasks > 0x7ffffff0LL) {
+      *err = "volume too large for the x pass grid";
       return CLK_INVALID_VALUE;
     }
-    if (vec4)
-      hipLaunchKernelGGL((sep_strided_kernel<OP, TS, TD, K, 4>), dim3((uint32_t)blocks), dim3(256), 0, st, p);
-    else
-      hipLaunchKernelGGL((sep_strided_kernel<OP, TS, TD, K, 1>), dim3((uint32_t)blocks), dim3(256), 0, st, p);
+    p.lds_stride = lds_pad(tseg + 2 * p.c) + 1;  // +1: the last ring refill reads one past
+    p.vec_ld = (W % kXK == 0) && ((uintptr_t)p.src % (kXK * sizeof(TS)) == 0);
+    p.vec_st = (W % kXK == 0) && ((uintptr_t)p.dst % (kXK * sizeof(TD)) == 0);
+    const size_t shmem = (size_t)4 * p.lds_stride * 4;
+    const uint32_t blocks = (uint32_t)((p.ntasks + 3) / 4);
+    if (sub == 4) hipLaunchKernelGGL((sep_x_kernel<OP, TS, TD, 4>), dim3(blocks), dim3(256), shmem, st, p);
+    else if (sub == 2) hipLaunchKernelGGL((sep_x_kernel<OP, TS, TD, 2>), dim3(blocks), dim3(256), shmem, st, p);
+    else hipLaunchKernelGGL((sep_x_kernel<OP, TS, TD, 1>), dim3(blocks), dim3(256), shmem, st, p);
+  } else {
+    // vector columns when the inner extent and both pointers allow it
+    const bool v4 = (p.inner % 4 == 0) && ((uintptr_t)p.src % (4 * sizeof(TS)) == 0) &&
+                    ((uintptr_t)p.dst % (4 * sizeof(TD)) == 0);
+    const bool v2 = (p.inner % 2 == 0) && ((uintptr_t)p.src % (2 * sizeof(TS)) == 0) &&
+                    ((uintptr_t)p.dst % (2 * sizeof(TD)) == 0);
+    int var = strided_variant();
+    if (var == 0) var = (p.ntaps <= 8) ? 1 : 2;
+    if (var == 2 && v2) rc = launch_strided<OP, TS, TD, 16, 2>(st, p, err);
+    else if (v4) rc = launch_strided<OP, TS, TD, 8, 4>(st, p, err);
+    else rc = launch_strided<OP, TS, TD, 8, 1>(st, p, err);
   }
+  if (rc) return rc;
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     *err = std::string("kernel launch failed: ") + hipGetErrorString(e);
