@@ -35,28 +35,79 @@ CLK_USHORT, CLK_FLOAT, CLK_UCHAR = 3, 4, 1
 
 
 def make_c2_volume(torch, D, H, W, seed=2, nblobs=500, amp=2000.0, sigma=4.0, lam=100.0):
-    """uint16 volume in int16 storage: Poisson(lam) + Gaussian blobs, rounded
-    and saturated.  Generated on the GPU from a seeded generator."""
+    """uint16 volume in int16 storage: Poisson(lam) + Gaussian blobs (each
+    evaluated within 4 sigma of its centre), rounded and saturated.
+    Generated on the GPU from a seeded generator with a few batched kernels."""
     g = torch.Generator(device="cuda").manual_seed(seed)
     vol = torch.poisson(torch.full((D, H, W), lam, device="cuda"), generator=g)
     cen = torch.rand((nblobs, 3), generator=g, device="cuda") * torch.tensor(
         [D, H, W], device="cuda", dtype=torch.float32)
-    cen = cen.cpu().numpy()
     r = int(4 * sigma)
-    for cz, cy, cx in cen:
-        z0, z1 = max(0, int(cz) - r), min(D, int(cz) + r + 1)
-        y0, y1 = max(0, int(cy) - r), min(H, int(cy) + r + 1)
-        x0, x1 = max(0, int(cx) - r), min(W, int(cx) + r + 1)
-        zz = torch.arange(z0, z1, device="cuda", dtype=torch.float32)[:, None, None] - float(cz)
-        yy = torch.arange(y0, y1, device="cuda", dtype=torch.float32)[None, :, None] - float(cy)
-        xx = torch.arange(x0, x1, device="cuda", dtype=torch.float32)[None, None, :] - float(cx)
-        vol[z0:z1, y0:y1, x0:x1] += amp * torch.exp(-(zz * zz + yy * yy + xx * xx) / (2 * sigma * sigma))
+    off = torch.arange(-r, r + 1, device="cuda")
+    base = cen.floor().long()                                    # (B, 3)
+    zz = base[:, 0, None] + off[None, :]                         # (B, S)
+    yy = base[:, 1, None] + off[None, :]
+    xx = base[:, 2, None] + off[None, :]
+    dz = (zz.float() - cen[:, 0, None]) ** 2
+    dy = (yy.float() - cen[:, 1, None]) ** 2
+    dx = (xx.float() - cen[:, 2, None]) ** 2
+    val = amp * torch.exp(-(dz[:, :, None, None] + dy[:, None, :, None] + dx[:, None, None, :])
+                          / (2 * sigma * sigma))                 # (B, S, S, S)
+    ok = ((zz >= 0) & (zz < D))[:, :, None, None] & ((yy >= 0) & (yy < H))[:, None, :, None] \
+        & ((xx >= 0) & (xx < W))[:, None, None, :]
+    flat = (zz.clamp(0, D - 1)[:, :, None, None] * H + yy.clamp(0, H - 1)[:, None, :, None]) * W \
+        + xx.clamp(0, W - 1)[:, None, None, :]
+    vol.view(-1).index_put_((flat[ok],), val[ok], accumulate=True)
+    del val, ok, flat
     vol = vol.round_().clamp_(0, 65535)
     assert float(vol.max()) < 32768
     out = vol.to(torch.int16)
     del vol
     return out
 
+
+def rocprof_summary(d):
+    """Reduce rocprofv3 CSV output under directory d (counter collections
+    and/or kernel stats) to a per-kernel markdown table for this library's
+    kernels and the device copy kernel."""
+    import csv
+    import glob
+    import re
+    rows = {}
+    def short(name):
+        n = name.split("(")[0].replace("void ", "")
+        return re.sub(r"\s+", " ", n)[:110]
+    for path in glob.glob(os.path.join(d, "**", "*_counter_collection.csv"), recursive=True):
+        per = {}
+        for r in csv.DictReader(open(path)):
+            k = r["Kernel_Name"]
+            if "clk::" not in k and "copy" not in k.lower():
+                continue
+            key = (short(k), r["Dispatch_Id"])
+            per.setdefault(key, {})
+            per[key][r["Counter_Name"]] = per[key].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+        for (k, _), cv in per.items():
+            acc = rows.setdefault(k, {})
+            for cn, v in cv.items():
+                acc.setdefault(cn, []).append(v)
+    for path in glob.glob(os.path.join(d, "**", "*_kernel_stats.csv"), recursive=True):
+        for r in csv.DictReader(open(path)):
+            k = r["Name"]
+            if "clk::" not in k and "copy" not in k.lower():
+                continue
+            acc = rows.setdefault(short(k), {})
+            acc["calls"] = [float(r["Calls"])]
+            acc["avg_us"] = [float(r["AverageNs"]) / 1e3]
+    cols = sorted({c for v in rows.values() for c in v})
+    print("| kernel | " + " | ".join(cols) + " |")
+    print("|---|" + "---|" * len(cols))
+    for k in sorted(rows):
+        v = rows[k]
+        cells = []
+        for c in cols:
+            x = v.get(c)
+            cells.append("" if not x else "%.4g" % (sum(x) / len(x)))
+        print("| %s | %s |" % (k, " | ".join(cells)))
 
 def suite_c2():
     """(label, kernel, dim, N, s, src kind, dst kind) of one step."""
@@ -101,7 +152,14 @@ def main():
     ap.add_argument("--size", default="1024x1024x512", help="WxHxD")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", default=None, help="run only launches whose label contains this")
+    ap.add_argument("--rocprof-summary", default=None, metavar="DIR",
+                    help="print a per-kernel table of the rocprofv3 CSV files under DIR and exit")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 1 GiB device copy (the reference point for rocprof byte counters)")
     args = ap.parse_args()
+    if args.rocprof_summary:
+        rocprof_summary(args.rocprof_summary)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the filters shard by z-slab or by volume, and this "
               "build has no multi-GPU path (see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
@@ -149,6 +207,13 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    if args.copy_calibration:
+        ca = torch.empty(1 << 29, dtype=torch.int16, device="cuda").fill_(1)
+        cb = torch.empty_like(ca)
+        for _ in range(3):
+            cb.copy_(ca)
+        torch.cuda.synchronize()
+        del ca, cb
 
     # timed region: exactly K steps between synchronisations
     t_start = torch.cuda.Event(enable_timing=True)

@@ -47,6 +47,15 @@ __device__ __forceinline__ float div_uniform(float a, float b, float r) {
   return q1;
 }
 
+// The same quotient without the range guard, for launches whose numerators
+// are known to be 0 or within [2^-96, 2^120] in magnitude (sums of integer
+// pixels with weights not below 2^-48).
+__device__ __forceinline__ float div_uniform_fast(float a, float b, float r) {
+  const float q = a * r;
+  const float e = __builtin_fmaf(-q, b, a);
+  return __builtin_fmaf(e, r, q);
+}
+
 // Load COLS consecutive pixels of type T, widened to float.  `p` must be
 // aligned to sizeof(T)*COLS when COLS > 1.
 template <class T, int COLS> struct alignas(sizeof(T) * COLS) PixVec { T v[COLS]; };

@@ -35,6 +35,7 @@ struct SepParams {
   int32_t lds_stride;  // x pass: 4-byte words of LDS per wave
   int32_t vec_ld;      // x pass: rows are aligned for vector loads
   int32_t vec_st;      // x pass: rows are aligned for vector stores
+  int32_t fastdiv;     // numerators never need the guarded divide
   float w[kMaxTaps];   // Gaussian weights in tap order v = -c..c
 };
 

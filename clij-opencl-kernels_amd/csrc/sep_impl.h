@@ -52,6 +52,7 @@ template <int OP, class TS>
 using vt_t = std::conditional_t<((OP == OP_MIN || OP == OP_MAX) && Pix<TS>::kInt), int32_t, float>;
 
 template <bool B> struct BoolC { static constexpr bool value = B; };
+template <int I> struct IntC { static constexpr int value = I; };
 
 template <class TS, class VT, int COLS>
 __device__ __forceinline__ void load_vt(const TS* __restrict__ p, VT (&out)[COLS]) {
@@ -95,7 +96,8 @@ __device__ __forceinline__ TD clamp_int(int32_t v) {
 template <int OP, class TS, class TD, class VT>
 __device__ __forceinline__ TD finish(VT acc, const SepParams& p) {
   if constexpr (OP == OP_GAUSS || OP == OP_MEAN) {
-    return convert_sat<TD>(div_uniform(acc, p.div, p.rdiv));
+    return convert_sat<TD>(p.fastdiv ? div_uniform_fast(acc, p.div, p.rdiv)
+                                     : div_uniform(acc, p.div, p.rdiv));
   } else if constexpr (std::is_same<VT, int32_t>::value) {
     return clamp_int<TD>(acc);
   } else {
@@ -212,6 +214,10 @@ constexpr int kXSeg = 64 * kXK;   // outputs per sub-segment (one wave-wide vect
 constexpr int kMaxCX = 1024;      // larger half widths along x use the strided kernel
 
 __host__ __device__ __forceinline__ int lds_pad(int e) { return e + e / kXK; }
+}  // namespace clk
+#include "sep_xreg.h"
+#include "sep_ystatic.h"
+namespace clk {
 
 // One wave filters SUB*512 consecutive outputs of one row: SUB vector loads
 // per lane (all in flight together) plus c halo pixels per side, staged in
@@ -344,6 +350,7 @@ inline int env_int(const char* name, int dflt) {
   return (v && *v) ? atoi(v) : dflt;
 }
 inline int strided_rounds() { static const int r = env_int("CLK_ROUNDS", 2); return r > 0 ? r : 2; }
+inline bool no_xreg() { static const bool v = env_int("CLK_NO_XREG", 0) != 0; return v; }
 inline int strided_variant() { static const int v = env_int("CLK_STRIDED_VARIANT", 0); return v; }
 
 template <int OP, class TS, class TD, int K, int COLS>
@@ -372,12 +379,116 @@ int32_t launch_strided(hipStream_t st, SepParams& p, std::string* err) {
   return 0;
 }
 
+// Segments along the axis for a strided launch of nblk column blocks.
+template <int K>
+inline void plan_segments(SepParams& p, int64_t nblk, int64_t target_blocks) {
+  int64_t nseg = (target_blocks + nblk - 1) / nblk;
+  const int64_t maxseg = (p.L + K - 1) / K;
+  if (nseg > maxseg) nseg = maxseg;
+  if (nseg < 1) nseg = 1;
+  int64_t seg = (p.L + nseg - 1) / nseg;
+  seg = (seg + K - 1) / K * K;
+  nseg = (p.L + seg - 1) / seg;
+  p.seg = (int32_t)seg;
+  p.nseg = (int32_t)nseg;
+  p.nblk = (uint32_t)nblk;
+}
+
+template <int OP, class TS, class TD, int C, int COLS>
+int32_t launch_ystatic_cc(hipStream_t st, SepParams& p, std::string* err) {
+  p.ncolgroups = p.inner / COLS;
+  p.ncols_total = p.outer * p.ncolgroups;
+  const int64_t nblk = (p.ncols_total + 255) / 256;
+  const auto kern = sep_ystatic_kernel<OP, TS, TD, C, COLS>;
+  plan_segments<8>(p, nblk, (int64_t)resident_blocks((const void*)kern, 0) * strided_rounds());
+  const int64_t blocks = nblk * p.nseg;
+  if (blocks > 0x7fffffffLL) {
+    *err = "volume too large for the strided pass grid";
+    return CLK_INVALID_VALUE;
+  }
+  hipLaunchKernelGGL(kern, dim3((uint32_t)blocks), dim3(256), 0, st, p);
+  return 0;
+}
+
+// Wider column groups while the window registers stay within ~112.
+template <int OP, class TS, class TD, int C>
+int32_t launch_ystatic_c(hipStream_t st, SepParams& p, bool v4, std::string* err) {
+  constexpr int WN = 8 + 2 * C;
+  if constexpr ((WN + 8) * 4 <= 112) {
+    if (v4) return launch_ystatic_cc<OP, TS, TD, C, 4>(st, p, err);
+  }
+  if ((p.inner % 2) == 0) return launch_ystatic_cc<OP, TS, TD, C, 2>(st, p, err);
+  return launch_ystatic_cc<OP, TS, TD, C, 1>(st, p, err);
+}
+
+template <int OP, class TS, class TD>
+int32_t launch_ystatic(hipStream_t st, SepParams& p, bool v4, std::string* err) {
+  if constexpr (std::is_same<TS, TD>::value || std::is_same<TD, float>::value) {
+    switch (p.c) {
+      case 1: return launch_ystatic_c<OP, TS, TD, 1>(st, p, v4, err);
+      case 2: return launch_ystatic_c<OP, TS, TD, 2>(st, p, v4, err);
+      case 3: return launch_ystatic_c<OP, TS, TD, 3>(st, p, v4, err);
+      case 4: return launch_ystatic_c<OP, TS, TD, 4>(st, p, v4, err);
+      case 5: return launch_ystatic_c<OP, TS, TD, 5>(st, p, v4, err);
+      case 6: return launch_ystatic_c<OP, TS, TD, 6>(st, p, v4, err);
+      case 7: return launch_ystatic_c<OP, TS, TD, 7>(st, p, v4, err);
+      case 8: return launch_ystatic_c<OP, TS, TD, 8>(st, p, v4, err);
+    }
+    if constexpr (OP == OP_GAUSS) {
+      switch (p.c) {
+        case 9: return launch_ystatic_c<OP, TS, TD, 9>(st, p, v4, err);
+        case 10: return launch_ystatic_c<OP, TS, TD, 10>(st, p, v4, err);
+        case 11: return launch_ystatic_c<OP, TS, TD, 11>(st, p, v4, err);
+        case 12: return launch_ystatic_c<OP, TS, TD, 12>(st, p, v4, err);
+      }
+    }
+  }
+  *err = "internal: no static-width strided kernel for this half width";
+  return CLK_INVALID_VALUE;
+}
+
+template <int OP, class TS, class TD, int C>
+void launch_xreg_c(hipStream_t st, const SepParams& p) {
+  const uint32_t blocks = (uint32_t)((p.ntasks + 3) / 4);
+  hipLaunchKernelGGL((sep_xreg_kernel<OP, TS, TD, C>), dim3(blocks), dim3(256), 0, st, p);
+}
+
+template <int OP, class TS, class TD>
+int32_t launch_xreg(hipStream_t st, const SepParams& p, std::string* err) {
+  if constexpr (std::is_same<TS, TD>::value || std::is_same<TD, float>::value) {
+    switch (p.c) {
+      case 1: launch_xreg_c<OP, TS, TD, 1>(st, p); return 0;
+      case 2: launch_xreg_c<OP, TS, TD, 2>(st, p); return 0;
+      case 3: launch_xreg_c<OP, TS, TD, 3>(st, p); return 0;
+      case 4: launch_xreg_c<OP, TS, TD, 4>(st, p); return 0;
+      case 5: launch_xreg_c<OP, TS, TD, 5>(st, p); return 0;
+      case 6: launch_xreg_c<OP, TS, TD, 6>(st, p); return 0;
+      case 7: launch_xreg_c<OP, TS, TD, 7>(st, p); return 0;
+      case 8: launch_xreg_c<OP, TS, TD, 8>(st, p); return 0;
+    }
+  }
+  *err = "internal: no register-window x kernel for this half width";
+  return CLK_INVALID_VALUE;
+}
+
 template <int OP, class TS, class TD>
 int32_t launch_sep_typed(hipStream_t st, SepParams& p, int axis_is_x, std::string* err) {
   const int64_t V = p.outer * p.L * p.inner;
   if (V == 0) return 0;
   int32_t rc = 0;
-  if (axis_is_x && p.c <= kMaxCX) {
+  constexpr bool kXRegPair = std::is_same<TS, TD>::value || std::is_same<TD, float>::value;
+  const int xreg_maxc = 8;  // wider Gaussian windows run faster through LDS
+  if (kXRegPair && axis_is_x && p.c >= 1 && p.c <= xreg_maxc && p.L % kXK == 0 &&
+      (uintptr_t)p.src % (kXK * sizeof(TS)) == 0 && (uintptr_t)p.dst % (kXK * sizeof(TD)) == 0 &&
+      !no_xreg()) {
+    p.nseg = (int32_t)((p.L + kXSeg - 1) / kXSeg);
+    p.ntasks = p.outer * p.nseg;
+    if (p.ntasks > 0x7ffffff0LL) {
+      *err = "volume too large for the x pass grid";
+      return CLK_INVALID_VALUE;
+    }
+    rc = launch_xreg<OP, TS, TD>(st, p, err);
+  } else if (axis_is_x && p.c <= kMaxCX) {
     const int64_t W = p.L;
     // larger tasks for long rows and narrow halos (more loads in flight per
     // wave, less per-task work); CLK_XSUB overrides when tuning
@@ -405,8 +516,26 @@ int32_t launch_sep_typed(hipStream_t st, SepParams& p, int axis_is_x, std::strin
                     ((uintptr_t)p.dst % (4 * sizeof(TD)) == 0);
     const bool v2 = (p.inner % 2 == 0) && ((uintptr_t)p.src % (2 * sizeof(TS)) == 0) &&
                     ((uintptr_t)p.dst % (2 * sizeof(TD)) == 0);
+    // static-width kernel: fast pairs, half width 1..8 (Gaussian 1..12),
+    // block offsets within 32 bits, axis length within int
+    constexpr bool kStaticPair = std::is_same<TS, TD>::value || std::is_same<TD, float>::value;
+    const int smaxc = (OP == OP_GAUSS) ? 12 : 8;
+    const int64_t maxbytes = (int64_t)(sizeof(TS) > sizeof(TD) ? sizeof(TS) : sizeof(TD));
+    const int64_t slabs_per_block = 256 / (p.inner / 4 > 0 ? p.inner / 4 : 1) + 2;
+    const bool off32 = slabs_per_block * p.L * p.inner * maxbytes < (int64_t(1) << 31);
+    if (kStaticPair && p.c >= 1 && p.c <= smaxc && off32 && p.L < (int64_t(1) << 30) &&
+        strided_variant() == 0) {
+      rc = launch_ystatic<OP, TS, TD>(st, p, v4, err);
+      if (rc) return rc;
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) {
+        *err = std::string("kernel launch failed: ") + hipGetErrorString(e);
+        return CLK_OUT_OF_RESOURCES;
+      }
+      return 0;
+    }
     int var = strided_variant();
-    if (var == 0) var = (p.ntaps <= 8) ? 1 : 2;
+    if (var <= 0) var = (p.ntaps <= 8) ? 1 : 2;
     if (var == 2 && v2) rc = launch_strided<OP, TS, TD, 16, 2>(st, p, err);
     else if (v4) rc = launch_strided<OP, TS, TD, 8, 4>(st, p, err);
     else rc = launch_strided<OP, TS, TD, 8, 1>(st, p, err);

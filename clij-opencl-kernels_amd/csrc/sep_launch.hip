@@ -136,9 +136,14 @@ int32_t launch_separable(int op, hipStream_t st, const clk_image& dst, const clk
     }
     p.div = hsum;
     p.rdiv = 1.0f / hsum;
+    // integer pixels: every nonzero numerator is at least the smallest weight
+    float hmin = 1.f;
+    for (int v = 0; v <= 2 * c; ++v) hmin = (p.w[v] < hmin) ? p.w[v] : hmin;
+    p.fastdiv = (src.dtype != CLK_FLOAT) && (hmin >= 0x1p-48f);
   } else if (op == OP_MEAN) {
     p.div = (float)(2 * c + 1);  // mean_separable_3d_x.cl:26 counts the taps in float
     p.rdiv = 1.0f / p.div;
+    p.fastdiv = (src.dtype != CLK_FLOAT);  // integer sums: 0 or >= 1 in magnitude
   }
   int32_t rc;
   switch (op) {
