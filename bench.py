@@ -144,12 +144,63 @@ def cpu_baseline(nthreads, sample=(128, 256, 256)):
                       "oracle/clk_oracle.c with OpenMP, %.1f s" % (W, H, D, dt)}
 
 
+def make_c3_volume(torch, D, H, W, seed=3):
+    """uint8 volume: uniform integers in [40, 200] with 5% salt-and-pepper
+    voxels (0 or 255, equal odds)."""
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    vol = torch.randint(40, 201, (D, H, W), generator=g, device="cuda", dtype=torch.int32)
+    u = torch.rand((D, H, W), generator=g, device="cuda")
+    salt = torch.rand((D, H, W), generator=g, device="cuda") < 0.5
+    vol = torch.where(u < 0.05, torch.where(salt, 255, 0), vol)
+    return vol.to(torch.uint8)
+
+
+def bench_median(args, torch, clkmi):
+    W, H, D = (int(v) for v in (args.size or "512x512x256").split("x"))
+    V = W * H * D
+    torch.cuda.init()
+    src = make_c3_volume(torch, D, H, W)
+    dst = torch.empty_like(src)
+    si = clkmi.Image(src.data_ptr(), CLK_UCHAR, W, H, D)
+    di = clkmi.Image(dst.data_ptr(), CLK_UCHAR, W, H, D)
+    stream = torch.cuda.current_stream()
+    res = []
+    for r in (1, 2):
+        n = 2 * r + 1
+        for _ in range(args.warmup):
+            clkmi.median_box_3d(di, si, n, n, n, stream=stream)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record(stream)
+        for _ in range(args.steps):
+            clkmi.median_box_3d(di, si, n, n, n, stream=stream)
+        b.record(stream)
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / args.steps
+        res.append({"radius": r, "ms": round(ms, 4), "Gvoxel_per_s": round(V / (ms * 1e-3) / 1e9, 2),
+                    "GBps": round(2 * V / (ms * 1e-3) / 1e9, 1)})
+    tot = sum(x["ms"] for x in res)
+    out = {
+        "metric": "Gvoxel/s, median_box_3d", "value": round(2 * V / (tot * 1e-3) / 1e9, 3),
+        "unit": "Gvoxel/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(tot, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u8",
+        "data": "synthetic: uint8 uniform [40,200] + 5% salt-and-pepper, seed 3",
+        "config": {"workload": "median_box_3d r=1 and r=2 on uint8 %dx%dx%d" % (W, H, D)},
+        "medians": res,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--size", default="1024x1024x512", help="WxHxD")
+    ap.add_argument("--size", default=None, help="WxHxD (default: the workload's own size)")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3"],
+                    help="c2: separable suite on uint16 1024x1024x512 (the headline); "
+                         "c3: median_box_3d r=1 and r=2 on uint8 512x512x256")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", default=None, help="run only launches whose label contains this")
     ap.add_argument("--rocprof-summary", default=None, metavar="DIR",
@@ -168,7 +219,9 @@ def main():
     import torch
     import clkmi
 
-    W, H, D = (int(v) for v in args.size.split("x"))
+    if args.workload == "c3":
+        return bench_median(args, torch, clkmi)
+    W, H, D = (int(v) for v in (args.size or "1024x1024x512").split("x"))
     V = W * H * D
     torch.cuda.init()
     src = make_c2_volume(torch, D, H, W)
