@@ -75,13 +75,15 @@ def rocprof_summary(d):
     import re
     rows = {}
     def short(name):
+        if "neg_kernel" in name:
+            return "torch neg int32 1 GiB (calibration: 1 GiB read + 1 GiB write)"
         n = name.split("(")[0].replace("void ", "")
         return re.sub(r"\s+", " ", n)[:110]
     for path in glob.glob(os.path.join(d, "**", "*_counter_collection.csv"), recursive=True):
         per = {}
         for r in csv.DictReader(open(path)):
             k = r["Kernel_Name"]
-            if "clk::" not in k and "copy" not in k.lower():
+            if "clk::" not in k and "neg_kernel" not in k:
                 continue
             key = (short(k), r["Dispatch_Id"])
             per.setdefault(key, {})
@@ -93,7 +95,7 @@ def rocprof_summary(d):
     for path in glob.glob(os.path.join(d, "**", "*_kernel_stats.csv"), recursive=True):
         for r in csv.DictReader(open(path)):
             k = r["Name"]
-            if "clk::" not in k and "copy" not in k.lower():
+            if "clk::" not in k and "neg_kernel" not in k:
                 continue
             acc = rows.setdefault(short(k), {})
             acc["calls"] = [float(r["Calls"])]
@@ -261,10 +263,13 @@ def main():
         step()
     torch.cuda.synchronize()
     if args.copy_calibration:
-        ca = torch.empty(1 << 29, dtype=torch.int16, device="cuda").fill_(1)
+        # a plain streaming kernel with known traffic: 1 GiB read + 1 GiB
+        # written (torch's int32 negation), the scale for rocprof's
+        # FETCH_SIZE / WRITE_SIZE
+        ca = torch.ones(1 << 28, dtype=torch.int32, device="cuda")
         cb = torch.empty_like(ca)
         for _ in range(3):
-            cb.copy_(ca)
+            torch.neg(ca, out=cb)
         torch.cuda.synchronize()
         del ca, cb
 

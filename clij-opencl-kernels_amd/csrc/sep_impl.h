@@ -521,8 +521,13 @@ int32_t launch_sep_typed(hipStream_t st, SepParams& p, int axis_is_x, std::strin
     constexpr bool kStaticPair = std::is_same<TS, TD>::value || std::is_same<TD, float>::value;
     const int smaxc = (OP == OP_GAUSS) ? 12 : 8;
     const int64_t maxbytes = (int64_t)(sizeof(TS) > sizeof(TD) ? sizeof(TS) : sizeof(TD));
-    const int64_t slabs_per_block = 256 / (p.inner / 4 > 0 ? p.inner / 4 : 1) + 2;
-    const bool off32 = slabs_per_block * p.L * p.inner * maxbytes < (int64_t(1) << 31);
+    // a block's lanes lie in at most ceil(256 / column groups) + 1 slabs
+    // (column groups >= inner / 4), so the largest per-lane offset is below
+    // (that many slabs - 1) * slab + inner pixels
+    const int64_t ncg_min = p.inner / 4 > 0 ? p.inner / 4 : 1;
+    int64_t extra = (256 + ncg_min - 1) / ncg_min;
+    if (extra > p.outer - 1) extra = p.outer - 1;
+    const bool off32 = (extra * p.L * p.inner + p.inner) * maxbytes < (int64_t(1) << 31);
     if (kStaticPair && p.c >= 1 && p.c <= smaxc && off32 && p.L < (int64_t(1) << 30) &&
         strided_variant() == 0) {
       rc = launch_ystatic<OP, TS, TD>(st, p, v4, err);
