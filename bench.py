@@ -66,50 +66,94 @@ def make_c2_volume(torch, D, H, W, seed=2, nblobs=500, amp=2000.0, sigma=4.0, la
     return out
 
 
+PROFILE_COUNTERS = os.path.join(ROOT, "profiles", "separable_u16_1024x1024x512_counters.md")
+
+
 def rocprof_summary(d):
-    """Reduce rocprofv3 CSV output under directory d (counter collections
-    and/or kernel stats) to a per-kernel markdown table for this library's
-    kernels and the device copy kernel."""
+    """Per-launch table from rocprofv3 output under directory d: one
+    sub-directory per --pmc pass of `bench.py --steps 1 --warmup 1
+    --copy-calibration` (same dispatch order in every pass) and optionally a
+    --kernel-trace --stats run.  HBM bytes are counted from the TCC request
+    counters by request size (128/64/32-byte reads, 64/32-byte writes)."""
     import csv
     import glob
-    import re
-    rows = {}
-    def short(name):
-        if "neg_kernel" in name:
-            return "torch neg int32 1 GiB (calibration: 1 GiB read + 1 GiB write)"
-        n = name.split("(")[0].replace("void ", "")
-        return re.sub(r"\s+", " ", n)[:110]
-    for path in glob.glob(os.path.join(d, "**", "*_counter_collection.csv"), recursive=True):
-        per = {}
+    suite = [x[0] for x in suite_c2()]
+    bpp = {"u16": 2, "f32": 4}
+    alg = {x[0]: bpp[x[5]] + bpp[x[6]] for x in suite_c2()}
+    per_launch = {}
+    calib = {}
+    for path in sorted(glob.glob(os.path.join(d, "**", "*_counter_collection.csv"), recursive=True)):
+        disp = {}
+        names = {}
         for r in csv.DictReader(open(path)):
             k = r["Kernel_Name"]
             if "clk::" not in k and "neg_kernel" not in k:
                 continue
-            key = (short(k), r["Dispatch_Id"])
-            per.setdefault(key, {})
-            per[key][r["Counter_Name"]] = per[key].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
-        for (k, _), cv in per.items():
-            acc = rows.setdefault(k, {})
-            for cn, v in cv.items():
-                acc.setdefault(cn, []).append(v)
+            i = int(r["Dispatch_Id"])
+            names[i] = k
+            disp.setdefault(i, {})
+            disp[i][r["Counter_Name"]] = disp[i].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+        clk_ids = [i for i in sorted(disp) if "clk::" in names[i]]
+        for j, i in enumerate(clk_ids):
+            lab = suite[j % len(suite)]
+            e = per_launch.setdefault(lab, {"kernel": names[i].split("(")[0].replace("void ", "")})
+            for cn, v in disp[i].items():
+                e.setdefault(cn, []).append(v)
+        for i in sorted(disp):
+            if "neg_kernel" in names[i]:
+                for cn, v in disp[i].items():
+                    calib.setdefault(cn, []).append(v)
+    trace = {}
     for path in glob.glob(os.path.join(d, "**", "*_kernel_stats.csv"), recursive=True):
         for r in csv.DictReader(open(path)):
-            k = r["Name"]
-            if "clk::" not in k and "neg_kernel" not in k:
-                continue
-            acc = rows.setdefault(short(k), {})
-            acc["calls"] = [float(r["Calls"])]
-            acc["avg_us"] = [float(r["AverageNs"]) / 1e3]
-    cols = sorted({c for v in rows.values() for c in v})
-    print("| kernel | " + " | ".join(cols) + " |")
-    print("|---|" + "---|" * len(cols))
-    for k in sorted(rows):
-        v = rows[k]
-        cells = []
-        for c in cols:
-            x = v.get(c)
-            cells.append("" if not x else "%.4g" % (sum(x) / len(x)))
-        print("| %s | %s |" % (k, " | ".join(cells)))
+            trace[r["Name"].split("(")[0].replace("void ", "")] = float(r["AverageNs"]) / 1e3
+
+    def avg(e, k):
+        v = e.get(k)
+        return sum(v) / len(v) if v else None
+
+    def bytes_rw(e):
+        r128, r64, r32 = avg(e, "TCC_EA0_RDREQ_128B_sum"), avg(e, "TCC_EA0_RDREQ_64B_sum"), avg(e, "TCC_EA0_RDREQ_32B_sum")
+        w, w64 = avg(e, "TCC_EA0_WRREQ_sum"), avg(e, "TCC_EA0_WRREQ_64B_sum")
+        rd = None if r128 is None else 128 * r128 + 64 * (r64 or 0) + 32 * (r32 or 0)
+        wr = None if w is None else 64 * w64 + 32 * (w - w64)
+        return rd, wr
+
+    V = 1024 * 1024 * 512
+    print("| launch | kernel | avg us (trace) | read bytes | write bytes | algorithmic bytes | traffic / algorithmic | FETCH_SIZE KiB | WRITE_SIZE KiB | VALUBusy | OccupancyPercent |")
+    print("|---|---|---|---|---|---|---|---|---|---|---|")
+    fmt = lambda x, f="%.4g": "" if x is None else f % x
+    for lab in suite:
+        e = per_launch.get(lab)
+        if not e:
+            continue
+        rd, wr = bytes_rw(e)
+        ab = alg[lab] * V
+        ratio = (rd + wr) / ab if rd is not None and wr is not None else None
+        print("| %s | %s | %s | %s | %s | %d | %s | %s | %s | %s | %s |" % (
+            lab, e["kernel"], fmt(trace.get(e["kernel"])), fmt(rd), fmt(wr), ab, fmt(ratio, "%.3f"),
+            fmt(avg(e, "FETCH_SIZE")), fmt(avg(e, "WRITE_SIZE")), fmt(avg(e, "VALUBusy"), "%.1f"),
+            fmt(avg(e, "OccupancyPercent"), "%.1f")))
+    if calib:
+        rd, wr = bytes_rw(calib)
+        print()
+        print("Calibration kernel (torch int32 negation, 1 GiB read + 1 GiB written): "
+              "read bytes %s, write bytes %s, FETCH_SIZE %s KiB, WRITE_SIZE %s KiB (nominal 1048576 KiB each)."
+              % (fmt(rd), fmt(wr), fmt(avg(calib, "FETCH_SIZE")), fmt(avg(calib, "WRITE_SIZE"))))
+
+
+def committed_traffic(label):
+    """HBM bytes per launch (read + write) of `label` from the committed
+    counter profile, or None."""
+    try:
+        for line in open(PROFILE_COUNTERS):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) >= 5 and cells[0] == label and cells[3] and cells[4]:
+                return float(cells[3]) + float(cells[4])
+    except (OSError, ValueError):
+        return None
+    return None
+
 
 def suite_c2():
     """(label, kernel, dim, N, s, src kind, dst kind) of one step."""
@@ -329,7 +373,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(dom["GBps"] / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": committed_traffic(dom["launch"]) if (W, H, D) == (1024, 1024, 512) else None,
         },
         "cpu_baseline": cpu,
         "launches": launches,
