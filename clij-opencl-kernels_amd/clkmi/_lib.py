@@ -31,7 +31,11 @@ EXPORTED = (
     "clk_mean_box_3d", "clk_minimum_box_3d", "clk_maximum_box_3d", "clk_init", "clk_malloc",
     "clk_free", "clk_memcpy_h2d", "clk_memcpy_d2h", "clk_stream_create", "clk_stream_destroy",
     "clk_stream_sync", "clk_dtype_size", "clk_last_error_string", "clk_version",
+    "clk_batch_filter_3d", "clk_set_tuning", "clk_get_tuning",
 )
+
+CLK_JOB_GAUSSIAN_BLUR_3D, CLK_JOB_MEAN_BOX_3D, CLK_JOB_MINIMUM_BOX_3D, CLK_JOB_MAXIMUM_BOX_3D, \
+    CLK_JOB_MEDIAN_BOX_3D = 0, 1, 2, 3, 4
 
 
 class ClkImage(ctypes.Structure):
@@ -53,6 +57,17 @@ class ClkArg(ctypes.Structure):
         ("f", ctypes.c_float),
         ("pad", ctypes.c_int32),
         ("image", ctypes.POINTER(ClkImage)),
+    ]
+
+
+class ClkFilterJob(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("dst", ClkImage),
+        ("src", ClkImage),
+        ("sigma", ctypes.c_float * 3),
+        ("radius", ctypes.c_int32 * 3),
     ]
 
 
@@ -93,6 +108,12 @@ def lib():
             fn = getattr(L, name)
             fn.argtypes = [vp, img, img, i32, i32, i32]
             fn.restype = i32
+        L.clk_batch_filter_3d.argtypes = [vp, ctypes.POINTER(ClkFilterJob), i32]
+        L.clk_batch_filter_3d.restype = i32
+        L.clk_set_tuning.argtypes = [ctypes.c_char_p, i32]
+        L.clk_set_tuning.restype = i32
+        L.clk_get_tuning.argtypes = [ctypes.c_char_p, ctypes.POINTER(i32)]
+        L.clk_get_tuning.restype = i32
         L.clk_init.argtypes = []
         L.clk_init.restype = i32
         L.clk_malloc.argtypes = [ctypes.POINTER(vp), i64]

@@ -1,0 +1,463 @@
+// Batched execution of many 3D filters (clk_batch_filter_3d).
+//
+// A 3D separable filter is up to three single-axis passes; a median is one
+// launch.  The planner turns every job into its passes (same temporaries
+// and pass order as the single calls), then runs "stage" 0, 1, 2 -- the
+// first, second and third active pass of every job.  Within a stage the
+// passes are grouped by kernel class (kernel kind, op, pixel types, column
+// width); each class is ONE launch whose blocks come from a device tile
+// table (job, virtual block index), and each block runs the same kernel
+// body as the single launch with that job's parameters.  So a batch of 256
+// uneven volumes costs a few launches per stage instead of ~768, and the
+// results equal the single calls bit for bit.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/clkmi.h"
+#include "clk_device.h"
+#include "clk_internal.h"
+#include "clk_tuning.h"
+#include "median_impl.h"
+#include "sep_impl.h"
+
+namespace clk {
+
+struct BatchTile {
+  uint32_t job;   // index into the class's parameter array
+  uint32_t vblk;  // block index within that job's launch
+};
+
+template <int OP, class TS, class TD>
+__global__ __launch_bounds__(256) void batch_x_kernel(const SepParams* __restrict__ ps,
+                                                      const BatchTile* __restrict__ tiles) {
+  extern __shared__ uint32_t lds[];
+  const BatchTile t = tiles[blockIdx.x];
+  sep_x_body<OP, TS, TD, 1>(ps[t.job], t.vblk, lds);
+}
+
+template <int OP, class TS, class TD, int COLS>
+__global__ __launch_bounds__(256) void batch_strided_kernel(const SepParams* __restrict__ ps,
+                                                            const BatchTile* __restrict__ tiles) {
+  const BatchTile t = tiles[blockIdx.x];
+  sep_strided_body<OP, TS, TD, 8, COLS>(ps[t.job], t.vblk);
+}
+
+template <class TS, class TD, int KIND>
+__global__ __launch_bounds__(256) void batch_median_kernel(const MedParams* __restrict__ ms,
+                                                           const BatchTile* __restrict__ tiles) {
+  const BatchTile t = tiles[blockIdx.x];
+  const MedParams& p = ms[t.job];
+  const uint32_t bx = t.vblk % p.gx;
+  const uint32_t r = t.vblk / p.gx;
+  const uint32_t by = r % p.gy, bz = r / p.gy;
+  if constexpr (KIND == MED_PK8) median_27_pk8_body<TS, TD>(p, bx, by, bz);
+  else if constexpr (KIND == MED_27) median_27_body<TS, TD>(p, bx, by, bz);
+  else median_generic_body<TS, TD>(p, bx, by, bz);
+}
+
+template <class TS, class TD>
+__global__ __launch_bounds__(64) void batch_hist8_kernel(const MedParams* __restrict__ ms,
+                                                         const BatchTile* __restrict__ tiles) {
+  const BatchTile t = tiles[blockIdx.x];
+  const MedParams& p = ms[t.job];
+  const uint32_t bx = t.vblk % p.gx;
+  const uint32_t r = t.vblk / p.gx;
+  median_hist8_body<TS, TD>(p, bx, r % p.gy, r / p.gy);
+}
+
+namespace {
+
+enum ClassKind : int { K_X = 0, K_STRIDED4 = 1, K_STRIDED1 = 2, K_MEDIAN = 3 };
+
+// Pixel-type pairs with batched kernels: same type, to float, from float.
+bool batch_pair(int ts, int td) { return ts == td || td == CLK_FLOAT || ts == CLK_FLOAT; }
+
+struct ClassKey {
+  int kind, op, ts, td;  // op: separable op, or MedKind for medians
+  bool operator<(const ClassKey& o) const {
+    return std::tie(kind, op, ts, td) < std::tie(o.kind, o.op, o.ts, o.td);
+  }
+};
+
+struct ClassWork {
+  std::vector<SepParams> sep;
+  std::vector<MedParams> med;
+  std::vector<BatchTile> tiles;
+  size_t lds = 0;
+  // device offsets inside the table area
+  size_t params_off = 0, tiles_off = 0;
+};
+
+// Pinned staging buffers for the launch tables, reused round robin; a slot
+// is reused only after its previous upload completed.
+struct Staging {
+  void* host = nullptr;
+  size_t cap = 0;
+  hipEvent_t ev = nullptr;
+  bool pending = false;
+};
+std::mutex g_stage_mu;
+Staging g_stage[4];
+int g_stage_next = 0;
+
+size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+int32_t sigma_n(float sigma) {
+  int32_t n = (int32_t)(sigma * 8.0f);
+  return (n % 2 == 0) ? n + 1 : n;
+}
+
+template <int OP, class TS, class TD>
+void launch_class_typed(hipStream_t st, const ClassKey& k, const ClassWork& w, char* dev) {
+  const uint32_t nt = (uint32_t)w.tiles.size();
+  const BatchTile* tiles = reinterpret_cast<const BatchTile*>(dev + w.tiles_off);
+  const SepParams* ps = reinterpret_cast<const SepParams*>(dev + w.params_off);
+  if (k.kind == K_X)
+    hipLaunchKernelGGL((batch_x_kernel<OP, TS, TD>), dim3(nt), dim3(256), w.lds, st, ps, tiles);
+  else if (k.kind == K_STRIDED4)
+    hipLaunchKernelGGL((batch_strided_kernel<OP, TS, TD, 4>), dim3(nt), dim3(256), 0, st, ps, tiles);
+  else
+    hipLaunchKernelGGL((batch_strided_kernel<OP, TS, TD, 1>), dim3(nt), dim3(256), 0, st, ps, tiles);
+}
+
+template <class TS, class TD>
+void launch_median_class(hipStream_t st, const ClassKey& k, const ClassWork& w, char* dev) {
+  const uint32_t nt = (uint32_t)w.tiles.size();
+  const BatchTile* tiles = reinterpret_cast<const BatchTile*>(dev + w.tiles_off);
+  const MedParams* ms = reinterpret_cast<const MedParams*>(dev + w.params_off);
+  switch (k.op) {
+    case MED_PK8: hipLaunchKernelGGL((batch_median_kernel<TS, TD, MED_PK8>), dim3(nt), dim3(256), 0, st, ms, tiles); break;
+    case MED_HIST8: hipLaunchKernelGGL((batch_hist8_kernel<TS, TD>), dim3(nt), dim3(64), 0, st, ms, tiles); break;
+    case MED_27: hipLaunchKernelGGL((batch_median_kernel<TS, TD, MED_27>), dim3(nt), dim3(256), 0, st, ms, tiles); break;
+    default: hipLaunchKernelGGL((batch_median_kernel<TS, TD, MED_GENERIC>), dim3(nt), dim3(256), 0, st, ms, tiles); break;
+  }
+}
+
+// Dispatch (op, ts, td) to a template instantiation; only batch_pair()
+// combinations reach here.
+template <int OP, class TS>
+void sep_dst(hipStream_t st, const ClassKey& k, const ClassWork& w, char* dev) {
+  switch (k.td) {
+    case CLK_CHAR: if constexpr (std::is_same<TS, int8_t>::value || std::is_same<TS, float>::value) launch_class_typed<OP, TS, int8_t>(st, k, w, dev); break;
+    case CLK_UCHAR: if constexpr (std::is_same<TS, uint8_t>::value || std::is_same<TS, float>::value) launch_class_typed<OP, TS, uint8_t>(st, k, w, dev); break;
+    case CLK_SHORT: if constexpr (std::is_same<TS, int16_t>::value || std::is_same<TS, float>::value) launch_class_typed<OP, TS, int16_t>(st, k, w, dev); break;
+    case CLK_USHORT: if constexpr (std::is_same<TS, uint16_t>::value || std::is_same<TS, float>::value) launch_class_typed<OP, TS, uint16_t>(st, k, w, dev); break;
+    case CLK_FLOAT: launch_class_typed<OP, TS, float>(st, k, w, dev); break;
+  }
+}
+
+template <int OP>
+void sep_src(hipStream_t st, const ClassKey& k, const ClassWork& w, char* dev) {
+  switch (k.ts) {
+    case CLK_CHAR: sep_dst<OP, int8_t>(st, k, w, dev); break;
+    case CLK_UCHAR: sep_dst<OP, uint8_t>(st, k, w, dev); break;
+    case CLK_SHORT: sep_dst<OP, int16_t>(st, k, w, dev); break;
+    case CLK_USHORT: sep_dst<OP, uint16_t>(st, k, w, dev); break;
+    case CLK_FLOAT: sep_dst<OP, float>(st, k, w, dev); break;
+  }
+}
+
+template <class TS>
+void med_dst(hipStream_t st, const ClassKey& k, const ClassWork& w, char* dev) {
+  switch (k.td) {
+    case CLK_CHAR: if constexpr (std::is_same<TS, int8_t>::value || std::is_same<TS, float>::value) launch_median_class<TS, int8_t>(st, k, w, dev); break;
+    case CLK_UCHAR: if constexpr (std::is_same<TS, uint8_t>::value || std::is_same<TS, float>::value) launch_median_class<TS, uint8_t>(st, k, w, dev); break;
+    case CLK_SHORT: if constexpr (std::is_same<TS, int16_t>::value || std::is_same<TS, float>::value) launch_median_class<TS, int16_t>(st, k, w, dev); break;
+    case CLK_USHORT: if constexpr (std::is_same<TS, uint16_t>::value || std::is_same<TS, float>::value) launch_median_class<TS, uint16_t>(st, k, w, dev); break;
+    case CLK_FLOAT: launch_median_class<TS, float>(st, k, w, dev); break;
+  }
+}
+
+void launch_class(hipStream_t st, const ClassKey& k, const ClassWork& w, char* dev) {
+  if (k.kind == K_MEDIAN) {
+    switch (k.ts) {
+      case CLK_CHAR: med_dst<int8_t>(st, k, w, dev); break;
+      case CLK_UCHAR: med_dst<uint8_t>(st, k, w, dev); break;
+      case CLK_SHORT: med_dst<int16_t>(st, k, w, dev); break;
+      case CLK_USHORT: med_dst<uint16_t>(st, k, w, dev); break;
+      case CLK_FLOAT: med_dst<float>(st, k, w, dev); break;
+    }
+    return;
+  }
+  switch (k.op) {
+    case OP_GAUSS: sep_src<OP_GAUSS>(st, k, w, dev); break;
+    case OP_MEAN: sep_src<OP_MEAN>(st, k, w, dev); break;
+    case OP_MIN: sep_src<OP_MIN>(st, k, w, dev); break;
+    case OP_MAX: sep_src<OP_MAX>(st, k, w, dev); break;
+  }
+}
+
+// One single-axis pass of a job.  src_tmp/dst_tmp >= 0 are byte offsets
+// of a temporary in the batch's temp area (the pointer is set once the area
+// is allocated).
+struct Pass {
+  int op;
+  clk_image dst, src;
+  int dim, N;
+  float s;
+  int64_t src_tmp, dst_tmp;
+};
+
+// A pass that does not go through a batched class (rare pixel-type pairs,
+// degenerate parameters) runs as a single launch, in stage order.
+struct Single {
+  int stage;
+  Pass pass;
+};
+
+}  // namespace
+
+int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std::string* err) {
+  // ---- plan: passes per stage, temporaries --------------------------------
+  std::vector<std::vector<Pass>> stage_passes(3);
+  std::vector<std::pair<clk_filter_job, int>> medians;  // job, unused
+  std::vector<std::pair<clk_image, clk_image>> copies;  // no active axis: converting copy
+
+  int64_t temp_bytes = 0;
+  for (int j = 0; j < njobs; ++j) {
+    const clk_filter_job& jb = jobs[j];
+    if (jb.kind == CLK_JOB_MEDIAN_BOX_3D) {
+      medians.push_back({jb, 0});
+      continue;
+    }
+    const int op = jb.kind;  // CLK_JOB_* of separable filters equal the Op codes
+    bool act[3];
+    int Ns[3];
+    for (int a = 0; a < 3; ++a) {
+      if (op == OP_GAUSS) {
+        act[a] = jb.sigma[a] > 0.f;
+        Ns[a] = sigma_n(jb.sigma[a]);
+      } else {
+        act[a] = jb.radius[a] > 0;
+        Ns[a] = 2 * jb.radius[a] + 1;
+      }
+    }
+    int nact = (int)act[0] + (int)act[1] + (int)act[2];
+    if (nact == 0) {
+      copies.push_back({jb.dst, jb.src});
+      continue;
+    }
+    // temporaries: float for Gaussian and mean; the source type for min/max,
+    // whose values are exact in any type that holds the source
+    const int tdt = (op == OP_MIN || op == OP_MAX) ? jb.src.dtype : CLK_FLOAT;
+    const int64_t V = jb.dst.width * jb.dst.height * jb.dst.depth;
+    const int ntmp = nact - 1 < 2 ? nact - 1 : 2;
+    std::vector<int64_t> toff(ntmp);
+    for (int i = 0; i < ntmp; ++i) {
+      toff[i] = temp_bytes;
+      temp_bytes += (int64_t)align256((size_t)(V * clk_dtype_size(tdt)));
+    }
+    int done = 0, ti = 0, stage = 0;
+    clk_image cur = jb.src;
+    int64_t cur_tmp = -1;
+    for (int a = 0; a < 3; ++a) {
+      if (!act[a]) continue;
+      ++done;
+      clk_image out = jb.dst;
+      int64_t out_tmp = -1;
+      if (done < nact) {
+        out.dtype = tdt;
+        out.ptr = nullptr;
+        out_tmp = toff[ti];
+        ti ^= 1;
+      }
+      stage_passes[stage++].push_back({op, out, cur, a, Ns[a], op == OP_GAUSS ? jb.sigma[a] : 0.f,
+                                       cur_tmp, out_tmp});
+      cur = out;
+      cur_tmp = out_tmp;
+    }
+  }
+
+  // ---- build classes ------------------------------------------------------
+  const int bseg = tuning(TUNE_BATCH_SEG) > 0 ? tuning(TUNE_BATCH_SEG) : 64;
+  std::vector<std::map<ClassKey, ClassWork>> classes(3);
+  std::vector<Single> singles;
+  void* temp_base = nullptr;
+  if (temp_bytes > 0 && hipMallocAsync(&temp_base, (size_t)temp_bytes, st) != hipSuccess) {
+    *err = "batch temporaries: allocation failed";
+    return CLK_MEM_OBJECT_ALLOCATION_FAILURE;
+  }
+  for (int s = 0; s < 3; ++s)
+    for (Pass& ps : stage_passes[s]) {
+      if (ps.dst_tmp >= 0) ps.dst.ptr = static_cast<char*>(temp_base) + ps.dst_tmp;
+      if (ps.src_tmp >= 0) ps.src.ptr = static_cast<char*>(temp_base) + ps.src_tmp;
+    }
+  for (int s = 0; s < 3; ++s) {
+    for (const Pass& ps : stage_passes[s]) {
+      SepParams p;
+      int axis_is_x = 0;
+      bool nan_fill = false;
+      int32_t rc = prepare_separable(ps.op, ps.dst, ps.src, ps.dim, ps.N, ps.s, p, &axis_is_x, &nan_fill, err);
+      if (rc) {
+        if (temp_base) (void)hipFreeAsync(temp_base, st);
+        return rc;
+      }
+      const int64_t V = ps.dst.width * ps.dst.height * ps.dst.depth;
+      if (V == 0) continue;
+      if (nan_fill || !batch_pair(ps.src.dtype, ps.dst.dtype) || p.L >= (int64_t(1) << 30)) {
+        singles.push_back({s, ps});
+        continue;
+      }
+      ClassKey key{0, ps.op, ps.src.dtype, ps.dst.dtype};
+      uint32_t blocks = 0;
+      size_t lds = 0;
+      if (axis_is_x && p.c <= kMaxCX) {
+        key.kind = K_X;
+        const int64_t W = p.L;
+        p.nseg = (int32_t)((W + kXSeg - 1) / kXSeg);
+        p.ntasks = p.outer * p.nseg;
+        p.lds_stride = lds_pad(kXSeg + 2 * p.c) + 1;
+        const int ss = clk_dtype_size(ps.src.dtype), ds = clk_dtype_size(ps.dst.dtype);
+        p.vec_ld = (W % kXK == 0) && ((uintptr_t)p.src % (kXK * ss) == 0);
+        p.vec_st = (W % kXK == 0) && ((uintptr_t)p.dst % (kXK * ds) == 0);
+        blocks = (uint32_t)((p.ntasks + 3) / 4);
+        lds = (size_t)4 * p.lds_stride * 4;
+      } else {
+        const int ss = clk_dtype_size(ps.src.dtype), ds = clk_dtype_size(ps.dst.dtype);
+        const bool v4 = (p.inner % 4 == 0) && ((uintptr_t)p.src % (4 * ss) == 0) &&
+                        ((uintptr_t)p.dst % (4 * ds) == 0);
+        const int cols = v4 ? 4 : 1;
+        key.kind = v4 ? K_STRIDED4 : K_STRIDED1;
+        p.ncolgroups = p.inner / cols;
+        p.ncols_total = p.outer * p.ncolgroups;
+        const int64_t nblk = (p.ncols_total + 255) / 256;
+        int64_t seg = ((int64_t)bseg + 7) / 8 * 8;
+        if (seg > (p.L + 7) / 8 * 8) seg = (p.L + 7) / 8 * 8;
+        const int64_t nseg = (p.L + seg - 1) / seg;
+        p.seg = (int32_t)seg;
+        p.nseg = (int32_t)nseg;
+        p.nblk = (uint32_t)nblk;
+        blocks = (uint32_t)(nblk * nseg);
+      }
+      ClassWork& w = classes[s][key];
+      const uint32_t jid = (uint32_t)w.sep.size();
+      w.sep.push_back(p);
+      if (lds > w.lds) w.lds = lds;
+      for (uint32_t b = 0; b < blocks; ++b) w.tiles.push_back({jid, b});
+    }
+  }
+  for (auto& mj : medians) {
+    const clk_filter_job& jb = mj.first;
+    MedParams p;
+    memset(&p, 0, sizeof(p));
+    p.src = jb.src.ptr;
+    p.dst = jb.dst.ptr;
+    p.W = jb.dst.width;
+    p.H = jb.dst.height;
+    p.D = jb.dst.depth;
+    p.ex = jb.radius[0];
+    p.ey = jb.radius[1];
+    p.ez = jb.radius[2];
+    p.n = (2 * p.ex + 1) * (2 * p.ey + 1) * (2 * p.ez + 1);
+    if (p.W * p.H * p.D == 0) continue;
+    if (!batch_pair(jb.src.dtype, jb.dst.dtype)) {
+      std::string e2;
+      const int32_t rc = launch_median(st, jb.dst, jb.src, 2 * p.ex + 1, 2 * p.ey + 1, 2 * p.ez + 1, &e2);
+      if (rc) { *err = e2; if (temp_base) (void)hipFreeAsync(temp_base, st); return rc; }
+      continue;
+    }
+    dim3 grid;
+    int threads;
+    const MedKind mk = plan_median(p, clk_dtype_size(jb.dst.dtype), &grid, &threads);
+    ClassKey key{K_MEDIAN, (int)mk, jb.src.dtype, jb.dst.dtype};
+    ClassWork& w = classes[0][key];
+    const uint32_t jid = (uint32_t)w.med.size();
+    w.med.push_back(p);
+    const uint64_t blocks = (uint64_t)grid.x * grid.y * grid.z;
+    for (uint64_t b = 0; b < blocks; ++b) w.tiles.push_back({jid, (uint32_t)b});
+  }
+
+  // ---- upload the launch tables -----------------------------------------------
+  size_t tbytes = 0;
+  for (int s = 0; s < 3; ++s)
+    for (auto& kv : classes[s]) {
+      ClassWork& w = kv.second;
+      w.params_off = tbytes;
+      tbytes += align256(w.sep.size() * sizeof(SepParams) + w.med.size() * sizeof(MedParams));
+      w.tiles_off = tbytes;
+      tbytes += align256(w.tiles.size() * sizeof(BatchTile));
+    }
+  char* dev_tables = nullptr;
+  if (tbytes > 0) {
+    if (hipMallocAsync((void**)&dev_tables, tbytes, st) != hipSuccess) {
+      if (temp_base) (void)hipFreeAsync(temp_base, st);
+      *err = "batch tables: allocation failed";
+      return CLK_MEM_OBJECT_ALLOCATION_FAILURE;
+    }
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    Staging& sg = g_stage[g_stage_next];
+    g_stage_next = (g_stage_next + 1) % 4;
+    if (sg.pending) {
+      (void)hipEventSynchronize(sg.ev);
+      sg.pending = false;
+    }
+    if (!sg.ev && hipEventCreateWithFlags(&sg.ev, hipEventDisableTiming) != hipSuccess) sg.ev = nullptr;
+    if (sg.cap < tbytes) {
+      if (sg.host) (void)hipHostFree(sg.host);
+      sg.host = nullptr;
+      sg.cap = 0;
+      if (hipHostMalloc(&sg.host, tbytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipFreeAsync(dev_tables, st);
+        if (temp_base) (void)hipFreeAsync(temp_base, st);
+        *err = "batch tables: pinned staging allocation failed";
+        return CLK_OUT_OF_RESOURCES;
+      }
+      sg.cap = tbytes;
+    }
+    char* h = static_cast<char*>(sg.host);
+    for (int s = 0; s < 3; ++s)
+      for (auto& kv : classes[s]) {
+        const ClassWork& w = kv.second;
+        if (!w.sep.empty()) memcpy(h + w.params_off, w.sep.data(), w.sep.size() * sizeof(SepParams));
+        if (!w.med.empty()) memcpy(h + w.params_off, w.med.data(), w.med.size() * sizeof(MedParams));
+        memcpy(h + w.tiles_off, w.tiles.data(), w.tiles.size() * sizeof(BatchTile));
+      }
+    if (hipMemcpyAsync(dev_tables, h, tbytes, hipMemcpyHostToDevice, st) != hipSuccess) {
+      (void)hipFreeAsync(dev_tables, st);
+      if (temp_base) (void)hipFreeAsync(temp_base, st);
+      *err = "batch tables: upload failed";
+      return CLK_OUT_OF_RESOURCES;
+    }
+    if (sg.ev && hipEventRecord(sg.ev, st) == hipSuccess) sg.pending = true;
+  }
+
+  // ---- launch stage by stage --------------------------------------------------
+  int32_t rc = 0;
+  for (auto& c : copies) {
+    rc = launch_convert_copy(st, c.first, c.second, err);
+    if (rc) break;
+  }
+  for (int s = 0; s < 3 && rc == 0; ++s) {
+    for (auto& kv : classes[s]) {
+      if (kv.second.tiles.empty()) continue;
+      if (kv.second.tiles.size() > 0x7fffffffULL) {
+        *err = "batch too large for one launch";
+        rc = CLK_INVALID_VALUE;
+        break;
+      }
+      launch_class(st, kv.first, kv.second, dev_tables);
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) {
+        *err = std::string("batch launch failed: ") + hipGetErrorString(e);
+        rc = CLK_OUT_OF_RESOURCES;
+        break;
+      }
+    }
+    for (const Single& sg : singles) {
+      if (sg.stage != s || rc) continue;
+      rc = launch_separable(sg.pass.op, st, sg.pass.dst, sg.pass.src, sg.pass.dim, sg.pass.N, sg.pass.s, err);
+    }
+  }
+  if (dev_tables) (void)hipFreeAsync(dev_tables, st);
+  if (temp_base) (void)hipFreeAsync(temp_base, st);
+  return rc;
+}
+
+}  // namespace clk

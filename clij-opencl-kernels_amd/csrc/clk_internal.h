@@ -54,9 +54,19 @@ int32_t launch_separable(int op, hipStream_t st, const clk_image& dst,
                          const clk_image& src, int32_t dim, int32_t N, float s,
                          std::string* err);
 
+// Fill SepParams for one single-axis launch (weights, divisor, axis view).
+// Sets *fill_nan when the launch reduces to writing CONVERT(NaN) everywhere.
+int32_t prepare_separable(int op, const clk_image& dst, const clk_image& src, int32_t dim, int32_t N,
+                          float s, SepParams& p, int* axis_is_x, bool* fill_nan, std::string* err);
+int32_t launch_fill_nan(hipStream_t st, const clk_image& dst, std::string* err);
+
 // median_box_3d launch; arguments validated by the caller.
 int32_t launch_median(hipStream_t st, const clk_image& dst, const clk_image& src,
                       int32_t Nx, int32_t Ny, int32_t Nz, std::string* err);
+
+// Run a batch of 3D filter jobs stage by stage, one launch per kernel class
+// and stage (batch.hip).
+int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std::string* err);
 
 // Converting copy dst <- CONVERT(src) (used by the host chain for axes that
 // are not filtered).

@@ -1,0 +1,64 @@
+// Process-wide launch-shape knobs (see clk_tuning.h).
+#include "clk_tuning.h"
+
+#include <atomic>
+#include <ctype.h>
+#include <mutex>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../include/clkmi.h"
+
+namespace clk {
+namespace {
+
+const char* kNames[TUNE_COUNT] = {"strided_rounds", "x_sub", "batch_seg",
+                                  "no_xreg", "strided_variant", "no_hist8"};
+const int32_t kDefaults[TUNE_COUNT] = {2, 0, 64, 0, 0, 0};
+std::atomic<int32_t> g_val[TUNE_COUNT];
+std::once_flag g_once;
+
+void init() {
+  for (int k = 0; k < TUNE_COUNT; ++k) {
+    char env[64] = "CLK_";
+    for (int i = 0; kNames[k][i] && i < 58; ++i) env[4 + i] = (char)toupper((unsigned char)kNames[k][i]);
+    const char* v = getenv(env);
+    g_val[k].store((v && *v) ? (int32_t)atoi(v) : kDefaults[k]);
+  }
+}
+
+int find(const char* key) {
+  if (!key) return -1;
+  for (int k = 0; k < TUNE_COUNT; ++k)
+    if (strcmp(key, kNames[k]) == 0) return k;
+  return -1;
+}
+
+}  // namespace
+
+int32_t tuning(TuneKey k) {
+  std::call_once(g_once, init);
+  return g_val[k].load(std::memory_order_relaxed);
+}
+
+const char* tuning_name(int k) { return (k >= 0 && k < TUNE_COUNT) ? kNames[k] : nullptr; }
+
+}  // namespace clk
+
+extern "C" int32_t clk_set_tuning(const char* key, int32_t value) {
+  std::call_once(clk::g_once, clk::init);
+  const int k = clk::find(key);
+  if (k < 0) return CLK_INVALID_VALUE;
+  if (k == clk::TUNE_BATCH_SEG && (value < 8 || value > (1 << 20))) return CLK_INVALID_VALUE;
+  if (k == clk::TUNE_STRIDED_ROUNDS && (value < 1 || value > 64)) return CLK_INVALID_VALUE;
+  clk::g_val[k].store(value);
+  return CLK_SUCCESS;
+}
+
+extern "C" int32_t clk_get_tuning(const char* key, int32_t* value) {
+  std::call_once(clk::g_once, clk::init);
+  const int k = clk::find(key);
+  if (k < 0 || !value) return CLK_INVALID_VALUE;
+  *value = clk::g_val[k].load();
+  return CLK_SUCCESS;
+}

@@ -99,11 +99,14 @@ int32_t chain3(int op, const char* name, void* stream, const clk_image* dst, con
   void* tmp[2] = {nullptr, nullptr};
   const int ntmp = nact - 1 < 2 ? nact - 1 : 2;
   for (int i = 0; i < ntmp; ++i) {
-    if (V > 0 && hipMallocAsync(&tmp[i], (size_t)V * sizeof(float), st) != hipSuccess) {
+    if (V > 0 && hipMallocAsync(&tmp[i], (size_t)V * clk_dtype_size((op == clk::OP_MIN || op == clk::OP_MAX) ? src->dtype : CLK_FLOAT), st) != hipSuccess) {
       for (int j = 0; j < i; ++j) (void)hipFreeAsync(tmp[j], st);
       return fail(CLK_MEM_OBJECT_ALLOCATION_FAILURE, std::string(name) + ": temporary allocation failed");
     }
   }
+  // temporaries: float for Gaussian and mean; the source type for min/max,
+  // whose values are exact in any type that holds the source
+  const int tdt = (op == clk::OP_MIN || op == clk::OP_MAX) ? src->dtype : CLK_FLOAT;
   clk_image cur = *src;
   int done = 0, ti = 0;
   rc = 0;
@@ -115,7 +118,7 @@ int32_t chain3(int op, const char* name, void* stream, const clk_image* dst, con
       out = *dst;
     } else {
       out = *dst;
-      out.dtype = CLK_FLOAT;
+      out.dtype = tdt;
       out.ptr = tmp[ti];
       ti ^= 1;
     }
@@ -269,6 +272,28 @@ int32_t clk_minimum_box_3d(void* stream, const clk_image* dst, const clk_image* 
 int32_t clk_maximum_box_3d(void* stream, const clk_image* dst, const clk_image* src, int32_t rx,
                            int32_t ry, int32_t rz) {
   return box_chain(clk::OP_MAX, "maximum_box_3d", stream, dst, src, rx, ry, rz);
+}
+
+int32_t clk_batch_filter_3d(void* stream, const clk_filter_job* jobs, int32_t njobs) {
+  if (njobs < 0 || (njobs > 0 && !jobs)) return fail(CLK_INVALID_VALUE, "clk_batch_filter_3d: invalid job list");
+  for (int32_t j = 0; j < njobs; ++j) {
+    const clk_filter_job& jb = jobs[j];
+    const std::string tag = "clk_batch_filter_3d: job " + std::to_string(j);
+    if (jb.kind < CLK_JOB_GAUSSIAN_BLUR_3D || jb.kind > CLK_JOB_MEDIAN_BOX_3D || jb.reserved != 0)
+      return fail(CLK_INVALID_VALUE, tag + ": invalid kind");
+    int32_t rc = check_pair(tag.c_str(), &jb.dst, &jb.src);
+    if (rc) return rc;
+    if (jb.kind != CLK_JOB_GAUSSIAN_BLUR_3D)
+      for (int a = 0; a < 3; ++a)
+        if (jb.radius[a] < 0) return fail(CLK_INVALID_VALUE, tag + ": negative radius");
+    if (jb.kind == CLK_JOB_GAUSSIAN_BLUR_3D)
+      for (int a = 0; a < 3; ++a)
+        if (jb.sigma[a] > 0 && sigma_to_n(jb.sigma[a]) > CLK_GAUSSIAN_MAX_TAPS)
+          return fail(CLK_INVALID_VALUE, tag + ": sigma too large");
+  }
+  std::string err;
+  const int32_t rc = clk::run_batch((hipStream_t)stream, jobs, njobs, &err);
+  return rc ? fail(rc, err) : ok();
 }
 
 int32_t clk_init(void) {

@@ -1,0 +1,345 @@
+// median_box_3d for gfx950.
+//
+// Reference: kernels/median_box_3d_x.cl:23-65.  Each output is the element
+// of rank n/2 (0-based, ascending) of the n = (2ex+1)(2ey+1)(2ez+1)
+// clamp-to-edge box values, each value taken as (dst_t)(float)src
+// (median_box_3d_x.cl:54-55), written without CONVERT (:64).  The reference
+// finds it with an insertion sort of a private array (:3-20); the value of
+// that element does not depend on how it is found, so the kernels here use
+// selection methods that fit registers and the VALU:
+//
+//  * 3x3x3 boxes: the 27 values sit in registers and a forgetful-selection
+//    network (repeatedly discard the minimum and maximum of a k+2 element
+//    buffer, then feed the next value) leaves the median; about 400 min/max
+//    per voxel, no memory traffic beyond the gather.
+//  * 8-bit dst and up to 255 values: each lane walks a z-column of outputs
+//    with a 256-bin histogram of its box in LDS (one byte per bin); a step
+//    in z removes the leaving plane, adds the entering one and moves the
+//    tracked median a few bins (Huang's running median, along z).  The
+//    bins of each lane are rotated by 4*lane so that lanes holding equal
+//    values hit different LDS banks.
+//  * any other box: a bisection over the dst_t value range counts, per probe,
+//    how many window values are <= the probe (8, 16 or 32 probes).  No array,
+//    so no MAX_ARRAY_SIZE limit.
+//
+// Integer pixel types are exact.  For float dst the value of the rank-n/2
+// element is reproduced; a zero median is returned as +0 and NaN inputs have
+// no defined order (the reference's insertion sort is order-dependent there).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdlib.h>
+
+#include "clk_device.h"
+#include "clk_internal.h"
+#include "clk_tuning.h"
+
+namespace clk {
+
+inline bool hist8_enabled() { return tuning(TUNE_NO_HIST8) == 0; }
+
+// (dst_t)(float)READ(src): C conversion of the float to the dst pixel type.
+// In range it truncates toward zero; out of range the reference's C
+// conversion is undefined and this one saturates (NaN -> 0).
+template <class TS, class TD>
+__device__ __forceinline__ float gather_value(TS x) {
+  return static_cast<float>(convert_sat<TD>(static_cast<float>(x)));
+}
+
+// Order-preserving integer key of a dst_t value (held in a float).
+template <class TD>
+__device__ __forceinline__ uint32_t value_key(float v) {
+  if constexpr (Pix<TD>::kInt) {
+    return static_cast<uint32_t>(static_cast<int32_t>(v) - static_cast<int32_t>(Pix<TD>::lo));
+  } else {
+    if (v == 0.f) v = 0.f;  // -0 and +0 are one value
+    const uint32_t u = __float_as_uint(v);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+  }
+}
+
+template <class TD>
+__device__ __forceinline__ float key_value(uint32_t k) {
+  if constexpr (Pix<TD>::kInt) {
+    return static_cast<float>(static_cast<int32_t>(k) + static_cast<int32_t>(Pix<TD>::lo));
+  } else {
+    const uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    return __uint_as_float(u);
+  }
+}
+
+template <class TD> struct KeyBits { static constexpr int v = sizeof(TD) * 8; };
+template <> struct KeyBits<float> { static constexpr int v = 32; };
+
+}  // namespace clk
+#include "clk_median.h"
+namespace clk {
+
+template <class TS, class TD>
+__device__ __forceinline__ void median_generic_body(const MedParams& p, uint32_t bx, uint32_t by, uint32_t bz) {
+  const int64_t x = (int64_t)bx * 256 + threadIdx.x;
+  const int64_t y = by;
+  const int64_t z = bz;
+  if (x >= p.W) return;
+  const TS* __restrict__ src = static_cast<const TS*>(p.src);
+  const int64_t k = p.n / 2;  // rank of the result
+  uint32_t lo = 0;
+  uint32_t hi = (KeyBits<TD>::v == 32) ? 0xffffffffu : ((1u << KeyBits<TD>::v) - 1u);
+  while (lo < hi) {
+    const uint32_t mid = lo + ((hi - lo) >> 1);
+    int64_t cnt = 0;
+    for (int dz = -p.ez; dz <= p.ez; ++dz) {
+      const int64_t zz = clamp64(z + dz, 0, p.D - 1);
+      for (int dy = -p.ey; dy <= p.ey; ++dy) {
+        const int64_t yy = clamp64(y + dy, 0, p.H - 1);
+        const TS* rowp = src + (zz * p.H + yy) * p.W;
+        for (int dx = -p.ex; dx <= p.ex; ++dx) {
+          const int64_t xx = clamp64(x + dx, 0, p.W - 1);
+          cnt += value_key<TD>(gather_value<TS, TD>(rowp[xx])) <= mid;
+        }
+      }
+    }
+    if (cnt >= k + 1) hi = mid; else lo = mid + 1;
+  }
+  static_cast<TD*>(p.dst)[(z * p.H + y) * p.W + x] = static_cast<TD>(key_value<TD>(lo));
+}
+
+// Median (rank N/2) of N = 2k+1 values held in registers by forgetful
+// selection: keep a buffer of k+2 values; its minimum and maximum cannot be
+// the median of the values seen so far plus the rest, so move them to the
+// buffer ends with compare-exchanges, drop both, and put the next unseen
+// value into the freed slot.  After k-1 rounds the median of the last 3 is
+// the answer.
+typedef unsigned short us2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ float vmin(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ float vmax(float a, float b) { return fmaxf(a, b); }
+// two 16-bit lanes per instruction (v_pk_min_u16 / v_pk_max_u16)
+__device__ __forceinline__ us2 vmin(us2 a, us2 b) { return __builtin_elementwise_min(a, b); }
+__device__ __forceinline__ us2 vmax(us2 a, us2 b) { return __builtin_elementwise_max(a, b); }
+
+template <int N, class T>
+__device__ __forceinline__ T forgetful_median(T (&a)[N]) {
+  constexpr int k = N / 2;
+  constexpr int m = k + 2;
+  constexpr int hi = m - 1;
+#pragma unroll
+  for (int step = 0; step < N - m; ++step) {
+    const int lo = step;
+#pragma unroll
+    for (int i = lo + 1; i <= hi; ++i) {
+      const T u = a[lo], w = a[i];
+      a[lo] = vmin(u, w);
+      a[i] = vmax(u, w);
+    }
+#pragma unroll
+    for (int i = lo + 1; i < hi; ++i) {
+      const T u = a[i], w = a[hi];
+      a[i] = vmin(u, w);
+      a[hi] = vmax(u, w);
+    }
+    a[hi] = a[m + step];
+  }
+  const T x0 = a[N - m], x1 = a[N - m + 1], x2 = a[hi];  // the last 3 (N - m + 2 == hi)
+  return vmax(vmin(x0, x1), vmin(vmax(x0, x1), x2));
+}
+
+// 3x3x3 median, 8-bit dst: a block filters 512 consecutive x of one (y, z).
+// The 9 source rows (x-1 .. x+512, clamped) are staged in LDS as 16-bit
+// value pairs twice, pairs starting at even and at odd positions, so the
+// three pairs a thread needs per row (its two outputs shifted by -1, 0, +1)
+// are three aligned 32-bit reads; the selection network then runs on two
+// voxels per instruction.
+template <class TS, class TD>
+__device__ __forceinline__ void median_27_pk8_body(const MedParams& p, uint32_t bx, uint32_t by, uint32_t bz) {
+  __shared__ uint32_t pe[9][257];
+  __shared__ uint32_t po[9][256];
+  const int t = threadIdx.x;
+  const int64_t xt = (int64_t)bx * 512;
+  const int64_t y = by, z = bz;
+  const int W = (int)p.W;
+  const TS* __restrict__ src = static_cast<const TS*>(p.src);
+  constexpr int kBias = -static_cast<int>(Pix<TD>::lo);  // i8 values shift to 0..255
+  auto val = [&](const TS* row, int64_t x) -> uint32_t {
+    x = x < 0 ? 0 : (x >= W ? W - 1 : x);
+    return (uint32_t)((int)gather_value<TS, TD>(row[x]) + kBias);
+  };
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    const int64_t zz = clamp64(z + r / 3 - 1, 0, p.D - 1);
+    const int64_t yy = clamp64(y + r % 3 - 1, 0, p.H - 1);
+    const TS* row = src + (zz * p.H + yy) * p.W;
+    // v[i] = value at x = xt - 1 + i
+    const int64_t xb = xt - 1 + 2 * t;
+    const uint32_t v0 = val(row, xb), v1 = val(row, xb + 1), v2 = val(row, xb + 2);
+    pe[r][t] = v0 | (v1 << 16);
+    po[r][t] = v1 | (v2 << 16);
+    if (t == 255) pe[r][256] = v2 | (val(row, xb + 3) << 16);
+  }
+  __syncthreads();
+  us2 a[27];
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    const uint32_t w0 = pe[r][t], w1 = po[r][t], w2 = pe[r][t + 1];
+    __builtin_memcpy(&a[3 * r + 0], &w0, 4);
+    __builtin_memcpy(&a[3 * r + 1], &w1, 4);
+    __builtin_memcpy(&a[3 * r + 2], &w2, 4);
+  }
+  const us2 m = forgetful_median<27>(a);
+  const int64_t x = xt + 2 * t;
+  TD* __restrict__ dst = static_cast<TD*>(p.dst) + (z * p.H + y) * p.W;
+  if (x < W) dst[x] = static_cast<TD>((int)m.x - kBias);
+  if (x + 1 < W) dst[x + 1] = static_cast<TD>((int)m.y - kBias);
+}
+
+template <class TS, class TD>
+__device__ __forceinline__ void median_27_body(const MedParams& p, uint32_t bx, uint32_t by, uint32_t bz) {
+  const int64_t x = (int64_t)bx * 256 + threadIdx.x;
+  const int64_t y = by;
+  const int64_t z = bz;
+  if (x >= p.W) return;
+  const TS* __restrict__ src = static_cast<const TS*>(p.src);
+  float a[27];
+  int i = 0;
+#pragma unroll
+  for (int dz = -1; dz <= 1; ++dz) {
+    const int64_t zz = clamp64(z + dz, 0, p.D - 1);
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int64_t yy = clamp64(y + dy, 0, p.H - 1);
+      const TS* rowp = src + (zz * p.H + yy) * p.W;
+#pragma unroll
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int64_t xx = clamp64(x + dx, 0, p.W - 1);
+        a[i++] = gather_value<TS, TD>(rowp[xx]);
+      }
+    }
+  }
+  float m = forgetful_median<27, float>(a);
+  if constexpr (!Pix<TD>::kInt) {
+    if (m == 0.f) m = 0.f;
+  }
+  static_cast<TD*>(p.dst)[(z * p.H + y) * p.W + x] = static_cast<TD>(m);
+}
+
+// 8-bit dst: histogram bin of a gathered value.
+template <class TD>
+__device__ __forceinline__ int bin8(float v) {
+  return static_cast<int>(v) - static_cast<int>(Pix<TD>::lo);
+}
+
+constexpr int kHistZSeg = 32;  // outputs per lane along z
+
+template <class TS, class TD>
+__device__ __forceinline__ void median_hist8_body(const MedParams& p, uint32_t bx, uint32_t by, uint32_t bz) {
+  // 256 byte counters per lane packed four to a dword; dword d of lane l is
+  // stored at l*64 + ((d + l) & 63) so equal values of different lanes fall
+  // in different banks.  Counters change by LDS atomic adds whose result is
+  // not returned (no dependency chain between updates); -1 on a byte is
+  // +(2^32 - 2^(8j)), which cannot borrow because a count never goes below 0.
+  __shared__ uint32_t hist[64 * 64];
+  const int lane = threadIdx.x;
+  const int64_t x = (int64_t)bx * 64 + lane;
+  const int64_t y = by;
+  const int64_t z0 = (int64_t)bz * kHistZSeg;
+  const int64_t z1 = (z0 + kHistZSeg < p.D) ? z0 + kHistZSeg : p.D;
+  if (x >= p.W) return;  // lanes never share data: no barrier follows
+  uint32_t* const h = hist + lane * 64;
+#pragma unroll 8
+  for (int i = 0; i < 64; ++i) h[i] = 0u;
+  const TS* __restrict__ src = static_cast<const TS*>(p.src);
+  const int ex = p.ex, ey = p.ey, ez = p.ez;
+  const int W = (int)p.W;
+  const int xi = (int)x;
+  // row pointer of a (z, y) row: uniform across the wave
+  auto rowp = [&](int64_t zz, int64_t yy) -> const TS* {
+    zz = clamp64(zz, 0, p.D - 1);
+    yy = clamp64(yy, 0, p.H - 1);
+    return src + (zz * p.H + yy) * p.W;
+  };
+  auto bin_at = [&](const TS* row, int dx) -> int {
+    int xx = xi + dx;
+    xx = xx < 0 ? 0 : (xx >= W ? W - 1 : xx);
+    return bin8<TD>(gather_value<TS, TD>(row[xx]));
+  };
+  auto word = [&](int b) -> uint32_t* { return h + (((b >> 2) + lane) & 63); };
+  auto add = [&](int b, bool inc) {
+    const uint32_t one = 1u << ((b & 3) * 8);
+    __hip_atomic_fetch_add(word(b), inc ? one : 0u - one, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  };
+  auto count = [&](int b) -> int { return (int)((*word(b) >> ((b & 3) * 8)) & 255u); };
+  const int k = p.n / 2;
+  for (int dz = -ez; dz <= ez; ++dz)
+    for (int dy = -ey; dy <= ey; ++dy) {
+      const TS* row = rowp(z0 + dz, y + dy);
+      for (int dx = -ex; dx <= ex; ++dx) add(bin_at(row, dx), true);
+    }
+  int med = 0, lt = 0;
+  for (int c = count(0); lt + c <= k; c = count(med)) {
+    lt += c;
+    ++med;
+  }
+  TD* __restrict__ dst = static_cast<TD*>(p.dst);
+  const int64_t plane = p.H * p.W;
+  dst[z0 * plane + y * p.W + x] = static_cast<TD>(med + static_cast<int>(Pix<TD>::lo));
+  for (int64_t z = z0 + 1; z < z1; ++z) {
+    const int64_t zo = z - 1 - ez, zi = z + ez;
+    for (int dy = -ey; dy <= ey; ++dy) {
+      const TS* ro = rowp(zo, y + dy);
+      const TS* ri = rowp(zi, y + dy);
+      for (int dx = -ex; dx <= ex; ++dx) {
+        const int a = bin_at(ro, dx);
+        const int b = bin_at(ri, dx);
+        add(a, false);
+        add(b, true);
+        lt += (b < med) - (a < med);
+      }
+    }
+    // median = the bin m with lt(m) <= k < lt(m) + count(m)
+    while (lt > k) {
+      --med;
+      lt -= count(med);
+    }
+    for (int c = count(med); lt + c <= k; c = count(med)) {
+      lt += c;
+      ++med;
+    }
+    dst[z * plane + y * p.W + x] = static_cast<TD>(med + static_cast<int>(Pix<TD>::lo));
+  }
+}
+
+#define CLK_MEDIAN_KERNEL(NAME, THREADS)                                              \
+  template <class TS, class TD>                                                      \
+  __global__ __launch_bounds__(THREADS) void NAME##_kernel(const MedParams p) {      \
+    NAME##_body<TS, TD>(p, blockIdx.x, blockIdx.y, blockIdx.z);                      \
+  }
+CLK_MEDIAN_KERNEL(median_generic, 256)
+CLK_MEDIAN_KERNEL(median_27_pk8, 256)
+CLK_MEDIAN_KERNEL(median_27, 256)
+CLK_MEDIAN_KERNEL(median_hist8, 64)
+#undef CLK_MEDIAN_KERNEL
+
+enum MedKind : int { MED_PK8 = 0, MED_HIST8 = 1, MED_27 = 2, MED_GENERIC = 3 };
+
+// Kernel and grid for one median launch; sets p.gx, p.gy.
+inline MedKind plan_median(MedParams& p, int td_size, dim3* grid, int* threads) {
+  MedKind k;
+  if (td_size == 1 && p.ex == 1 && p.ey == 1 && p.ez == 1) {
+    k = MED_PK8;
+    *grid = dim3((uint32_t)((p.W + 511) / 512), (uint32_t)p.H, (uint32_t)p.D);
+    *threads = 256;
+  } else if (td_size == 1 && p.n <= 255 && hist8_enabled()) {
+    k = MED_HIST8;
+    *grid = dim3((uint32_t)((p.W + 63) / 64), (uint32_t)p.H, (uint32_t)((p.D + kHistZSeg - 1) / kHistZSeg));
+    *threads = 64;
+  } else {
+    k = (p.ex == 1 && p.ey == 1 && p.ez == 1) ? MED_27 : MED_GENERIC;
+    *grid = dim3((uint32_t)((p.W + 255) / 256), (uint32_t)p.H, (uint32_t)p.D);
+    *threads = 256;
+  }
+  p.gx = grid->x;
+  p.gy = grid->y;
+  return k;
+}
+
+}  // namespace clk

@@ -45,6 +45,7 @@
 
 #include "clk_device.h"
 #include "clk_internal.h"
+#include "clk_tuning.h"
 
 namespace clk {
 
@@ -107,11 +108,11 @@ __device__ __forceinline__ TD finish(VT acc, const SepParams& p) {
 
 // ---- y / z (strided) pass ---------------------------------------------------
 template <int OP, class TS, class TD, int K, int COLS>
-__global__ __launch_bounds__(256) void sep_strided_kernel(const SepParams p) {
+__device__ __forceinline__ void sep_strided_body(const SepParams& p, const uint32_t vblk) {
   using VT = vt_t<OP, TS>;
   constexpr bool kFloatSrc = !Pix<TS>::kInt;
-  const uint32_t sid = blockIdx.x / p.nblk;
-  const uint32_t b = blockIdx.x - sid * p.nblk;
+  const uint32_t sid = vblk / p.nblk;
+  const uint32_t b = vblk - sid * p.nblk;
   const int64_t g = (int64_t)b * 256 + threadIdx.x;
   if (g >= p.ncols_total) return;
   const int64_t o = g / p.ncolgroups;
@@ -208,6 +209,11 @@ __global__ __launch_bounds__(256) void sep_strided_kernel(const SepParams p) {
   }
 }
 
+template <int OP, class TS, class TD, int K, int COLS>
+__global__ __launch_bounds__(256) void sep_strided_kernel(const SepParams p) {
+  sep_strided_body<OP, TS, TD, K, COLS>(p, blockIdx.x);
+}
+
 // ---- x pass (contiguous axis, LDS-staged row segments) ----------------------
 constexpr int kXK = 8;            // outputs per lane per sub-segment
 constexpr int kXSeg = 64 * kXK;   // outputs per sub-segment (one wave-wide vector load)
@@ -224,19 +230,18 @@ namespace clk {
 // LDS, then SUB blocks of 8 outputs per lane through the register ring.  The
 // grid is one wave per task, in address order (no persistent loop).
 template <int OP, class TS, class TD, int SUB>
-__global__ __launch_bounds__(256) void sep_x_kernel(const SepParams p) {
+__device__ __forceinline__ void sep_x_body(const SepParams& p, const uint32_t vblk, uint32_t* lds_raw) {
   using VT = vt_t<OP, TS>;
   constexpr bool kFloatSrc = !Pix<TS>::kInt;
   constexpr int K = kXK;
   constexpr int TSEG = kXSeg * SUB;
-  extern __shared__ uint32_t lds_raw[];
   const int wv = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   VT* const buf = reinterpret_cast<VT*>(lds_raw) + wv * p.lds_stride;
   const int64_t W = p.L;
   const int c = p.c;
   const int ntaps = p.ntaps;
-  const uint32_t task = blockIdx.x * 4u + (uint32_t)wv;
+  const uint32_t task = vblk * 4u + (uint32_t)wv;
   const bool valid = task < (uint32_t)p.ntasks;
   const uint32_t row = task / (uint32_t)p.nseg;
   const int64_t x0 = (int64_t)(task - row * (uint32_t)p.nseg) * TSEG;
@@ -318,6 +323,12 @@ __global__ __launch_bounds__(256) void sep_x_kernel(const SepParams p) {
   }
 }
 
+template <int OP, class TS, class TD, int SUB>
+__global__ __launch_bounds__(256) void sep_x_kernel(const SepParams p) {
+  extern __shared__ uint32_t lds_raw[];
+  sep_x_body<OP, TS, TD, SUB>(p, blockIdx.x, lds_raw);
+}
+
 // ---- host-side launch of one (op, src type, dst type) ----------------------
 
 // Resident blocks of a kernel on the current device, cached per kernel.
@@ -345,13 +356,9 @@ inline int resident_blocks(const void* kernel, size_t shmem) {
 // Blocks of 256 threads per resident slot that a launch aims for; a few
 // rounds of equal-sized blocks keep every CU busy to the end.  CLK_ROUNDS
 // and CLK_STRIDED_VARIANT override the defaults (used when tuning).
-inline int env_int(const char* name, int dflt) {
-  const char* v = getenv(name);
-  return (v && *v) ? atoi(v) : dflt;
-}
-inline int strided_rounds() { static const int r = env_int("CLK_ROUNDS", 2); return r > 0 ? r : 2; }
-inline bool no_xreg() { static const bool v = env_int("CLK_NO_XREG", 0) != 0; return v; }
-inline int strided_variant() { static const int v = env_int("CLK_STRIDED_VARIANT", 0); return v; }
+inline int strided_rounds() { const int r = tuning(TUNE_STRIDED_ROUNDS); return r > 0 ? r : 2; }
+inline bool no_xreg() { return tuning(TUNE_NO_XREG) != 0; }
+inline int strided_variant() { return tuning(TUNE_STRIDED_VARIANT); }
 
 template <int OP, class TS, class TD, int K, int COLS>
 int32_t launch_strided(hipStream_t st, SepParams& p, std::string* err) {
@@ -492,8 +499,7 @@ int32_t launch_sep_typed(hipStream_t st, SepParams& p, int axis_is_x, std::strin
     const int64_t W = p.L;
     // larger tasks for long rows and narrow halos (more loads in flight per
     // wave, less per-task work); CLK_XSUB overrides when tuning
-    static const int sub_env = env_int("CLK_XSUB", 0);
-    int sub = sub_env;
+    int sub = tuning(TUNE_X_SUB);
     if (sub != 1 && sub != 2 && sub != 4) sub = (W >= 4 * kXSeg && p.c <= 256) ? 2 : 1;
     const int tseg = kXSeg * sub;
     p.nseg = (int32_t)((W + tseg - 1) / tseg);

@@ -94,32 +94,33 @@ static int32_t fill_nan(hipStream_t st, const clk_image& dst, std::string* err) 
   return 0;
 }
 
-int32_t launch_separable(int op, hipStream_t st, const clk_image& dst, const clk_image& src,
-                         int32_t dim, int32_t N, float s, std::string* err) {
+int32_t prepare_separable(int op, const clk_image& dst, const clk_image& src, int32_t dim, int32_t N,
+                          float s, SepParams& p, int* axis_is_x, bool* fill_nan_out, std::string* err) {
   const int64_t W = dst.width, H = dst.height, D = dst.depth;
-  if (W * H * D == 0) return 0;
+  *fill_nan_out = false;
   int c = (N - 1) / 2;  // C division truncates toward zero, as the kernels' (N-1)/2
   if (c < 0) {
     // empty tap loop: min/max keep the centre (== a one-tap window);
     // mean and Gaussian divide 0 by 0.
-    if (op == OP_MIN || op == OP_MAX) c = 0;
-    else return fill_nan(st, dst, err);
+    if (op == OP_MIN || op == OP_MAX) {
+      c = 0;
+    } else {
+      *fill_nan_out = true;
+      return 0;
+    }
   }
   if (op == OP_GAUSS && 2 * (int64_t)c + 1 > kMaxTaps) {
     *err = "gaussian_blur_separable_3d: N exceeds CLK_GAUSSIAN_MAX_TAPS";
     return CLK_INVALID_VALUE;
   }
-  SepParams* pp = new (std::nothrow) SepParams;
-  if (!pp) { *err = "host allocation failed"; return CLK_OUT_OF_RESOURCES; }
-  SepParams& p = *pp;
   memset(&p, 0, sizeof(p));
   p.src = src.ptr;
   p.dst = dst.ptr;
   p.c = c;
   p.ntaps = 2 * c + 1;
-  int axis_is_x = 0;
+  *axis_is_x = 0;
   switch (dim) {
-    case 0: p.inner = 1; p.L = W; p.outer = H * D; axis_is_x = 1; break;
+    case 0: p.inner = 1; p.L = W; p.outer = H * D; *axis_is_x = 1; break;
     case 1: p.inner = W; p.L = H; p.outer = D; break;
     case 2: p.inner = W * H; p.L = D; p.outer = 1; break;
     default: p.inner = W * H * D; p.L = 1; p.outer = 1; break;  // dir = (0,0,0)
@@ -145,13 +146,33 @@ int32_t launch_separable(int op, hipStream_t st, const clk_image& dst, const clk
     p.rdiv = 1.0f / p.div;
     p.fastdiv = (src.dtype != CLK_FLOAT);  // integer sums: 0 or >= 1 in magnitude
   }
-  int32_t rc;
-  switch (op) {
-    case OP_GAUSS: rc = launch_sep_gauss(st, p, axis_is_x, src.dtype, dst.dtype, err); break;
-    case OP_MEAN:  rc = launch_sep_mean(st, p, axis_is_x, src.dtype, dst.dtype, err); break;
-    case OP_MIN:   rc = launch_sep_min(st, p, axis_is_x, src.dtype, dst.dtype, err); break;
-    case OP_MAX:   rc = launch_sep_max(st, p, axis_is_x, src.dtype, dst.dtype, err); break;
-    default: *err = "invalid op"; rc = CLK_INVALID_VALUE; break;
+  return 0;
+}
+
+int32_t launch_fill_nan(hipStream_t st, const clk_image& dst, std::string* err) {
+  return fill_nan(st, dst, err);
+}
+
+int32_t launch_separable(int op, hipStream_t st, const clk_image& dst, const clk_image& src,
+                         int32_t dim, int32_t N, float s, std::string* err) {
+  const int64_t W = dst.width, H = dst.height, D = dst.depth;
+  if (W * H * D == 0) return 0;
+  SepParams* pp = new (std::nothrow) SepParams;
+  if (!pp) { *err = "host allocation failed"; return CLK_OUT_OF_RESOURCES; }
+  SepParams& p = *pp;
+  int axis_is_x = 0;
+  bool nan_fill = false;
+  int32_t rc = prepare_separable(op, dst, src, dim, N, s, p, &axis_is_x, &nan_fill, err);
+  if (rc == 0 && nan_fill) {
+    rc = fill_nan(st, dst, err);
+  } else if (rc == 0) {
+    switch (op) {
+      case OP_GAUSS: rc = launch_sep_gauss(st, p, axis_is_x, src.dtype, dst.dtype, err); break;
+      case OP_MEAN:  rc = launch_sep_mean(st, p, axis_is_x, src.dtype, dst.dtype, err); break;
+      case OP_MIN:   rc = launch_sep_min(st, p, axis_is_x, src.dtype, dst.dtype, err); break;
+      case OP_MAX:   rc = launch_sep_max(st, p, axis_is_x, src.dtype, dst.dtype, err); break;
+      default: *err = "invalid op"; rc = CLK_INVALID_VALUE; break;
+    }
   }
   delete pp;
   return rc;

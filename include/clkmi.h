@@ -149,6 +149,46 @@ int32_t clk_maximum_box_3d(void* stream, const clk_image* dst,
                            const clk_image* src, int32_t rx, int32_t ry,
                            int32_t rz);
 
+/* ---- batches: many 3D filters in a few launches ------------------------ */
+
+/* One 3D filter of a batch, with the semantics of the matching single call:
+ *   CLK_JOB_GAUSSIAN_BLUR_3D  clk_gaussian_blur_3d(dst, src, sigma[0..2])
+ *   CLK_JOB_MEAN_BOX_3D       clk_mean_box_3d(dst, src, radius[0..2])
+ *   CLK_JOB_MINIMUM_BOX_3D    clk_minimum_box_3d(...)
+ *   CLK_JOB_MAXIMUM_BOX_3D    clk_maximum_box_3d(...)
+ *   CLK_JOB_MEDIAN_BOX_3D     clk_median_box_3d(dst, src, 2r+1 per axis)  */
+enum {
+  CLK_JOB_GAUSSIAN_BLUR_3D = 0,
+  CLK_JOB_MEAN_BOX_3D = 1,
+  CLK_JOB_MINIMUM_BOX_3D = 2,
+  CLK_JOB_MAXIMUM_BOX_3D = 3,
+  CLK_JOB_MEDIAN_BOX_3D = 4
+};
+
+typedef struct clk_filter_job {
+  int32_t   kind;       /* CLK_JOB_* */
+  int32_t   reserved;   /* must be 0 */
+  clk_image dst;
+  clk_image src;
+  float     sigma[3];   /* Gaussian, per axis x,y,z */
+  int32_t   radius[3];  /* box filters and median, per axis */
+} clk_filter_job;
+
+/* Run njobs independent 3D filters (volumes of any sizes and pixel types) as
+ * one batch: the single-axis passes of all jobs run stage by stage, with one
+ * kernel launch per kernel class and stage instead of one per volume and
+ * axis.  Results equal those of the single calls bit for bit.  The batch's
+ * temporaries and launch tables are stream-ordered allocations freed when
+ * the stream passes the batch.  No reference counterpart: the reference
+ * leaves launching to its host. */
+int32_t clk_batch_filter_3d(void* stream, const clk_filter_job* jobs, int32_t njobs);
+
+/* Launch-shape knobs ("strided_rounds", "x_sub", "batch_seg", "no_xreg",
+ * "strided_variant", "no_hist8"); the autotuner sets them.  Results never
+ * depend on them. */
+int32_t clk_set_tuning(const char* key, int32_t value);
+int32_t clk_get_tuning(const char* key, int32_t* value);
+
 /* ---- device memory and streams (for hosts without their own HIP) ------- */
 
 int32_t clk_init(void);

@@ -62,3 +62,28 @@ def test_unbuilt_library_fails_loudly(tmp_path, monkeypatch):
     monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "missing.so"))
     with pytest.raises(RuntimeError, match="not built"):
         _lib.lib()
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """ctypes mirrors of clk_image, clk_arg and clk_filter_job have the C
+    sizes and offsets (compiled with gcc from include/clkmi.h)."""
+    import subprocess
+    from clkmi import _lib
+    src = tmp_path / "layout.c"
+    src.write_text('''#include <stdio.h>
+#include <stddef.h>
+#include "clkmi.h"
+int main(void) {
+  printf("%zu %zu %zu %zu\\n", sizeof(clk_image), sizeof(clk_arg), offsetof(clk_arg, image), sizeof(clk_filter_job));
+  printf("%zu %zu %zu %zu\\n", offsetof(clk_filter_job, dst), offsetof(clk_filter_job, src),
+         offsetof(clk_filter_job, sigma), offsetof(clk_filter_job, radius));
+  return 0;
+}
+''')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    want = [ctypes.sizeof(_lib.ClkImage), ctypes.sizeof(_lib.ClkArg), _lib.ClkArg.image.offset,
+            ctypes.sizeof(_lib.ClkFilterJob), _lib.ClkFilterJob.dst.offset, _lib.ClkFilterJob.src.offset,
+            _lib.ClkFilterJob.sigma.offset, _lib.ClkFilterJob.radius.offset]
+    assert [int(v) for v in out] == want

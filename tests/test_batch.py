@@ -1,0 +1,127 @@
+"""Batched execution (clk_batch_filter_3d): every job's result equals the
+single call's and the oracle chain's, bit for bit, whatever the mix of
+volume sizes, pixel types, ops and tuning."""
+import numpy as np
+import pytest
+
+import clk_oracle as O
+from helpers import empty_device, from_device, random_volume, same_bits_or_value, to_device
+
+pytestmark = pytest.mark.gpu
+
+SINGLE = {"gaussian_blur_3d": None, "mean_box_3d": O.OP_MEAN, "minimum_box_3d": O.OP_MIN,
+          "maximum_box_3d": O.OP_MAX}
+
+
+def oracle_job(kind, src, dst_t, sigma, radius):
+    if kind == "gaussian_blur_3d":
+        return O.gaussian_blur_3d(src, dst_t, *sigma)
+    if kind == "median_box_3d":
+        return O.median_box_3d(src, dst_t, *(2 * r + 1 for r in radius))
+    return O.box_3d(SINGLE[kind], src, dst_t, *radius)
+
+
+def make_jobs(rng, n, kinds, pairs, lo=5, hi=40):
+    jobs = []
+    for i in range(n):
+        shape = tuple(int(v) for v in rng.integers(lo, hi + 1, 3))
+        kind = kinds[i % len(kinds)]
+        src_t, dst_t = pairs[rng.integers(len(pairs))]
+        if kind == "median_box_3d" and src_t == np.float32 and dst_t != np.float32:
+            dst_t = np.float32
+        src = random_volume(shape, src_t, seed=1000 + i, lo=0 if np.dtype(src_t).kind == "u" else None,
+                            hi=3000 if np.dtype(src_t).itemsize >= 2 and np.dtype(src_t).kind != "f" else None)
+        sigma = tuple(float(rng.choice([0.0, rng.uniform(0.5, 4.0)], p=[0.1, 0.9])) for _ in range(3))
+        radius = tuple(int(rng.integers(0 if kind != "median_box_3d" else 1, 6 if kind != "median_box_3d" else 3))
+                       for _ in range(3))
+        jobs.append((kind, src, dst_t, sigma, radius))
+    return jobs
+
+
+def run_batch(jobs):
+    import clkmi
+    keep, descs, outs = [], [], []
+    for kind, src, dst_t, sigma, radius in jobs:
+        ts, si = to_device(src)
+        td, di = empty_device(src.shape, dst_t)
+        keep += [ts, td]
+        outs.append((td, src.shape, dst_t))
+        descs.append(clkmi.FilterJob(kind, di, si, sigma, radius))
+    clkmi.batch_filter_3d(descs)
+    return [from_device(*o) for o in outs]
+
+
+def run_single(job):
+    import clkmi
+    kind, src, dst_t, sigma, radius = job
+    ts, si = to_device(src)
+    td, di = empty_device(src.shape, dst_t)
+    if kind == "gaussian_blur_3d":
+        clkmi.gaussian_blur_3d(si, di, *sigma)
+    elif kind == "median_box_3d":
+        clkmi.median_box_3d(di, si, *(2 * r + 1 for r in radius))
+    else:
+        getattr(clkmi, kind)(si, di, *radius)
+    return from_device(td, src.shape, dst_t)
+
+
+PAIRS = [(np.uint16, np.uint16), (np.uint16, np.float32), (np.float32, np.float32), (np.uint8, np.uint8),
+         (np.int16, np.float32), (np.float32, np.uint16), (np.int8, np.int8), (np.uint16, np.uint8)]
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_batch_matches_single_calls_and_oracle(gpu, seed):
+    rng = np.random.default_rng(seed)
+    jobs = make_jobs(rng, 40, ["gaussian_blur_3d", "minimum_box_3d", "maximum_box_3d", "mean_box_3d",
+                               "median_box_3d"], PAIRS)
+    got = run_batch(jobs)
+    for i, (job, g) in enumerate(zip(jobs, got)):
+        want = oracle_job(job[0], job[1], job[2], job[3], job[4])
+        assert same_bits_or_value(g, want), "job %d %s %s" % (i, job[0], job[1].shape)
+        assert same_bits_or_value(g, run_single(job)), "job %d single call" % i
+
+
+@pytest.mark.parametrize("seg", [8, 32, 256])
+def test_batch_tuning_does_not_change_results(gpu, seg):
+    import clkmi
+    rng = np.random.default_rng(7)
+    jobs = make_jobs(rng, 12, ["gaussian_blur_3d", "maximum_box_3d"], PAIRS[:3], lo=20, hi=70)
+    old = clkmi.get_tuning("batch_seg")
+    try:
+        clkmi.set_tuning("batch_seg", seg)
+        got = run_batch(jobs)
+    finally:
+        clkmi.set_tuning("batch_seg", old)
+    for job, g in zip(jobs, got):
+        assert same_bits_or_value(g, oracle_job(*job))
+
+
+def test_batch_c4_shaped_workload(gpu):
+    """The uneven-batch workload at reduced count: uint16 volumes with axes
+    in [32, 256), Gaussian sigma in [0.5, 4] -> float32, min/max r 1..5."""
+    rng = np.random.default_rng(4)
+    jobs = []
+    for i in range(12):
+        shape = tuple(int(v) for v in rng.integers(32, 120, 3))
+        src = rng.poisson(200, size=shape).astype(np.uint16)
+        k = ["gaussian_blur_3d", "minimum_box_3d", "maximum_box_3d"][i % 3]
+        sigma = tuple(float(v) for v in rng.uniform(0.5, 4.0, 3))
+        r = int(rng.integers(1, 6))
+        jobs.append((k, src, np.float32 if k.startswith("g") else np.uint16, sigma, (r, r, r)))
+    got = run_batch(jobs)
+    for job, g in zip(jobs, got):
+        assert same_bits_or_value(g, oracle_job(*job))
+
+
+def test_batch_errors(gpu):
+    import clkmi
+    src = random_volume((4, 5, 6), np.uint8, seed=0)
+    ts, si = to_device(src)
+    td, di = empty_device((4, 5, 6), np.uint8)
+    with pytest.raises(clkmi.ClkError) as e:
+        clkmi.batch_filter_3d([clkmi.FilterJob("maximum_box_3d", si, si, radius=(1, 1, 1))])
+    assert e.value.code == -8
+    with pytest.raises(clkmi.ClkError) as e:
+        clkmi.batch_filter_3d([clkmi.FilterJob("minimum_box_3d", di, si, radius=(1, -1, 1))])
+    assert e.value.code == -30
+    clkmi.batch_filter_3d([])
