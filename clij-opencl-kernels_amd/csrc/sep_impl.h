@@ -456,7 +456,7 @@ int32_t launch_ystatic(hipStream_t st, SepParams& p, bool v4, std::string* err) 
 
 template <int OP, class TS, class TD, int C>
 void launch_xreg_c(hipStream_t st, const SepParams& p) {
-  const uint32_t blocks = (uint32_t)((p.ntasks + 3) / 4);
+  const uint32_t blocks = (uint32_t)xreg_blocks(p);
   hipLaunchKernelGGL((sep_xreg_kernel<OP, TS, TD, C>), dim3(blocks), dim3(256), 0, st, p);
 }
 
@@ -473,6 +473,18 @@ int32_t launch_xreg(hipStream_t st, const SepParams& p, std::string* err) {
       case 7: launch_xreg_c<OP, TS, TD, 7>(st, p); return 0;
       case 8: launch_xreg_c<OP, TS, TD, 8>(st, p); return 0;
     }
+    if constexpr (OP == OP_GAUSS) {
+      switch (p.c) {
+        case 9: launch_xreg_c<OP, TS, TD, 9>(st, p); return 0;
+        case 10: launch_xreg_c<OP, TS, TD, 10>(st, p); return 0;
+        case 11: launch_xreg_c<OP, TS, TD, 11>(st, p); return 0;
+        case 12: launch_xreg_c<OP, TS, TD, 12>(st, p); return 0;
+        case 13: launch_xreg_c<OP, TS, TD, 13>(st, p); return 0;
+        case 14: launch_xreg_c<OP, TS, TD, 14>(st, p); return 0;
+        case 15: launch_xreg_c<OP, TS, TD, 15>(st, p); return 0;
+        case 16: launch_xreg_c<OP, TS, TD, 16>(st, p); return 0;
+      }
+    }
   }
   *err = "internal: no register-window x kernel for this half width";
   return CLK_INVALID_VALUE;
@@ -484,13 +496,13 @@ int32_t launch_sep_typed(hipStream_t st, SepParams& p, int axis_is_x, std::strin
   if (V == 0) return 0;
   int32_t rc = 0;
   constexpr bool kXRegPair = std::is_same<TS, TD>::value || std::is_same<TD, float>::value;
-  const int xreg_maxc = 8;  // wider Gaussian windows run faster through LDS
+  // wider windows (Gaussian only) on short rows, where a 512-output LDS
+  // segment would leave lanes idle; long rows stage through LDS
+  const int xreg_maxc = (OP == OP_GAUSS && p.L <= 256) ? 16 : 8;
   if (kXRegPair && axis_is_x && p.c >= 1 && p.c <= xreg_maxc && p.L % kXK == 0 &&
       (uintptr_t)p.src % (kXK * sizeof(TS)) == 0 && (uintptr_t)p.dst % (kXK * sizeof(TD)) == 0 &&
       !no_xreg()) {
-    p.nseg = (int32_t)((p.L + kXSeg - 1) / kXSeg);
-    p.ntasks = p.outer * p.nseg;
-    if (p.ntasks > 0x7ffffff0LL) {
+    if (xreg_blocks(p) > 0x7fffffffLL) {
       *err = "volume too large for the x pass grid";
       return CLK_INVALID_VALUE;
     }

@@ -1,17 +1,15 @@
 // x pass for small half widths: the whole window of a lane lives in
 // registers.  Included by sep_impl.h.
 //
-// A wave filters 512 consecutive outputs of one row; lane l owns outputs
-// x0+8l .. x0+8l+7 and loads exactly that 8-pixel chunk with one vector
-// load.  The c pixels it needs on each side are the neighbouring lanes'
+// Lane l of a wave owns 8 consecutive outputs (one 8-pixel chunk) and loads
+// exactly that chunk with one vector load.  The c pixels it needs on each side are the neighbouring lanes'
 // chunks, moved across with DPP wave shifts (wave_shr:1 gives lane l the
 // value of lane l-1, wave_shl:1 that of lane l+1); only lanes 0 and 63
 // load halo chunks, which enter through the shift's "old" operand.  With
 // the half width C a template parameter every tap is a static register, so
 // there is no LDS, no barrier and no per-element address arithmetic.
-// Clamp-to-edge: a chunk that starts at or beyond the row end is the last
-// pixel repeated, one that starts before 0 the first pixel; the fast path
-// requires W % 8 == 0 and aligned rows, so chunks never straddle the end.
+// The fast path requires W % 8 == 0 and aligned rows, so chunks never
+// straddle a row end.
 #pragma once
 
 namespace clk {
@@ -54,45 +52,91 @@ __device__ __forceinline__ void load_chunk(const TS* __restrict__ row, int64_t x
 template <int OP, class TS>
 using xr_vt = std::conditional_t<(OP != OP_GAUSS && Pix<TS>::kInt), int32_t, float>;
 
+template <class TS>
+__device__ __forceinline__ void fill_chunk(TS v, Chunk<TS>& c) {
+  TS t[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) t[i] = v;
+  __builtin_memcpy(c.w, t, sizeof(t));
+}
+
+// Rows are packed: the wave's 64 lanes take 64 consecutive 8-pixel chunks
+// of the row-major volume, whatever row each falls in, so rows shorter than
+// 512 pixels keep every lane busy.  A lane whose chunk starts a row uses the
+// row's first pixel repeated as its left neighbour (clamp to edge) instead
+// of the lane to its left, and likewise at the row end.
 template <int OP, class TS, class TD, int C>
-__global__ __launch_bounds__(256) void sep_xreg_kernel(const SepParams p) {
+__device__ __forceinline__ void sep_xreg_body(const SepParams& p, const uint32_t vblk) {
+  static_assert(C >= 1 && C <= 16, "at most two neighbour chunks per side");
   using VT = xr_vt<OP, TS>;
   constexpr bool kFloatSrc = !Pix<TS>::kInt;
-  constexpr int NB = (C + 7) / 8;   // neighbour chunks needed on each side
+  constexpr int NB = (C + 7) / 8;  // neighbour chunks per side
   constexpr int WIN = 8 + 2 * C;
   const int lane = threadIdx.x & 63;
-  const uint32_t task = blockIdx.x * 4u + (threadIdx.x >> 6);
-  if (task >= (uint32_t)p.ntasks) return;  // whole waves only (4 tasks per block)
-  const uint32_t row = task / (uint32_t)p.nseg;
-  const int64_t x0 = (int64_t)(task - row * (uint32_t)p.nseg) * kXSeg;
-  const int64_t W = p.L;
-  const TS* __restrict__ rp = static_cast<const TS*>(p.src) + (int64_t)row * W;
-  const int64_t xl = x0 + 8 * lane;
+  const uint64_t wave = (uint64_t)vblk * 4 + (threadIdx.x >> 6);
+  const uint64_t per_row = (uint64_t)p.L / 8;
+  const uint64_t total = (uint64_t)p.outer * per_row;
+  if (wave * 64 >= total) return;  // whole waves only
+  const uint64_t g = wave * 64 + lane;
+  const bool active = g < total;
+  const uint64_t gc = active ? g : total - 1;  // idle tail lanes mirror the last chunk
+  uint64_t row, j;
+  if (total < (uint64_t(1) << 32)) {
+    row = (uint32_t)gc / (uint32_t)per_row;
+    j = (uint32_t)gc - (uint32_t)row * (uint32_t)per_row;
+  } else {
+    row = gc / per_row;
+    j = gc - row * per_row;
+  }
+  const TS* __restrict__ rp = static_cast<const TS*>(p.src) + row * (uint64_t)p.L;
+  const int64_t xl = 8 * (int64_t)j;
+  auto load8 = [&](int64_t x, Chunk<TS>& c) {
+    const PixVec<TS, 8> v = load_nt<TS, 8>(rp + x);
+    __builtin_memcpy(c.w, &v, sizeof(v));
+  };
 
   Chunk<TS> own;
-  load_chunk<TS>(rp, xl, W, own);
-  // halo chunks, loaded by the two edge lanes only: lane 0 holds chunk -k,
-  // lane 63 chunk 64+k-1, in halo[k-1]
-  Chunk<TS> halo[NB];
+  load8(xl, own);
+  // chunks k = 1..NB away: from lane l-k / l+k by repeated wave shifts;
+  // lanes 0 and 63 fetch theirs (they enter as the shifts' old values)
+  Chunk<TS> lm[NB], rm[NB];
 #pragma unroll
   for (int k = 0; k < NB; ++k)
 #pragma unroll
-    for (int j = 0; j < Chunk<TS>::NW; ++j) halo[k].w[j] = 0u;
-  if (lane == 0 || lane == 63) {
+    for (int w = 0; w < Chunk<TS>::NW; ++w) lm[k].w[w] = rm[k].w[w] = 0u;
+  if (lane == 0) {
 #pragma unroll
     for (int k = 1; k <= NB; ++k)
-      load_chunk<TS>(rp, lane == 0 ? x0 - 8 * k : x0 + kXSeg + 8 * (k - 1), W, halo[k - 1]);
+      if (j >= (uint64_t)k) load8(xl - 8 * k, lm[k - 1]);
+  }
+  if (lane == 63) {
+#pragma unroll
+    for (int k = 1; k <= NB; ++k)
+      if (j + k < per_row) load8(xl + 8 * k, rm[k - 1]);
   }
   Chunk<TS> prev[NB], next[NB];
 #pragma unroll
-  for (int k = 0; k < NB; ++k) {
+  for (int k = 0; k < NB; ++k)
 #pragma unroll
-    for (int j = 0; j < Chunk<TS>::NW; ++j) {
-      prev[k].w[j] = dpp_from_left(halo[k].w[j], k == 0 ? own.w[j] : prev[k - 1].w[j]);
-      next[k].w[j] = dpp_from_right(halo[k].w[j], k == 0 ? own.w[j] : next[k - 1].w[j]);
+    for (int w = 0; w < Chunk<TS>::NW; ++w) {
+      prev[k].w[w] = dpp_from_left(lm[k].w[w], k == 0 ? own.w[w] : prev[k - 1].w[w]);
+      next[k].w[w] = dpp_from_right(rm[k].w[w], k == 0 ? own.w[w] : next[k - 1].w[w]);
     }
+  // clamp to edge: chunks before the row start are its first pixel, chunks
+  // past its end the last pixel
+  if (j < (uint64_t)NB) {
+    const TS first = rp[0];
+#pragma unroll
+    for (int k = 1; k <= NB; ++k)
+      if (j < (uint64_t)k) fill_chunk<TS>(first, prev[k - 1]);
   }
-  // window element i is the pixel at xl - C + i
+  if (j + NB >= per_row) {
+    const TS last = rp[p.L - 1];
+#pragma unroll
+    for (int k = 1; k <= NB; ++k)
+      if (j + k >= per_row) fill_chunk<TS>(last, next[k - 1]);
+  }
+
   VT win[WIN];
 #pragma unroll
   for (int i = 0; i < WIN; ++i) {
@@ -132,7 +176,19 @@ __global__ __launch_bounds__(256) void sep_xreg_kernel(const SepParams p) {
       out[q] = convert_sat<TD>(acc);
     }
   }
-  if (xl < W) store_nt<TD, 8>(static_cast<TD*>(p.dst) + (int64_t)row * W + xl, out);
+  if (active) store_nt<TD, 8>(static_cast<TD*>(p.dst) + row * (uint64_t)p.L + xl, out);
+}
+
+template <int OP, class TS, class TD, int C>
+__global__ __launch_bounds__(256) void sep_xreg_kernel(const SepParams p) {
+  sep_xreg_body<OP, TS, TD, C>(p, blockIdx.x);
+}
+
+// blocks of 4 waves for a packed x pass
+inline int64_t xreg_blocks(const SepParams& p) {
+  const int64_t chunks = p.outer * (p.L / 8);
+  const int64_t waves = (chunks + 63) / 64;
+  return (waves + 3) / 4;
 }
 
 }  // namespace clk

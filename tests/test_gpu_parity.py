@@ -236,20 +236,22 @@ def test_full_size_sampled_lines(gpu, name, N, s, dst):
     torch.cuda.empty_cache()
 
 
-@pytest.mark.parametrize("shape", [(3, 5, 8), (2, 3, 16), (2, 4, 64), (3, 2, 520), (2, 3, 1024),
-                                   (1, 2, 1544), (37, 41, 8), (20, 300, 12), (70, 9, 6)])
+@pytest.mark.parametrize("shape", [(3, 5, 8), (2, 3, 16), (3, 3, 40), (2, 4, 64), (2, 5, 256), (3, 2, 520),
+                                   (2, 3, 1024), (1, 2, 1544), (37, 41, 8), (20, 300, 12), (70, 9, 6)])
 @pytest.mark.parametrize("pair", [(np.uint16, np.uint16), (np.uint16, np.float32), (np.float32, np.float32),
                                   (np.int8, np.int8), (np.uint8, np.float32), (np.int16, np.int16)],
                          ids=lambda p: "%s-%s" % (DTYPE_NAMES[p[0]], DTYPE_NAMES[p[1]]))
 def test_static_width_paths(gpu, shape, pair):
     """Half widths with compile-time kernels: the register-window x pass
-    (W % 8 == 0, c = 1..8) and the static y/z pass (c = 1..8, Gaussian up to
-    12) -- every half width, short rows, partial 512-pixel segments, blocks
-    spanning several slabs, both axis ends."""
+    (W % 8 == 0, c = 1..8, Gaussian up to 16 on rows of at most 256 pixels,
+    rows packed across a wave, rows shorter than the halo) and the static y/z
+    pass (c = 1..8, Gaussian up to 12) -- every half width, short rows,
+    partial 512-pixel segments, blocks spanning several slabs, both axis
+    ends."""
     src_t, dst_t = pair
     src = random_volume(shape, src_t, seed=shape[2] + 7)
     for name in SEP_OPS:
-        cmax = 12 if name.startswith("gaussian") else 8
+        cmax = 16 if name.startswith("gaussian") else 8
         for c in range(1, cmax + 1):
             N = 2 * c + 1
             for dim in (0, 1, 2):
