@@ -375,7 +375,7 @@ def main():
                          "batched vs one call per volume; c5: served chains "
                          "(Gaussian -> max -> median) under Poisson arrivals")
     ap.add_argument("--requests", type=int, default=48, help="c5: requests per arrival rate")
-    ap.add_argument("--no-autotune", action="store_true", help="c4: keep default launch shapes")
+    ap.add_argument("--no-autotune", action="store_true", help="keep default launch shapes (no first-launch tuning)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", default=None, help="run only launches whose label contains this")
     ap.add_argument("--rocprof-summary", default=None, metavar="DIR",
@@ -436,6 +436,12 @@ def main():
             if events is not None:
                 events[i][1].record(stream)
 
+    if not args.no_autotune:
+        # first-launch tuning of launch shapes: one untimed pass picks, per
+        # launch signature, the fastest shape; results never depend on it
+        clkmi.set_tuning("autotune", 1)
+        step()
+        torch.cuda.synchronize()
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -498,6 +504,7 @@ def main():
             "volume_xyz": [W, H, D],
             "pixel": "uint16",
             "filters_per_step": nfilters,
+            "autotune": not args.no_autotune,
         },
         "roofline": {
             "kernel": dom["launch"],

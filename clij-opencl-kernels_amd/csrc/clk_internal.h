@@ -36,6 +36,8 @@ struct SepParams {
   int32_t vec_ld;      // x pass: rows are aligned for vector loads
   int32_t vec_st;      // x pass: rows are aligned for vector stores
   int32_t fastdiv;     // numerators never need the guarded divide
+  int32_t order;       // strided pass block order: bit 0 = segments of a column
+                       // block adjacent, bit 1 = contiguous ranges per XCD
   float w[kMaxTaps];   // Gaussian weights in tap order v = -c..c
 };
 

@@ -13,8 +13,10 @@ namespace clk {
 namespace {
 
 const char* kNames[TUNE_COUNT] = {"strided_rounds", "x_sub", "batch_seg",
-                                  "no_xreg", "strided_variant", "no_hist8"};
-const int32_t kDefaults[TUNE_COUNT] = {2, 0, 64, 0, 0, 0};
+                                  "no_xreg", "strided_variant", "no_hist8",
+                                  "xreg_maxc", "ystatic_cols", "strided_order",
+                                  "autotune"};
+const int32_t kDefaults[TUNE_COUNT] = {16, 0, 64, 0, 0, 0, 0, 0, 0, 0};
 std::atomic<int32_t> g_val[TUNE_COUNT];
 std::once_flag g_once;
 
@@ -36,9 +38,33 @@ int find(const char* key) {
 
 }  // namespace
 
+namespace {
+thread_local int32_t t_over[TUNE_COUNT];
+thread_local bool t_over_set[TUNE_COUNT];
+}  // namespace
+
 int32_t tuning(TuneKey k) {
+  if (t_over_set[k]) return t_over[k];
   std::call_once(g_once, init);
   return g_val[k].load(std::memory_order_relaxed);
+}
+
+ScopedTuning::ScopedTuning(const TuneSetting* s, int n) {
+  for (int k = 0; k < TUNE_COUNT; ++k) {
+    saved_[k] = t_over[k];
+    saved_set_[k] = t_over_set[k];
+  }
+  for (int i = 0; i < n; ++i) {
+    t_over[s[i].key] = s[i].value;
+    t_over_set[s[i].key] = true;
+  }
+}
+
+ScopedTuning::~ScopedTuning() {
+  for (int k = 0; k < TUNE_COUNT; ++k) {
+    t_over[k] = saved_[k];
+    t_over_set[k] = saved_set_[k];
+  }
 }
 
 const char* tuning_name(int k) { return (k >= 0 && k < TUNE_COUNT) ? kNames[k] : nullptr; }
@@ -51,6 +77,11 @@ extern "C" int32_t clk_set_tuning(const char* key, int32_t value) {
   if (k < 0) return CLK_INVALID_VALUE;
   if (k == clk::TUNE_BATCH_SEG && (value < 8 || value > (1 << 20))) return CLK_INVALID_VALUE;
   if (k == clk::TUNE_STRIDED_ROUNDS && (value < 1 || value > 64)) return CLK_INVALID_VALUE;
+  if (k == clk::TUNE_XREG_MAXC && (value < 0 || value > 16)) return CLK_INVALID_VALUE;
+  if (k == clk::TUNE_YSTATIC_COLS && value != 0 && value != 1 && value != 2 && value != 4)
+    return CLK_INVALID_VALUE;
+  if (k == clk::TUNE_STRIDED_ORDER && (value < 0 || value > 3)) return CLK_INVALID_VALUE;
+  if (k == clk::TUNE_AUTOTUNE && value != 0 && value != 1) return CLK_INVALID_VALUE;
   clk::g_val[k].store(value);
   return CLK_SUCCESS;
 }

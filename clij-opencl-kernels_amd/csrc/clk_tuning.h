@@ -7,16 +7,38 @@
 namespace clk {
 
 enum TuneKey : int {
-  TUNE_STRIDED_ROUNDS = 0,   // y/z single launch: blocks per resident slot
+  TUNE_STRIDED_ROUNDS = 0,   // y/z single launch: blocks per resident slot (0 = 16)
   TUNE_X_SUB = 1,            // x LDS pass: 512-output segments per wave (0 = auto)
   TUNE_BATCH_SEG = 2,        // batched y/z: outputs per thread along the axis
   TUNE_NO_XREG = 3,          // 1 = never use the register-window x pass
   TUNE_STRIDED_VARIANT = 4,  // 0 = auto; >0 forces a generic y/z variant
   TUNE_NO_HIST8 = 5,         // 1 = never use the histogram median
-  TUNE_COUNT = 6
+  TUNE_XREG_MAXC = 6,        // 0 = auto; else the largest half width for the register-window x pass
+  TUNE_YSTATIC_COLS = 7,     // 0 = auto; 1, 2 or 4 columns per lane in the static y/z pass
+  TUNE_STRIDED_ORDER = 8,    // static y/z pass block order (bit 0: segment-minor, bit 1: XCD ranges)
+  TUNE_AUTOTUNE = 9,         // 1 = time launch shapes at the first launch of each signature
+  TUNE_COUNT = 10
 };
 
 int32_t tuning(TuneKey k);
 const char* tuning_name(int k);
+
+// Overrides for the calling thread while in scope (the autotuner's
+// candidates); other threads keep the process-wide values.
+struct TuneSetting {
+  TuneKey key;
+  int32_t value;
+};
+class ScopedTuning {
+ public:
+  ScopedTuning(const TuneSetting* s, int n);
+  ~ScopedTuning();
+  ScopedTuning(const ScopedTuning&) = delete;
+  ScopedTuning& operator=(const ScopedTuning&) = delete;
+
+ private:
+  int32_t saved_[TUNE_COUNT];
+  bool saved_set_[TUNE_COUNT];
+};
 
 }  // namespace clk

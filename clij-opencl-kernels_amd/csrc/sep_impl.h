@@ -94,11 +94,21 @@ __device__ __forceinline__ TD clamp_int(int32_t v) {
   }
 }
 
+// The divide the launch planned (p.fastdiv), resolved at compile time where
+// the pixel type fixes it: float sources always take the guarded divide,
+// box means of integer sources never need it, and Gaussians of integer
+// sources follow the flag (set from the smallest weight).
+template <int OP, class TS>
+__device__ __forceinline__ float div_planned(float a, const SepParams& p) {
+  if constexpr (!Pix<TS>::kInt) return div_uniform(a, p.div, p.rdiv);
+  else if constexpr (OP == OP_MEAN) return div_uniform_fast(a, p.div, p.rdiv);
+  else return p.fastdiv ? div_uniform_fast(a, p.div, p.rdiv) : div_uniform(a, p.div, p.rdiv);
+}
+
 template <int OP, class TS, class TD, class VT>
 __device__ __forceinline__ TD finish(VT acc, const SepParams& p) {
   if constexpr (OP == OP_GAUSS || OP == OP_MEAN) {
-    return convert_sat<TD>(p.fastdiv ? div_uniform_fast(acc, p.div, p.rdiv)
-                                     : div_uniform(acc, p.div, p.rdiv));
+    return convert_sat<TD>(div_planned<OP, TS>(static_cast<float>(acc), p));
   } else if constexpr (std::is_same<VT, int32_t>::value) {
     return clamp_int<TD>(acc);
   } else {
@@ -356,9 +366,28 @@ inline int resident_blocks(const void* kernel, size_t shmem) {
 // Blocks of 256 threads per resident slot that a launch aims for; a few
 // rounds of equal-sized blocks keep every CU busy to the end.  CLK_ROUNDS
 // and CLK_STRIDED_VARIANT override the defaults (used when tuning).
-inline int strided_rounds() { const int r = tuning(TUNE_STRIDED_ROUNDS); return r > 0 ? r : 2; }
+inline int strided_rounds() { const int r = tuning(TUNE_STRIDED_ROUNDS); return r > 0 ? r : 16; }
 inline bool no_xreg() { return tuning(TUNE_NO_XREG) != 0; }
 inline int strided_variant() { return tuning(TUNE_STRIDED_VARIANT); }
+
+// Segments along the axis for a strided launch of nblk column blocks: about
+// target_blocks blocks in all, each segment a multiple of K and at least
+// twice the filter long (a segment re-reads 2c halo inputs).
+template <int K>
+inline void plan_segments(SepParams& p, int64_t nblk, int64_t target_blocks) {
+  int64_t nseg = (target_blocks + nblk - 1) / nblk;
+  int64_t minseg = 2 * (int64_t)p.ntaps;
+  if (minseg < K) minseg = K;
+  int64_t maxseg = (p.L + minseg - 1) / minseg;
+  if (nseg > maxseg) nseg = maxseg;
+  if (nseg < 1) nseg = 1;
+  int64_t seg = (p.L + nseg - 1) / nseg;
+  seg = (seg + K - 1) / K * K;
+  nseg = (p.L + seg - 1) / seg;
+  p.seg = (int32_t)seg;
+  p.nseg = (int32_t)nseg;
+  p.nblk = (uint32_t)nblk;
+}
 
 template <int OP, class TS, class TD, int K, int COLS>
 int32_t launch_strided(hipStream_t st, SepParams& p, std::string* err) {
@@ -366,48 +395,7 @@ int32_t launch_strided(hipStream_t st, SepParams& p, std::string* err) {
   p.ncols_total = p.outer * p.ncolgroups;
   const int64_t nblk = (p.ncols_total + 255) / 256;
   const auto kern = sep_strided_kernel<OP, TS, TD, K, COLS>;
-  const int64_t target = (int64_t)resident_blocks((const void*)kern, 0) * strided_rounds();
-  int64_t nseg = (target + nblk - 1) / nblk;
-  const int64_t maxseg = (p.L + K - 1) / K;
-  if (nseg > maxseg) nseg = maxseg;
-  if (nseg < 1) nseg = 1;
-  int64_t seg = (p.L + nseg - 1) / nseg;
-  seg = (seg + K - 1) / K * K;
-  nseg = (p.L + seg - 1) / seg;
-  const int64_t blocks = nblk * nseg;
-  if (blocks > 0x7fffffffLL) {
-    *err = "volume too large for the strided pass grid";
-    return CLK_INVALID_VALUE;
-  }
-  p.seg = (int32_t)seg;
-  p.nseg = (int32_t)nseg;
-  p.nblk = (uint32_t)nblk;
-  hipLaunchKernelGGL(kern, dim3((uint32_t)blocks), dim3(256), 0, st, p);
-  return 0;
-}
-
-// Segments along the axis for a strided launch of nblk column blocks.
-template <int K>
-inline void plan_segments(SepParams& p, int64_t nblk, int64_t target_blocks) {
-  int64_t nseg = (target_blocks + nblk - 1) / nblk;
-  const int64_t maxseg = (p.L + K - 1) / K;
-  if (nseg > maxseg) nseg = maxseg;
-  if (nseg < 1) nseg = 1;
-  int64_t seg = (p.L + nseg - 1) / nseg;
-  seg = (seg + K - 1) / K * K;
-  nseg = (p.L + seg - 1) / seg;
-  p.seg = (int32_t)seg;
-  p.nseg = (int32_t)nseg;
-  p.nblk = (uint32_t)nblk;
-}
-
-template <int OP, class TS, class TD, int C, int COLS>
-int32_t launch_ystatic_cc(hipStream_t st, SepParams& p, std::string* err) {
-  p.ncolgroups = p.inner / COLS;
-  p.ncols_total = p.outer * p.ncolgroups;
-  const int64_t nblk = (p.ncols_total + 255) / 256;
-  const auto kern = sep_ystatic_kernel<OP, TS, TD, C, COLS>;
-  plan_segments<8>(p, nblk, (int64_t)resident_blocks((const void*)kern, 0) * strided_rounds());
+  plan_segments<K>(p, nblk, (int64_t)resident_blocks((const void*)kern, 0) * strided_rounds());
   const int64_t blocks = nblk * p.nseg;
   if (blocks > 0x7fffffffLL) {
     *err = "volume too large for the strided pass grid";
@@ -417,36 +405,60 @@ int32_t launch_ystatic_cc(hipStream_t st, SepParams& p, std::string* err) {
   return 0;
 }
 
-// Wider column groups while the window registers stay within ~112.
-template <int OP, class TS, class TD, int C>
-int32_t launch_ystatic_c(hipStream_t st, SepParams& p, bool v4, std::string* err) {
-  constexpr int WN = 8 + 2 * C;
-  if constexpr ((WN + 8) * 4 <= 112) {
-    if (v4) return launch_ystatic_cc<OP, TS, TD, C, 4>(st, p, err);
+template <int OP, class TS, class TD, int C, int COLS, int K>
+int32_t launch_ystatic_k(hipStream_t st, SepParams& p, std::string* err) {
+  p.ncolgroups = p.inner / COLS;
+  p.ncols_total = p.outer * p.ncolgroups;
+  const int64_t nblk = (p.ncols_total + 255) / 256;
+  const auto kern = sep_ystatic_kernel<OP, TS, TD, C, COLS, K>;
+  plan_segments<K>(p, nblk, (int64_t)resident_blocks((const void*)kern, 0) * strided_rounds());
+  const int64_t blocks = nblk * p.nseg;
+  if (blocks > 0x7fffffffLL) {
+    *err = "volume too large for the strided pass grid";
+    return CLK_INVALID_VALUE;
   }
-  if ((p.inner % 2) == 0) return launch_ystatic_cc<OP, TS, TD, C, 2>(st, p, err);
+  p.order = tuning(TUNE_STRIDED_ORDER);
+  hipLaunchKernelGGL(kern, dim3((uint32_t)blocks), dim3(256), 0, st, p);
+  return 0;
+}
+
+template <int OP, class TS, class TD, int C, int COLS>
+int32_t launch_ystatic_cc(hipStream_t st, SepParams& p, std::string* err) {
+  return launch_ystatic_k<OP, TS, TD, C, COLS, 8>(st, p, err);
+}
+
+// Wider column groups while the window registers stay within ~112; the
+// ystatic_cols knob can ask for fewer (or, where they fit, four).
+template <int OP, class TS, class TD, int C>
+int32_t launch_ystatic_c(hipStream_t st, SepParams& p, bool v4, bool v2, std::string* err) {
+  constexpr int WN = 8 + 2 * C;
+  const int want = tuning(TUNE_YSTATIC_COLS);
+  if constexpr ((WN + 8) * 4 <= 112) {
+    if (v4 && (want == 0 || want == 4)) return launch_ystatic_cc<OP, TS, TD, C, 4>(st, p, err);
+  }
+  if (v2 && want != 1) return launch_ystatic_cc<OP, TS, TD, C, 2>(st, p, err);
   return launch_ystatic_cc<OP, TS, TD, C, 1>(st, p, err);
 }
 
 template <int OP, class TS, class TD>
-int32_t launch_ystatic(hipStream_t st, SepParams& p, bool v4, std::string* err) {
+int32_t launch_ystatic(hipStream_t st, SepParams& p, bool v4, bool v2, std::string* err) {
   if constexpr (std::is_same<TS, TD>::value || std::is_same<TD, float>::value) {
     switch (p.c) {
-      case 1: return launch_ystatic_c<OP, TS, TD, 1>(st, p, v4, err);
-      case 2: return launch_ystatic_c<OP, TS, TD, 2>(st, p, v4, err);
-      case 3: return launch_ystatic_c<OP, TS, TD, 3>(st, p, v4, err);
-      case 4: return launch_ystatic_c<OP, TS, TD, 4>(st, p, v4, err);
-      case 5: return launch_ystatic_c<OP, TS, TD, 5>(st, p, v4, err);
-      case 6: return launch_ystatic_c<OP, TS, TD, 6>(st, p, v4, err);
-      case 7: return launch_ystatic_c<OP, TS, TD, 7>(st, p, v4, err);
-      case 8: return launch_ystatic_c<OP, TS, TD, 8>(st, p, v4, err);
+      case 1: return launch_ystatic_c<OP, TS, TD, 1>(st, p, v4, v2, err);
+      case 2: return launch_ystatic_c<OP, TS, TD, 2>(st, p, v4, v2, err);
+      case 3: return launch_ystatic_c<OP, TS, TD, 3>(st, p, v4, v2, err);
+      case 4: return launch_ystatic_c<OP, TS, TD, 4>(st, p, v4, v2, err);
+      case 5: return launch_ystatic_c<OP, TS, TD, 5>(st, p, v4, v2, err);
+      case 6: return launch_ystatic_c<OP, TS, TD, 6>(st, p, v4, v2, err);
+      case 7: return launch_ystatic_c<OP, TS, TD, 7>(st, p, v4, v2, err);
+      case 8: return launch_ystatic_c<OP, TS, TD, 8>(st, p, v4, v2, err);
     }
     if constexpr (OP == OP_GAUSS) {
       switch (p.c) {
-        case 9: return launch_ystatic_c<OP, TS, TD, 9>(st, p, v4, err);
-        case 10: return launch_ystatic_c<OP, TS, TD, 10>(st, p, v4, err);
-        case 11: return launch_ystatic_c<OP, TS, TD, 11>(st, p, v4, err);
-        case 12: return launch_ystatic_c<OP, TS, TD, 12>(st, p, v4, err);
+        case 9: return launch_ystatic_c<OP, TS, TD, 9>(st, p, v4, v2, err);
+        case 10: return launch_ystatic_c<OP, TS, TD, 10>(st, p, v4, v2, err);
+        case 11: return launch_ystatic_c<OP, TS, TD, 11>(st, p, v4, v2, err);
+        case 12: return launch_ystatic_c<OP, TS, TD, 12>(st, p, v4, v2, err);
       }
     }
   }
@@ -498,7 +510,10 @@ int32_t launch_sep_typed(hipStream_t st, SepParams& p, int axis_is_x, std::strin
   constexpr bool kXRegPair = std::is_same<TS, TD>::value || std::is_same<TD, float>::value;
   // wider windows (Gaussian only) on short rows, where a 512-output LDS
   // segment would leave lanes idle; long rows stage through LDS
-  const int xreg_maxc = (OP == OP_GAUSS && p.L <= 256) ? 16 : 8;
+  const int xreg_cap = (OP == OP_GAUSS) ? 16 : 8;
+  const int xreg_want = tuning(TUNE_XREG_MAXC);
+  const int xreg_maxc = xreg_want > 0 ? (xreg_want < xreg_cap ? xreg_want : xreg_cap)
+                                      : ((OP == OP_GAUSS && p.L <= 256) ? 16 : 8);
   if (kXRegPair && axis_is_x && p.c >= 1 && p.c <= xreg_maxc && p.L % kXK == 0 &&
       (uintptr_t)p.src % (kXK * sizeof(TS)) == 0 && (uintptr_t)p.dst % (kXK * sizeof(TD)) == 0 &&
       !no_xreg()) {
@@ -512,7 +527,7 @@ int32_t launch_sep_typed(hipStream_t st, SepParams& p, int axis_is_x, std::strin
     // larger tasks for long rows and narrow halos (more loads in flight per
     // wave, less per-task work); CLK_XSUB overrides when tuning
     int sub = tuning(TUNE_X_SUB);
-    if (sub != 1 && sub != 2 && sub != 4) sub = (W >= 4 * kXSeg && p.c <= 256) ? 2 : 1;
+    if (sub != 1 && sub != 2 && sub != 4) sub = (W >= 2 * kXSeg && p.c <= 256) ? 2 : 1;
     const int tseg = kXSeg * sub;
     p.nseg = (int32_t)((W + tseg - 1) / tseg);
     p.ntasks = p.outer * p.nseg;
@@ -548,7 +563,7 @@ int32_t launch_sep_typed(hipStream_t st, SepParams& p, int axis_is_x, std::strin
     const bool off32 = (extra * p.L * p.inner + p.inner) * maxbytes < (int64_t(1) << 31);
     if (kStaticPair && p.c >= 1 && p.c <= smaxc && off32 && p.L < (int64_t(1) << 30) &&
         strided_variant() == 0) {
-      rc = launch_ystatic<OP, TS, TD>(st, p, v4, err);
+      rc = launch_ystatic<OP, TS, TD>(st, p, v4, v2, err);
       if (rc) return rc;
       const hipError_t e = hipGetLastError();
       if (e != hipSuccess) {

@@ -4,8 +4,14 @@
 #include <math.h>
 #include <string.h>
 
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
 #include "clk_device.h"
 #include "clk_internal.h"
+#include "clk_tuning.h"
 #include "sep_impl.h"
 
 namespace clk {
@@ -144,7 +150,9 @@ int32_t prepare_separable(int op, const clk_image& dst, const clk_image& src, in
   } else if (op == OP_MEAN) {
     p.div = (float)(2 * c + 1);  // mean_separable_3d_x.cl:26 counts the taps in float
     p.rdiv = 1.0f / p.div;
-    p.fastdiv = (src.dtype != CLK_FLOAT);  // integer sums: 0 or >= 1 in magnitude
+    // integer sums: 0 or >= 1 in magnitude (the kernels also assume this
+    // flag for integer sources at compile time)
+    p.fastdiv = (src.dtype != CLK_FLOAT);
   }
   return 0;
 }
@@ -153,6 +161,122 @@ int32_t launch_fill_nan(hipStream_t st, const clk_image& dst, std::string* err) 
   return fill_nan(st, dst, err);
 }
 
+namespace {
+
+// One launch of the pass kernel with the current tuning (prepare + pick).
+int32_t launch_once(int op, hipStream_t st, const clk_image& dst, const clk_image& src, int32_t dim,
+                    int32_t N, float s, SepParams& p, std::string* err) {
+  int axis_is_x = 0;
+  bool nan_fill = false;
+  int32_t rc = prepare_separable(op, dst, src, dim, N, s, p, &axis_is_x, &nan_fill, err);
+  if (rc) return rc;
+  if (nan_fill) return fill_nan(st, dst, err);
+  switch (op) {
+    case OP_GAUSS: return launch_sep_gauss(st, p, axis_is_x, src.dtype, dst.dtype, err);
+    case OP_MEAN:  return launch_sep_mean(st, p, axis_is_x, src.dtype, dst.dtype, err);
+    case OP_MIN:   return launch_sep_min(st, p, axis_is_x, src.dtype, dst.dtype, err);
+    case OP_MAX:   return launch_sep_max(st, p, axis_is_x, src.dtype, dst.dtype, err);
+  }
+  *err = "invalid op";
+  return CLK_INVALID_VALUE;
+}
+
+// ---- first-launch tuning of launch shapes ----------------------------------
+// With the autotune knob on, the first launch of each signature (device, op,
+// pixel types, axis, N, volume size) times a few launch shapes on the
+// caller's stream -- every candidate writes the same output, since results
+// never depend on launch shape -- and later launches of that signature use
+// the fastest.  Small volumes and streams under graph capture skip it.
+
+struct TuneSig {
+  int dev, op, ts, td, dim, N;
+  int64_t w, h, d;
+  bool operator<(const TuneSig& o) const {
+    return std::tie(dev, op, ts, td, dim, N, w, h, d) < std::tie(o.dev, o.op, o.ts, o.td, o.dim, o.N, o.w, o.h, o.d);
+  }
+};
+
+struct Candidate {
+  TuneSetting set[4];
+  int n;
+};
+
+// x axis: register window (when it applies) or LDS segments of 1/2/4 x 512
+const Candidate kXCands[] = {
+    {{}, 0},
+    {{{TUNE_NO_XREG, 1}, {TUNE_X_SUB, 1}}, 2},
+    {{{TUNE_NO_XREG, 1}, {TUNE_X_SUB, 2}}, 2},
+    {{{TUNE_NO_XREG, 1}, {TUNE_X_SUB, 4}}, 2},
+    {{{TUNE_XREG_MAXC, 16}}, 1},
+};
+// y / z: blocks per slot, XCD block ranges, column width
+const Candidate kYZCands[] = {
+    {{{TUNE_STRIDED_ROUNDS, 16}, {TUNE_STRIDED_ORDER, 0}, {TUNE_YSTATIC_COLS, 0}}, 3},
+    {{{TUNE_STRIDED_ROUNDS, 2}, {TUNE_STRIDED_ORDER, 0}, {TUNE_YSTATIC_COLS, 0}}, 3},
+    {{{TUNE_STRIDED_ROUNDS, 8}, {TUNE_STRIDED_ORDER, 0}, {TUNE_YSTATIC_COLS, 0}}, 3},
+    {{{TUNE_STRIDED_ROUNDS, 2}, {TUNE_STRIDED_ORDER, 2}, {TUNE_YSTATIC_COLS, 0}}, 3},
+    {{{TUNE_STRIDED_ROUNDS, 8}, {TUNE_STRIDED_ORDER, 2}, {TUNE_YSTATIC_COLS, 0}}, 3},
+    {{{TUNE_STRIDED_ROUNDS, 16}, {TUNE_STRIDED_ORDER, 2}, {TUNE_YSTATIC_COLS, 0}}, 3},
+    {{{TUNE_STRIDED_ROUNDS, 2}, {TUNE_STRIDED_ORDER, 0}, {TUNE_YSTATIC_COLS, 2}}, 3},
+    {{{TUNE_STRIDED_ROUNDS, 8}, {TUNE_STRIDED_ORDER, 0}, {TUNE_YSTATIC_COLS, 2}}, 3},
+    {{{TUNE_STRIDED_ROUNDS, 16}, {TUNE_STRIDED_ORDER, 0}, {TUNE_YSTATIC_COLS, 2}}, 3},
+    {{{TUNE_STRIDED_ROUNDS, 2}, {TUNE_STRIDED_ORDER, 2}, {TUNE_YSTATIC_COLS, 2}}, 3},
+};
+constexpr int64_t kTuneMinVoxels = int64_t(1) << 22;
+
+std::mutex g_tune_mu;
+std::map<TuneSig, Candidate> g_tuned;
+
+// Times every candidate in three rounds (the first warms up) and returns
+// the one with the smallest of its timed launches; rc != 0 when a launch
+// failed.
+int32_t tune_signature(int op, hipStream_t st, const clk_image& dst, const clk_image& src, int32_t dim,
+                       int32_t N, float s, SepParams& p, bool axis_x, Candidate* best, std::string* err) {
+  const Candidate* cands = axis_x ? kXCands : kYZCands;
+  const int n = axis_x ? (int)(sizeof(kXCands) / sizeof(kXCands[0]))
+                       : (int)(sizeof(kYZCands) / sizeof(kYZCands[0]));
+  constexpr int kTimed = 2;
+  std::vector<hipEvent_t> ev(2 * n * kTimed, nullptr);
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      for (auto& f : ev) if (f) (void)hipEventDestroy(f);
+      *err = "autotune: event creation failed";
+      return CLK_OUT_OF_RESOURCES;
+    }
+  int32_t rc = 0;
+  for (int pass = 0; pass <= kTimed && rc == 0; ++pass)
+    for (int i = 0; i < n && rc == 0; ++i) {
+      ScopedTuning scope(cands[i].set, cands[i].n);
+      hipEvent_t* e = pass > 0 ? &ev[2 * (i * kTimed + pass - 1)] : nullptr;
+      if (e) (void)hipEventRecord(e[0], st);
+      rc = launch_once(op, st, dst, src, dim, N, s, p, err);
+      if (e) (void)hipEventRecord(e[1], st);
+    }
+  if (rc == 0 && hipEventSynchronize(ev[2 * ((n - 1) * kTimed + kTimed - 1) + 1]) != hipSuccess) {
+    *err = "autotune: synchronisation failed";
+    rc = CLK_OUT_OF_RESOURCES;
+  }
+  if (rc == 0) {
+    float best_ms = -1.f;
+    int bi = 0;
+    for (int i = 0; i < n; ++i)
+      for (int t = 0; t < kTimed; ++t) {
+        float ms = 0.f;
+        const int k = 2 * (i * kTimed + t);
+        if (hipEventElapsedTime(&ms, ev[k], ev[k + 1]) != hipSuccess) continue;
+        if (best_ms < 0.f || ms < best_ms) {
+          best_ms = ms;
+          bi = i;
+        }
+      }
+    *best = cands[bi];
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  return rc;
+}
+
+}  // namespace
+
 int32_t launch_separable(int op, hipStream_t st, const clk_image& dst, const clk_image& src,
                          int32_t dim, int32_t N, float s, std::string* err) {
   const int64_t W = dst.width, H = dst.height, D = dst.depth;
@@ -160,20 +284,40 @@ int32_t launch_separable(int op, hipStream_t st, const clk_image& dst, const clk
   SepParams* pp = new (std::nothrow) SepParams;
   if (!pp) { *err = "host allocation failed"; return CLK_OUT_OF_RESOURCES; }
   SepParams& p = *pp;
-  int axis_is_x = 0;
-  bool nan_fill = false;
-  int32_t rc = prepare_separable(op, dst, src, dim, N, s, p, &axis_is_x, &nan_fill, err);
-  if (rc == 0 && nan_fill) {
-    rc = fill_nan(st, dst, err);
-  } else if (rc == 0) {
-    switch (op) {
-      case OP_GAUSS: rc = launch_sep_gauss(st, p, axis_is_x, src.dtype, dst.dtype, err); break;
-      case OP_MEAN:  rc = launch_sep_mean(st, p, axis_is_x, src.dtype, dst.dtype, err); break;
-      case OP_MIN:   rc = launch_sep_min(st, p, axis_is_x, src.dtype, dst.dtype, err); break;
-      case OP_MAX:   rc = launch_sep_max(st, p, axis_is_x, src.dtype, dst.dtype, err); break;
-      default: *err = "invalid op"; rc = CLK_INVALID_VALUE; break;
+  int32_t rc = 0;
+  bool done = false;
+  if (tuning(TUNE_AUTOTUNE) == 1 && W * H * D >= kTuneMinVoxels && N > 1) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    int dev = 0;
+    if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone &&
+        hipGetDevice(&dev) == hipSuccess) {
+      const TuneSig sig{dev, op, src.dtype, dst.dtype, dim, N, W, H, D};
+      Candidate c{};
+      bool have = false;
+      {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(sig);
+        if (it != g_tuned.end()) {
+          c = it->second;
+          have = true;
+        }
+      }
+      if (!have) {
+        rc = tune_signature(op, st, dst, src, dim, N, s, p, dim == 0, &c, err);
+        if (rc == 0) {
+          std::lock_guard<std::mutex> lk(g_tune_mu);
+          g_tuned[sig] = c;
+        }
+        done = true;  // the candidates already wrote dst
+      }
+      if (!done) {
+        ScopedTuning scope(c.set, c.n);
+        rc = launch_once(op, st, dst, src, dim, N, s, p, err);
+        done = true;
+      }
     }
   }
+  if (!done) rc = launch_once(op, st, dst, src, dim, N, s, p, err);
   delete pp;
   return rc;
 }

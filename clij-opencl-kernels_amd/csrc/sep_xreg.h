@@ -168,8 +168,7 @@ __device__ __forceinline__ void sep_xreg_body(const SepParams& p, const uint32_t
     }
     if constexpr (OP == OP_GAUSS || OP == OP_MEAN) {
       const float a = static_cast<float>(acc);  // exact for integer sums
-      const float r = p.fastdiv ? div_uniform_fast(a, p.div, p.rdiv) : div_uniform(a, p.div, p.rdiv);
-      out[q] = convert_sat<TD>(r);
+      out[q] = convert_sat<TD>(div_planned<OP, TS>(a, p));
     } else if constexpr (std::is_same<VT, int32_t>::value) {
       out[q] = clamp_int<TD>(acc);
     } else {

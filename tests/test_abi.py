@@ -87,3 +87,21 @@ int main(void) {
             ctypes.sizeof(_lib.ClkFilterJob), _lib.ClkFilterJob.dst.offset, _lib.ClkFilterJob.src.offset,
             _lib.ClkFilterJob.sigma.offset, _lib.ClkFilterJob.radius.offset]
     assert [int(v) for v in out] == want
+
+
+def test_tuning_knobs_validate_values():
+    """Launch-shape knobs accept their documented ranges only (host only)."""
+    import clkmi
+    for key, good, bad in [("autotune", 1, 2), ("strided_order", 3, 4), ("ystatic_cols", 4, 3),
+                           ("xreg_maxc", 16, 17), ("strided_rounds", 64, 0), ("batch_seg", 8, 7)]:
+        old = clkmi.get_tuning(key)
+        try:
+            clkmi.set_tuning(key, good)
+            assert clkmi.get_tuning(key) == good
+            with pytest.raises(clkmi.ClkError) as e:
+                clkmi.set_tuning(key, bad)
+            assert e.value.code == -30
+        finally:
+            clkmi.set_tuning(key, old)
+    with pytest.raises(clkmi.ClkError):
+        clkmi.set_tuning("no_such_knob", 1)

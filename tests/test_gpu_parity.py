@@ -258,3 +258,28 @@ def test_static_width_paths(gpu, shape, pair):
                 got = gpu_separable(name, src, dst_t, dim, N, 0.4 * c + 0.3)
                 want = O.separable(name, src, dst_t, dim, N, 0.4 * c + 0.3)
                 assert_parity(got, want, "%s %s dim=%d c=%d" % (name, shape, dim, c))
+
+
+KNOB_SETTINGS = [("strided_rounds", 2), ("strided_rounds", 8), ("strided_order", 1), ("strided_order", 2),
+                 ("strided_order", 3), ("ystatic_cols", 1), ("ystatic_cols", 2), ("x_sub", 1), ("x_sub", 2),
+                 ("x_sub", 4), ("no_xreg", 1), ("xreg_maxc", 16), ("strided_variant", 2)]
+
+
+@pytest.mark.parametrize("name,N,dst_t", [("gaussian_blur_separable_3d", 25, np.float32),
+                                          ("maximum_separable_3d", 7, np.uint16)])
+def test_launch_shapes_and_autotune_do_not_change_results(gpu, name, N, dst_t):
+    """Every launch-shape knob and the first-launch autotuner (on a volume
+    large enough to be tuned, 2^22 voxels) give the default launch's bits."""
+    import clkmi
+    src = random_volume((64, 256, 256), np.uint16, seed=N)
+    ref = [gpu_separable(name, src, dst_t, dim, N, 2.5) for dim in (0, 1, 2)]
+    for key, val in KNOB_SETTINGS + [("autotune", 1)]:
+        old = clkmi.get_tuning(key)
+        try:
+            clkmi.set_tuning(key, val)
+            for rep in range(2 if key == "autotune" else 1):  # tuning launch, then the tuned one
+                for dim in (0, 1, 2):
+                    got = gpu_separable(name, src, dst_t, dim, N, 2.5)
+                    assert same_bits_or_value(got, ref[dim]), "%s=%d dim=%d rep=%d" % (key, val, dim, rep)
+        finally:
+            clkmi.set_tuning(key, old)
