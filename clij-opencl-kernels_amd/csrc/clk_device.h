@@ -141,6 +141,23 @@ __device__ __forceinline__ PixVec<T, N> load_nt(const T* __restrict__ p) {
   return v;
 }
 
+// Streaming load of B bytes (8, 16 or 32; `p` aligned to min(B, 16)) as
+// 32-bit words, without going through a byte array.
+template <int B>
+__device__ __forceinline__ void load_nt_words(const void* __restrict__ p, uint32_t (&w)[B / 4]) {
+  static_assert(B == 8 || B % 16 == 0, "8 bytes or whole 16-byte vectors");
+  if constexpr (B == 8) {
+    const u32x2 t = __builtin_nontemporal_load(static_cast<const u32x2*>(p));
+    w[0] = t.x; w[1] = t.y;
+  } else {
+#pragma unroll
+    for (int k = 0; k < B / 16; ++k) {
+      const u32x4 t = __builtin_nontemporal_load(static_cast<const u32x4*>(p) + k);
+      w[4 * k] = t.x; w[4 * k + 1] = t.y; w[4 * k + 2] = t.z; w[4 * k + 3] = t.w;
+    }
+  }
+}
+
 __device__ __forceinline__ int64_t clamp64(int64_t v, int64_t lo, int64_t hi) {
   return v < lo ? lo : (v > hi ? hi : v);
 }

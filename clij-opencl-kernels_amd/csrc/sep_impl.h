@@ -466,10 +466,23 @@ int32_t launch_ystatic(hipStream_t st, SepParams& p, bool v4, bool v2, std::stri
   return CLK_INVALID_VALUE;
 }
 
+// Units of 64 chunks per wave for the register-window x pass: 4 (loads of
+// the next unit in flight while one is computed) when that still leaves
+// several waves per SIMD slot, else 1; the xreg_units knob overrides.
+inline int xreg_units(const SepParams& p) {
+  const int want = tuning(TUNE_XREG_UNITS);
+  if (want == 1 || want == 2 || want == 4) return want;
+  const int64_t units = (p.outer * (p.L / 8) + 63) / 64;
+  return units >= 4 * 65536 ? 4 : 1;
+}
+
 template <int OP, class TS, class TD, int C>
 void launch_xreg_c(hipStream_t st, const SepParams& p) {
-  const uint32_t blocks = (uint32_t)xreg_blocks(p);
-  hipLaunchKernelGGL((sep_xreg_kernel<OP, TS, TD, C>), dim3(blocks), dim3(256), 0, st, p);
+  const int R = xreg_units(p);
+  const uint32_t blocks = (uint32_t)xreg_blocks(p, R);
+  if (R == 4) hipLaunchKernelGGL((sep_xreg_kernel<OP, TS, TD, C, 4>), dim3(blocks), dim3(256), 0, st, p);
+  else if (R == 2) hipLaunchKernelGGL((sep_xreg_kernel<OP, TS, TD, C, 2>), dim3(blocks), dim3(256), 0, st, p);
+  else hipLaunchKernelGGL((sep_xreg_kernel<OP, TS, TD, C, 1>), dim3(blocks), dim3(256), 0, st, p);
 }
 
 template <int OP, class TS, class TD>
@@ -517,7 +530,7 @@ int32_t launch_sep_typed(hipStream_t st, SepParams& p, int axis_is_x, std::strin
   if (kXRegPair && axis_is_x && p.c >= 1 && p.c <= xreg_maxc && p.L % kXK == 0 &&
       (uintptr_t)p.src % (kXK * sizeof(TS)) == 0 && (uintptr_t)p.dst % (kXK * sizeof(TD)) == 0 &&
       !no_xreg()) {
-    if (xreg_blocks(p) > 0x7fffffffLL) {
+    if (xreg_blocks(p, 1) > 0x7fffffffLL) {
       *err = "volume too large for the x pass grid";
       return CLK_INVALID_VALUE;
     }

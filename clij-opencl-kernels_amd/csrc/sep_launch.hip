@@ -204,10 +204,13 @@ struct Candidate {
 // x axis: register window (when it applies) or LDS segments of 1/2/4 x 512
 const Candidate kXCands[] = {
     {{}, 0},
+    {{{TUNE_XREG_UNITS, 1}}, 1},
+    {{{TUNE_XREG_UNITS, 4}}, 1},
     {{{TUNE_NO_XREG, 1}, {TUNE_X_SUB, 1}}, 2},
     {{{TUNE_NO_XREG, 1}, {TUNE_X_SUB, 2}}, 2},
     {{{TUNE_NO_XREG, 1}, {TUNE_X_SUB, 4}}, 2},
-    {{{TUNE_XREG_MAXC, 16}}, 1},
+    {{{TUNE_XREG_MAXC, 16}, {TUNE_XREG_UNITS, 4}}, 2},
+    {{{TUNE_XREG_MAXC, 16}, {TUNE_XREG_UNITS, 2}}, 2},
 };
 // y / z: blocks per slot, XCD block ranges, column width
 const Candidate kYZCands[] = {

@@ -262,7 +262,8 @@ def test_static_width_paths(gpu, shape, pair):
 
 KNOB_SETTINGS = [("strided_rounds", 2), ("strided_rounds", 8), ("strided_order", 1), ("strided_order", 2),
                  ("strided_order", 3), ("ystatic_cols", 1), ("ystatic_cols", 2), ("x_sub", 1), ("x_sub", 2),
-                 ("x_sub", 4), ("no_xreg", 1), ("xreg_maxc", 16), ("strided_variant", 2)]
+                 ("x_sub", 4), ("no_xreg", 1), ("xreg_maxc", 16), ("strided_variant", 2), ("xreg_units", 2),
+                 ("xreg_units", 4)]
 
 
 @pytest.mark.parametrize("name,N,dst_t", [("gaussian_blur_separable_3d", 25, np.float32),
@@ -283,3 +284,26 @@ def test_launch_shapes_and_autotune_do_not_change_results(gpu, name, N, dst_t):
                     assert same_bits_or_value(got, ref[dim]), "%s=%d dim=%d rep=%d" % (key, val, dim, rep)
         finally:
             clkmi.set_tuning(key, old)
+
+
+@pytest.mark.parametrize("units", [2, 4])
+@pytest.mark.parametrize("shape", [(3, 5, 8), (7, 9, 40), (5, 33, 256), (2, 3, 1024), (4, 37, 64)])
+def test_register_x_pass_several_units_per_wave(gpu, shape, units):
+    """Waves filtering 2 or 4 consecutive units of 512 outputs (the next
+    unit's loads in flight), including grids whose last wave has fewer units."""
+    import clkmi
+    src = random_volume(shape, np.uint16, seed=units * 100 + shape[2])
+    old = clkmi.get_tuning("xreg_units")
+    try:
+        clkmi.set_tuning("xreg_units", units)
+        for name, cs in (("gaussian_blur_separable_3d", (1, 5, 12, 16)), ("mean_separable_3d", (1, 3, 8)),
+                         ("maximum_separable_3d", (2, 8))):
+            for c in cs:
+                if c > 8 and shape[2] > 256:
+                    continue  # wider windows take the LDS pass on long rows
+                dst_t = np.float32 if name.startswith("g") else np.uint16
+                got = gpu_separable(name, src, dst_t, 0, 2 * c + 1, 0.5 * c)
+                want = O.separable(name, src, dst_t, 0, 2 * c + 1, 0.5 * c)
+                assert_parity(got, want, "%s %s c=%d units=%d" % (name, shape, c, units))
+    finally:
+        clkmi.set_tuning("xreg_units", old)
