@@ -376,6 +376,7 @@ def main():
                          "(Gaussian -> max -> median) under Poisson arrivals")
     ap.add_argument("--requests", type=int, default=48, help="c5: requests per arrival rate")
     ap.add_argument("--no-autotune", action="store_true", help="keep default launch shapes (no first-launch tuning)")
+    ap.add_argument("--tune-cache", default=None, help="file of tuned launch shapes to load before and save after tuning")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", default=None, help="run only launches whose label contains this")
     ap.add_argument("--rocprof-summary", default=None, metavar="DIR",
@@ -438,10 +439,16 @@ def main():
 
     if not args.no_autotune:
         # first-launch tuning of launch shapes: one untimed pass picks, per
-        # launch signature, the fastest shape; results never depend on it
+        # launch signature, the fastest shape; results never depend on it.
+        # With --tune-cache the shapes of an earlier run are reused (and new
+        # ones added), so profiled runs launch exactly the tuned kernels.
         clkmi.set_tuning("autotune", 1)
+        if args.tune_cache and os.path.exists(args.tune_cache):
+            clkmi.autotune_load(args.tune_cache)
         step()
         torch.cuda.synchronize()
+        if args.tune_cache:
+            clkmi.autotune_save(args.tune_cache)
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()

@@ -20,7 +20,7 @@ __all__ = [
     "ClkError", "Image", "execute", "gaussian_blur_separable_3d", "mean_separable_3d",
     "minimum_separable_3d", "maximum_separable_3d", "median_box_3d", "gaussian_blur_3d",
     "mean_box_3d", "minimum_box_3d", "maximum_box_3d", "sigma_to_kernel_size", "KERNELS",
-    "FilterJob", "batch_filter_3d", "set_tuning", "get_tuning",
+    "FilterJob", "batch_filter_3d", "set_tuning", "get_tuning", "autotune_save", "autotune_load",
 ]
 
 # Kernel name -> declared parameters (reference .cl declarations).
@@ -212,3 +212,13 @@ def get_tuning(key):
     v = ctypes.c_int32()
     check(lib().clk_get_tuning(key.encode(), ctypes.byref(v)))
     return v.value
+
+
+def autotune_save(path):
+    """Write the launch shapes chosen by the first-launch autotuner to a file."""
+    check(lib().clk_autotune_save(str(path).encode()))
+
+
+def autotune_load(path):
+    """Merge launch shapes saved by autotune_save (no timing runs needed)."""
+    check(lib().clk_autotune_load(str(path).encode()))

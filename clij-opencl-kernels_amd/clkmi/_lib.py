@@ -31,7 +31,7 @@ EXPORTED = (
     "clk_mean_box_3d", "clk_minimum_box_3d", "clk_maximum_box_3d", "clk_init", "clk_malloc",
     "clk_free", "clk_memcpy_h2d", "clk_memcpy_d2h", "clk_stream_create", "clk_stream_destroy",
     "clk_stream_sync", "clk_dtype_size", "clk_last_error_string", "clk_version",
-    "clk_batch_filter_3d", "clk_set_tuning", "clk_get_tuning",
+    "clk_batch_filter_3d", "clk_set_tuning", "clk_get_tuning", "clk_autotune_save", "clk_autotune_load",
 )
 
 CLK_JOB_GAUSSIAN_BLUR_3D, CLK_JOB_MEAN_BOX_3D, CLK_JOB_MINIMUM_BOX_3D, CLK_JOB_MAXIMUM_BOX_3D, \
@@ -114,6 +114,10 @@ def lib():
         L.clk_set_tuning.restype = i32
         L.clk_get_tuning.argtypes = [ctypes.c_char_p, ctypes.POINTER(i32)]
         L.clk_get_tuning.restype = i32
+        for name in ("clk_autotune_save", "clk_autotune_load"):
+            fn = getattr(L, name)
+            fn.argtypes = [ctypes.c_char_p]
+            fn.restype = i32
         L.clk_init.argtypes = []
         L.clk_init.restype = i32
         L.clk_malloc.argtypes = [ctypes.POINTER(vp), i64]

@@ -56,6 +56,11 @@ int32_t launch_separable(int op, hipStream_t st, const clk_image& dst,
                          const clk_image& src, int32_t dim, int32_t N, float s,
                          std::string* err);
 
+// Tuned launch shapes of single launches: write them to / merge them from
+// a text file (see clk_autotune_save / clk_autotune_load).
+int32_t autotune_save(const char* path);
+int32_t autotune_load(const char* path);
+
 // Fill SepParams for one single-axis launch (weights, divisor, axis view).
 // Sets *fill_nan when the launch reduces to writing CONVERT(NaN) everywhere.
 int32_t prepare_separable(int op, const clk_image& dst, const clk_image& src, int32_t dim, int32_t N,

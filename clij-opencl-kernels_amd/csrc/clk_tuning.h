@@ -18,7 +18,8 @@ enum TuneKey : int {
   TUNE_STRIDED_ORDER = 8,    // static y/z pass block order (bit 0: segment-minor, bit 1: XCD ranges)
   TUNE_AUTOTUNE = 9,         // 1 = time launch shapes at the first launch of each signature
   TUNE_XREG_UNITS = 10,      // 0 = auto; 1, 2 or 4 units of 512 outputs per wave in the register x pass
-  TUNE_COUNT = 11
+  TUNE_X_DYNAMIC = 11,       // 1 = LDS x pass always takes the half width at run time
+  TUNE_COUNT = 12
 };
 
 int32_t tuning(TuneKey k);

@@ -347,6 +347,19 @@ int32_t clk_stream_destroy(void* stream) {
   return ok();
 }
 
+int32_t clk_autotune_save(const char* path) {
+  if (!path) return fail(CLK_INVALID_VALUE, "clk_autotune_save: NULL path");
+  if (clk::autotune_save(path)) return fail(CLK_INVALID_VALUE, std::string("clk_autotune_save: cannot write ") + path);
+  return ok();
+}
+
+int32_t clk_autotune_load(const char* path) {
+  if (!path) return fail(CLK_INVALID_VALUE, "clk_autotune_load: NULL path");
+  if (clk::autotune_load(path))
+    return fail(CLK_INVALID_VALUE, std::string("clk_autotune_load: cannot read or parse ") + path);
+  return ok();
+}
+
 int32_t clk_stream_sync(void* stream) {
   const hipError_t e = hipStreamSynchronize((hipStream_t)stream);
   if (e != hipSuccess) return fail(CLK_OUT_OF_RESOURCES, std::string("clk_stream_sync: ") + hipGetErrorString(e));

@@ -2,6 +2,8 @@
 // reference kernel signature onto SepParams and picks the pass kernel.
 // Also holds the small converting-copy and fill kernels.
 #include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <map>
@@ -212,32 +214,30 @@ const Candidate kXCands[] = {
     {{{TUNE_XREG_MAXC, 16}, {TUNE_XREG_UNITS, 4}}, 2},
     {{{TUNE_XREG_MAXC, 16}, {TUNE_XREG_UNITS, 2}}, 2},
 };
-// y / z: blocks per slot, XCD block ranges, column width
-const Candidate kYZCands[] = {
-    {{{TUNE_STRIDED_ROUNDS, 16}, {TUNE_STRIDED_ORDER, 0}, {TUNE_YSTATIC_COLS, 0}}, 3},
-    {{{TUNE_STRIDED_ROUNDS, 2}, {TUNE_STRIDED_ORDER, 0}, {TUNE_YSTATIC_COLS, 0}}, 3},
-    {{{TUNE_STRIDED_ROUNDS, 8}, {TUNE_STRIDED_ORDER, 0}, {TUNE_YSTATIC_COLS, 0}}, 3},
-    {{{TUNE_STRIDED_ROUNDS, 2}, {TUNE_STRIDED_ORDER, 2}, {TUNE_YSTATIC_COLS, 0}}, 3},
-    {{{TUNE_STRIDED_ROUNDS, 8}, {TUNE_STRIDED_ORDER, 2}, {TUNE_YSTATIC_COLS, 0}}, 3},
-    {{{TUNE_STRIDED_ROUNDS, 16}, {TUNE_STRIDED_ORDER, 2}, {TUNE_YSTATIC_COLS, 0}}, 3},
-    {{{TUNE_STRIDED_ROUNDS, 2}, {TUNE_STRIDED_ORDER, 0}, {TUNE_YSTATIC_COLS, 2}}, 3},
-    {{{TUNE_STRIDED_ROUNDS, 8}, {TUNE_STRIDED_ORDER, 0}, {TUNE_YSTATIC_COLS, 2}}, 3},
-    {{{TUNE_STRIDED_ROUNDS, 16}, {TUNE_STRIDED_ORDER, 0}, {TUNE_YSTATIC_COLS, 2}}, 3},
-    {{{TUNE_STRIDED_ROUNDS, 2}, {TUNE_STRIDED_ORDER, 2}, {TUNE_YSTATIC_COLS, 2}}, 3},
+// y / z: coordinate descent over blocks per slot, block order and column
+// width (then blocks per slot again, as the best count depends on order)
+struct TuneDim {
+  TuneKey key;
+  int n;
+  int32_t vals[6];
+};
+const TuneDim kYZDims[] = {
+    {TUNE_STRIDED_ROUNDS, 6, {2, 4, 8, 16, 32, 64}},
+    {TUNE_STRIDED_ORDER, 4, {0, 1, 2, 3}},
+    {TUNE_YSTATIC_COLS, 3, {0, 1, 2}},
+    {TUNE_STRIDED_ROUNDS, 6, {2, 4, 8, 16, 32, 64}},
 };
 constexpr int64_t kTuneMinVoxels = int64_t(1) << 22;
 
 std::mutex g_tune_mu;
 std::map<TuneSig, Candidate> g_tuned;
 
-// Times every candidate in three rounds (the first warms up) and returns
-// the one with the smallest of its timed launches; rc != 0 when a launch
+// Times every candidate in three rounds (the first warms up) and sets *bi
+// to the one with the smallest of its timed launches; rc != 0 when a launch
 // failed.
-int32_t tune_signature(int op, hipStream_t st, const clk_image& dst, const clk_image& src, int32_t dim,
-                       int32_t N, float s, SepParams& p, bool axis_x, Candidate* best, std::string* err) {
-  const Candidate* cands = axis_x ? kXCands : kYZCands;
-  const int n = axis_x ? (int)(sizeof(kXCands) / sizeof(kXCands[0]))
-                       : (int)(sizeof(kYZCands) / sizeof(kYZCands[0]));
+int32_t time_candidates(int op, hipStream_t st, const clk_image& dst, const clk_image& src, int32_t dim,
+                        int32_t N, float s, SepParams& p, const Candidate* cands, int n, int* bi,
+                        std::string* err) {
   constexpr int kTimed = 2;
   std::vector<hipEvent_t> ev(2 * n * kTimed, nullptr);
   for (auto& e : ev)
@@ -261,7 +261,7 @@ int32_t tune_signature(int op, hipStream_t st, const clk_image& dst, const clk_i
   }
   if (rc == 0) {
     float best_ms = -1.f;
-    int bi = 0;
+    *bi = 0;
     for (int i = 0; i < n; ++i)
       for (int t = 0; t < kTimed; ++t) {
         float ms = 0.f;
@@ -269,16 +269,100 @@ int32_t tune_signature(int op, hipStream_t st, const clk_image& dst, const clk_i
         if (hipEventElapsedTime(&ms, ev[k], ev[k + 1]) != hipSuccess) continue;
         if (best_ms < 0.f || ms < best_ms) {
           best_ms = ms;
-          bi = i;
+          *bi = i;
         }
       }
-    *best = cands[bi];
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
   return rc;
 }
 
+int32_t tune_signature(int op, hipStream_t st, const clk_image& dst, const clk_image& src, int32_t dim,
+                       int32_t N, float s, SepParams& p, bool axis_x, Candidate* best, std::string* err) {
+  int bi = 0;
+  if (axis_x) {
+    const int n = (int)(sizeof(kXCands) / sizeof(kXCands[0]));
+    const int32_t rc = time_candidates(op, st, dst, src, dim, N, s, p, kXCands, n, &bi, err);
+    if (rc == 0) *best = kXCands[bi];
+    return rc;
+  }
+  // current point: rounds, order, cols (the launcher's defaults)
+  Candidate cur{{{TUNE_STRIDED_ROUNDS, 16}, {TUNE_STRIDED_ORDER, 0}, {TUNE_YSTATIC_COLS, 0}}, 3};
+  for (const TuneDim& d : kYZDims) {
+    Candidate c[6];
+    const int slot = d.key == TUNE_STRIDED_ROUNDS ? 0 : (d.key == TUNE_STRIDED_ORDER ? 1 : 2);
+    for (int i = 0; i < d.n; ++i) {
+      c[i] = cur;
+      c[i].set[slot].value = d.vals[i];
+    }
+    const int32_t rc = time_candidates(op, st, dst, src, dim, N, s, p, c, d.n, &bi, err);
+    if (rc) return rc;
+    cur = c[bi];
+  }
+  *best = cur;
+  return 0;
+}
+
 }  // namespace
+
+int32_t autotune_save(const char* path) {
+  FILE* f = fopen(path, "w");
+  if (!f) return CLK_INVALID_VALUE;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  fprintf(f, "# clkmi launch shapes: op src dst dim N width height depth : key=value ...\n");
+  for (const auto& kv : g_tuned) {
+    const TuneSig& t = kv.first;
+    fprintf(f, "%d %d %d %d %d %lld %lld %lld :", t.op, t.ts, t.td, t.dim, t.N, (long long)t.w,
+            (long long)t.h, (long long)t.d);
+    for (int i = 0; i < kv.second.n; ++i)
+      fprintf(f, " %s=%d", tuning_name(kv.second.set[i].key), kv.second.set[i].value);
+    fprintf(f, "\n");
+  }
+  return fclose(f) == 0 ? 0 : CLK_INVALID_VALUE;
+}
+
+int32_t autotune_load(const char* path) {
+  FILE* f = fopen(path, "r");
+  if (!f) return CLK_INVALID_VALUE;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+  char line[512];
+  int32_t rc = 0;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  while (fgets(line, sizeof(line), f)) {
+    if (line[0] == '#' || line[0] == '\n') continue;
+    TuneSig t{};
+    long long w = 0, h = 0, d = 0;
+    int used = 0;
+    if (sscanf(line, "%d %d %d %d %d %lld %lld %lld :%n", &t.op, &t.ts, &t.td, &t.dim, &t.N, &w, &h, &d,
+               &used) < 8 || used == 0) {
+      rc = CLK_INVALID_VALUE;
+      break;
+    }
+    t.dev = dev;
+    t.w = w;
+    t.h = h;
+    t.d = d;
+    Candidate c{};
+    const char* q = line + used;
+    char key[64];
+    int val = 0, adv = 0;
+    while (c.n < 4 && sscanf(q, " %63[a-z_]=%d%n", key, &val, &adv) == 2) {
+      int k = 0;
+      while (k < TUNE_COUNT && strcmp(tuning_name(k), key) != 0) ++k;
+      if (k == TUNE_COUNT) {
+        rc = CLK_INVALID_VALUE;
+        break;
+      }
+      c.set[c.n++] = TuneSetting{(TuneKey)k, (int32_t)val};
+      q += adv;
+    }
+    if (rc) break;
+    g_tuned[t] = c;
+  }
+  fclose(f);
+  return rc;
+}
 
 int32_t launch_separable(int op, hipStream_t st, const clk_image& dst, const clk_image& src,
                          int32_t dim, int32_t N, float s, std::string* err) {
@@ -310,6 +394,12 @@ int32_t launch_separable(int op, hipStream_t st, const clk_image& dst, const clk
         if (rc == 0) {
           std::lock_guard<std::mutex> lk(g_tune_mu);
           g_tuned[sig] = c;
+          if (getenv("CLK_AUTOTUNE_LOG")) {
+            fprintf(stderr, "clkmi autotune: op=%d src=%d dst=%d dim=%d N=%d %lldx%lldx%lld ->", op, src.dtype,
+                    dst.dtype, dim, N, (long long)W, (long long)H, (long long)D);
+            for (int i = 0; i < c.n; ++i) fprintf(stderr, " %s=%d", tuning_name(c.set[i].key), c.set[i].value);
+            fprintf(stderr, "\n");
+          }
         }
         done = true;  // the candidates already wrote dst
       }

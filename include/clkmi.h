@@ -183,11 +183,23 @@ typedef struct clk_filter_job {
  * leaves launching to its host. */
 int32_t clk_batch_filter_3d(void* stream, const clk_filter_job* jobs, int32_t njobs);
 
-/* Launch-shape knobs ("strided_rounds", "x_sub", "batch_seg", "no_xreg",
- * "strided_variant", "no_hist8"); the autotuner sets them.  Results never
- * depend on them. */
+/* Launch-shape knobs; results never depend on them.  Keys: "strided_rounds"
+ * (y/z blocks per resident slot), "x_sub" (LDS x pass segments per wave),
+ * "batch_seg", "no_xreg", "strided_variant", "no_hist8", "xreg_maxc",
+ * "ystatic_cols", "strided_order", "xreg_units", "x_dynamic", and
+ * "autotune": 1 = the first single-axis launch of each signature (pixel
+ * types, axis, N, volume size; volumes of 2^22 voxels and more) times a set
+ * of launch shapes on the caller's stream, synchronising it once, and later
+ * launches use the fastest.  Not for streams under graph capture (they are
+ * left untuned). */
 int32_t clk_set_tuning(const char* key, int32_t value);
 int32_t clk_get_tuning(const char* key, int32_t* value);
+
+/* Write the launch shapes the autotuner chose so far to a text file, or
+ * merge such a file into this process's choices (for the current device),
+ * so a later process launches the tuned shapes without timing them again. */
+int32_t clk_autotune_save(const char* path);
+int32_t clk_autotune_load(const char* path);
 
 /* ---- device memory and streams (for hosts without their own HIP) ------- */
 

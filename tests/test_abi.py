@@ -106,3 +106,24 @@ def test_tuning_knobs_validate_values():
             clkmi.set_tuning(key, old)
     with pytest.raises(clkmi.ClkError):
         clkmi.set_tuning("no_such_knob", 1)
+
+
+def test_autotune_cache_round_trip(tmp_path):
+    """Saved launch shapes load back (host only; no tuning runs here)."""
+    import clkmi
+    src = tmp_path / "shapes.txt"
+    src.write_text("# comment\n0 3 4 1 25 1024 1024 512 : strided_rounds=16 strided_order=1 ystatic_cols=2\n"
+                   "2 3 3 0 7 1024 1024 512 : xreg_units=4\n")
+    clkmi.autotune_load(src)
+    out = tmp_path / "out.txt"
+    clkmi.autotune_save(out)
+    lines = [l for l in out.read_text().splitlines() if not l.startswith("#")]
+    assert "0 3 4 1 25 1024 1024 512 : strided_rounds=16 strided_order=1 ystatic_cols=2" in lines
+    assert "2 3 3 0 7 1024 1024 512 : xreg_units=4" in lines
+    bad = tmp_path / "bad.txt"
+    bad.write_text("0 3 4 1 25 1024 1024 512 : no_such_knob=1\n")
+    with pytest.raises(clkmi.ClkError) as e:
+        clkmi.autotune_load(bad)
+    assert e.value.code == -30
+    with pytest.raises(clkmi.ClkError):
+        clkmi.autotune_load(tmp_path / "missing.txt")

@@ -263,7 +263,7 @@ def test_static_width_paths(gpu, shape, pair):
 KNOB_SETTINGS = [("strided_rounds", 2), ("strided_rounds", 8), ("strided_order", 1), ("strided_order", 2),
                  ("strided_order", 3), ("ystatic_cols", 1), ("ystatic_cols", 2), ("x_sub", 1), ("x_sub", 2),
                  ("x_sub", 4), ("no_xreg", 1), ("xreg_maxc", 16), ("strided_variant", 2), ("xreg_units", 2),
-                 ("xreg_units", 4)]
+                 ("xreg_units", 4), ("x_dynamic", 1)]
 
 
 @pytest.mark.parametrize("name,N,dst_t", [("gaussian_blur_separable_3d", 25, np.float32),
