@@ -19,7 +19,8 @@ enum TuneKey : int {
   TUNE_AUTOTUNE = 9,         // 1 = time launch shapes at the first launch of each signature
   TUNE_XREG_UNITS = 10,      // 0 = auto; 1, 2 or 4 units of 512 outputs per wave in the register x pass
   TUNE_X_DYNAMIC = 11,       // 1 = LDS x pass always takes the half width at run time
-  TUNE_COUNT = 12
+  TUNE_X_TASKS = 12,         // 0/1 = one task per wave in the static LDS x pass, 2 = two (prefetched)
+  TUNE_COUNT = 13
 };
 
 int32_t tuning(TuneKey k);

@@ -238,10 +238,8 @@ namespace clk {
 // One wave filters SUB*512 consecutive outputs of one row: SUB vector loads
 // per lane (all in flight together) plus c halo pixels per side, staged in
 // LDS, then SUB blocks of 8 outputs per lane through the register ring.  The
-// grid is one wave per task, in address order (no persistent loop).  SC > 0
-// fixes the half width at compile time (the tap loops unroll completely);
-// SC = 0 takes it from the parameters.
-template <int OP, class TS, class TD, int SUB, int SC = 0>
+// grid is one wave per task, in address order (no persistent loop).
+template <int OP, class TS, class TD, int SUB>
 __device__ __forceinline__ void sep_x_body(const SepParams& p, const uint32_t vblk, uint32_t* lds_raw) {
   using VT = vt_t<OP, TS>;
   constexpr bool kFloatSrc = !Pix<TS>::kInt;
@@ -251,8 +249,8 @@ __device__ __forceinline__ void sep_x_body(const SepParams& p, const uint32_t vb
   const int lane = threadIdx.x & 63;
   VT* const buf = reinterpret_cast<VT*>(lds_raw) + wv * p.lds_stride;
   const int64_t W = p.L;
-  const int c = SC > 0 ? SC : p.c;
-  const int ntaps = SC > 0 ? 2 * SC + 1 : p.ntaps;
+  const int c = p.c;
+  const int ntaps = p.ntaps;
   const uint32_t task = vblk * 4u + (uint32_t)wv;
   const bool valid = task < (uint32_t)p.ntasks;
   const uint32_t row = task / (uint32_t)p.nseg;
@@ -299,7 +297,6 @@ __device__ __forceinline__ void sep_x_body(const SepParams& p, const uint32_t vb
       else acc[q] = win[q];
     }
     int vv = 0;
-#pragma unroll
     for (; vv + K <= ntaps; vv += K) {
       const VT* lv = lb + (K + 1) * (vv / K);  // lds_pad(vv + r + K) = (K+1)*(vv/K) + r + K + 1
 #pragma unroll
@@ -336,11 +333,17 @@ __device__ __forceinline__ void sep_x_body(const SepParams& p, const uint32_t vb
   }
 }
 
-template <int OP, class TS, class TD, int SUB, int SC = 0>
+template <int OP, class TS, class TD, int SUB>
 __global__ __launch_bounds__(256) void sep_x_kernel(const SepParams p) {
   extern __shared__ uint32_t lds_raw[];
-  sep_x_body<OP, TS, TD, SUB, SC>(p, blockIdx.x, lds_raw);
+  sep_x_body<OP, TS, TD, SUB>(p, blockIdx.x, lds_raw);
 }
+
+}  // namespace clk
+
+#include "sep_xstatic.h"
+
+namespace clk {
 
 // ---- host-side launch of one (op, src type, dst type) ----------------------
 
@@ -522,26 +525,30 @@ int32_t launch_xreg(hipStream_t st, const SepParams& p, std::string* err) {
 // widths the register pass leaves to LDS on long rows (9..16), one or two
 // 512-output segments per wave.  Returns false when no such kernel exists.
 template <int OP, class TS, class TD, int C>
-bool launch_x_static_c(hipStream_t st, const SepParams& p, int sub, uint32_t blocks, size_t shmem) {
-  if (sub == 2) hipLaunchKernelGGL((sep_x_kernel<OP, TS, TD, 2, C>), dim3(blocks), dim3(256), shmem, st, p);
-  else if (sub == 1) hipLaunchKernelGGL((sep_x_kernel<OP, TS, TD, 1, C>), dim3(blocks), dim3(256), shmem, st, p);
+bool launch_x_static_c(hipStream_t st, const SepParams& p, int sub, size_t shmem) {
+  const int R = tuning(TUNE_X_TASKS) == 2 ? 2 : 1;
+  const uint32_t blocks = (uint32_t)((p.ntasks + 4 * R - 1) / (4 * R));
+  if (sub == 2 && R == 2) hipLaunchKernelGGL((sep_xs_kernel<OP, TS, TD, C, 2, 2>), dim3(blocks), dim3(256), shmem, st, p);
+  else if (sub == 2) hipLaunchKernelGGL((sep_xs_kernel<OP, TS, TD, C, 2, 1>), dim3(blocks), dim3(256), shmem, st, p);
+  else if (sub == 1 && R == 2) hipLaunchKernelGGL((sep_xs_kernel<OP, TS, TD, C, 1, 2>), dim3(blocks), dim3(256), shmem, st, p);
+  else if (sub == 1) hipLaunchKernelGGL((sep_xs_kernel<OP, TS, TD, C, 1, 1>), dim3(blocks), dim3(256), shmem, st, p);
   else return false;
   return true;
 }
 
 template <int OP, class TS, class TD>
-bool launch_x_static(hipStream_t st, const SepParams& p, int sub, uint32_t blocks, size_t shmem) {
+bool launch_x_static(hipStream_t st, const SepParams& p, int sub, size_t shmem) {
   if constexpr (OP == OP_GAUSS && (std::is_same<TS, TD>::value || std::is_same<TD, float>::value)) {
     if (tuning(TUNE_X_DYNAMIC)) return false;
     switch (p.c) {
-      case 9: return launch_x_static_c<OP, TS, TD, 9>(st, p, sub, blocks, shmem);
-      case 10: return launch_x_static_c<OP, TS, TD, 10>(st, p, sub, blocks, shmem);
-      case 11: return launch_x_static_c<OP, TS, TD, 11>(st, p, sub, blocks, shmem);
-      case 12: return launch_x_static_c<OP, TS, TD, 12>(st, p, sub, blocks, shmem);
-      case 13: return launch_x_static_c<OP, TS, TD, 13>(st, p, sub, blocks, shmem);
-      case 14: return launch_x_static_c<OP, TS, TD, 14>(st, p, sub, blocks, shmem);
-      case 15: return launch_x_static_c<OP, TS, TD, 15>(st, p, sub, blocks, shmem);
-      case 16: return launch_x_static_c<OP, TS, TD, 16>(st, p, sub, blocks, shmem);
+      case 9: return launch_x_static_c<OP, TS, TD, 9>(st, p, sub, shmem);
+      case 10: return launch_x_static_c<OP, TS, TD, 10>(st, p, sub, shmem);
+      case 11: return launch_x_static_c<OP, TS, TD, 11>(st, p, sub, shmem);
+      case 12: return launch_x_static_c<OP, TS, TD, 12>(st, p, sub, shmem);
+      case 13: return launch_x_static_c<OP, TS, TD, 13>(st, p, sub, shmem);
+      case 14: return launch_x_static_c<OP, TS, TD, 14>(st, p, sub, shmem);
+      case 15: return launch_x_static_c<OP, TS, TD, 15>(st, p, sub, shmem);
+      case 16: return launch_x_static_c<OP, TS, TD, 16>(st, p, sub, shmem);
     }
   }
   return false;
@@ -585,7 +592,7 @@ int32_t launch_sep_typed(hipStream_t st, SepParams& p, int axis_is_x, std::strin
     p.vec_st = (W % kXK == 0) && ((uintptr_t)p.dst % (kXK * sizeof(TD)) == 0);
     const size_t shmem = (size_t)4 * p.lds_stride * 4;
     const uint32_t blocks = (uint32_t)((p.ntasks + 3) / 4);
-    if (!launch_x_static<OP, TS, TD>(st, p, sub, blocks, shmem)) {
+    if (!launch_x_static<OP, TS, TD>(st, p, sub, shmem)) {
       if (sub == 4) hipLaunchKernelGGL((sep_x_kernel<OP, TS, TD, 4>), dim3(blocks), dim3(256), shmem, st, p);
       else if (sub == 2) hipLaunchKernelGGL((sep_x_kernel<OP, TS, TD, 2>), dim3(blocks), dim3(256), shmem, st, p);
       else hipLaunchKernelGGL((sep_x_kernel<OP, TS, TD, 1>), dim3(blocks), dim3(256), shmem, st, p);

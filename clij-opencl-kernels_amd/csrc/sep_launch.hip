@@ -210,6 +210,8 @@ const Candidate kXCands[] = {
     {{{TUNE_XREG_UNITS, 4}}, 1},
     {{{TUNE_NO_XREG, 1}, {TUNE_X_SUB, 1}}, 2},
     {{{TUNE_NO_XREG, 1}, {TUNE_X_SUB, 2}}, 2},
+    {{{TUNE_NO_XREG, 1}, {TUNE_X_SUB, 2}, {TUNE_X_TASKS, 2}}, 3},
+    {{{TUNE_NO_XREG, 1}, {TUNE_X_SUB, 1}, {TUNE_X_TASKS, 2}}, 3},
     {{{TUNE_NO_XREG, 1}, {TUNE_X_SUB, 4}}, 2},
     {{{TUNE_XREG_MAXC, 16}, {TUNE_XREG_UNITS, 4}}, 2},
     {{{TUNE_XREG_MAXC, 16}, {TUNE_XREG_UNITS, 2}}, 2},

@@ -263,7 +263,7 @@ def test_static_width_paths(gpu, shape, pair):
 KNOB_SETTINGS = [("strided_rounds", 2), ("strided_rounds", 8), ("strided_order", 1), ("strided_order", 2),
                  ("strided_order", 3), ("ystatic_cols", 1), ("ystatic_cols", 2), ("x_sub", 1), ("x_sub", 2),
                  ("x_sub", 4), ("no_xreg", 1), ("xreg_maxc", 16), ("strided_variant", 2), ("xreg_units", 2),
-                 ("xreg_units", 4), ("x_dynamic", 1)]
+                 ("xreg_units", 4), ("x_dynamic", 1), ("x_tasks", 2)]
 
 
 @pytest.mark.parametrize("name,N,dst_t", [("gaussian_blur_separable_3d", 25, np.float32),
@@ -307,3 +307,27 @@ def test_register_x_pass_several_units_per_wave(gpu, shape, units):
                 assert_parity(got, want, "%s %s c=%d units=%d" % (name, shape, c, units))
     finally:
         clkmi.set_tuning("xreg_units", old)
+
+
+@pytest.mark.parametrize("tasks", [1, 2])
+@pytest.mark.parametrize("sub", [1, 2])
+@pytest.mark.parametrize("shape", [(3, 5, 520), (2, 7, 1544), (4, 3, 1030), (1, 1, 2100)])
+def test_static_lds_x_pass(gpu, shape, sub, tasks):
+    """The LDS x pass with a compile-time half width (Gaussian c = 9..16 on
+    rows longer than 256): 1 or 2 segments of 512 per task, 1 or 2 tasks per
+    wave, unaligned row lengths, partial last segments."""
+    import clkmi
+    keys = {"x_sub": sub, "x_tasks": tasks}
+    old = {k: clkmi.get_tuning(k) for k in keys}
+    try:
+        for k, v in keys.items():
+            clkmi.set_tuning(k, v)
+        for src_t in (np.uint16, np.float32):
+            src = random_volume(shape, src_t, seed=shape[2] + sub)
+            for c in (9, 12, 16):
+                got = gpu_separable("gaussian_blur_separable_3d", src, np.float32, 0, 2 * c + 1, 0.4 * c)
+                want = O.separable("gaussian_blur_separable_3d", src, np.float32, 0, 2 * c + 1, 0.4 * c)
+                assert_parity(got, want, "%s %s c=%d sub=%d tasks=%d" % (shape, DTYPE_NAMES[src_t], c, sub, tasks))
+    finally:
+        for k, v in old.items():
+            clkmi.set_tuning(k, v)
