@@ -144,12 +144,14 @@ __device__ __forceinline__ T forgetful_median(T (&a)[N]) {
   return vmax(vmin(x0, x1), vmin(vmax(x0, x1), x2));
 }
 
-// 3x3x3 median, 8-bit dst: a block filters 512 consecutive x of one (y, z).
-// The 9 source rows (x-1 .. x+512, clamped) are staged in LDS as 16-bit
-// value pairs twice, pairs starting at even and at odd positions, so the
-// three pairs a thread needs per row (its two outputs shifted by -1, 0, +1)
-// are three aligned 32-bit reads; the selection network then runs on two
-// voxels per instruction.
+// 3x3x3 median, 8- or 16-bit integer dst: a block filters 512 consecutive x
+// of one (y, z).  The 9 source rows (x-1 .. x+512, clamped), converted to
+// the dst type and biased to unsigned 16-bit keys (signed types shift by
+// -lo, which keeps their order), are staged in LDS as 16-bit value pairs
+// twice, pairs starting at even and at odd positions, so the three pairs a
+// thread needs per row (its two outputs shifted by -1, 0, +1) are three
+// aligned 32-bit reads; the selection network then runs on two voxels per
+// instruction (v_pk_min_u16 / v_pk_max_u16).
 template <class TS, class TD>
 __device__ __forceinline__ void median_27_pk8_body(const MedParams& p, uint32_t bx, uint32_t by, uint32_t bz) {
   __shared__ uint32_t pe[9][257];
@@ -159,7 +161,7 @@ __device__ __forceinline__ void median_27_pk8_body(const MedParams& p, uint32_t 
   const int64_t y = by, z = bz;
   const int W = (int)p.W;
   const TS* __restrict__ src = static_cast<const TS*>(p.src);
-  constexpr int kBias = -static_cast<int>(Pix<TD>::lo);  // i8 values shift to 0..255
+  constexpr int kBias = -static_cast<int>(Pix<TD>::lo);  // signed values shift to 0..2^bits-1
   auto val = [&](const TS* row, int64_t x) -> uint32_t {
     x = x < 0 ? 0 : (x >= W ? W - 1 : x);
     return (uint32_t)((int)gather_value<TS, TD>(row[x]) + kBias);
@@ -324,8 +326,8 @@ enum MedKind : int { MED_PK8 = 0, MED_HIST8 = 1, MED_27 = 2, MED_GENERIC = 3 };
 // Kernel and grid for one median launch; sets p.gx, p.gy.
 inline MedKind plan_median(MedParams& p, int td_size, dim3* grid, int* threads) {
   MedKind k;
-  if (td_size == 1 && p.ex == 1 && p.ey == 1 && p.ez == 1) {
-    k = MED_PK8;
+  if ((td_size == 1 || td_size == 2) && p.ex == 1 && p.ey == 1 && p.ez == 1) {
+    k = MED_PK8;  // integer dst of 8 or 16 bits (float dst is 4 bytes)
     *grid = dim3((uint32_t)((p.W + 511) / 512), (uint32_t)p.H, (uint32_t)p.D);
     *threads = 256;
   } else if (td_size == 1 && p.n <= 255 && hist8_enabled()) {
