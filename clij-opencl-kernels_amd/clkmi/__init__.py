@@ -21,6 +21,8 @@ __all__ = [
     "minimum_separable_3d", "maximum_separable_3d", "median_box_3d", "gaussian_blur_3d",
     "mean_box_3d", "minimum_box_3d", "maximum_box_3d", "sigma_to_kernel_size", "KERNELS",
     "FilterJob", "batch_filter_3d", "set_tuning", "get_tuning", "autotune_save", "autotune_load",
+    "gaussian_blur_separable_2d", "mean_separable_2d", "minimum_separable_2d", "maximum_separable_2d",
+    "median_box_2d", "gaussian_blur_2d", "mean_box_2d", "minimum_box_2d", "maximum_box_2d",
 ]
 
 # Kernel name -> declared parameters (reference .cl declarations).
@@ -30,6 +32,11 @@ KERNELS = {
     "minimum_separable_3d": ("dst", "src", "dim", "N", "s"),        # minimum_separable_3d_x.cl:5-9
     "maximum_separable_3d": ("dst", "src", "dim", "N", "s"),        # maximum_separable_3d_x.cl:5-9
     "median_box_3d": ("dst", "src", "Nx", "Ny", "Nz"),              # median_box_3d_x.cl:25-29
+    "gaussian_blur_separable_2d": ("dst", "src", "dim", "N", "s"),  # gaussian_blur_separable_2d_x.cl:8-12
+    "mean_separable_2d": ("dst", "src", "dim", "N", "s"),           # mean_separable_2d_x.cl:5-9
+    "minimum_separable_2d": ("dst", "src", "dim", "N", "s"),        # minimum_separable_2d_x.cl:5-9
+    "maximum_separable_2d": ("dst", "src", "dim", "N", "s"),        # maximum_separable_2d_x.cl:5-9
+    "median_box_2d": ("dst", "src", "Nx", "Ny"),                    # median_box_2d_x.cl:24-27
 }
 
 Image = namedtuple("Image", "ptr dtype width height depth")
@@ -48,19 +55,19 @@ def _torch_codes():
 
 
 def to_clk_image(img):
-    """Describe a torch tensor (z, y, x) or an Image as a clk_image."""
+    """Describe a torch tensor (z, y, x) or (y, x), or an Image, as a clk_image."""
     if isinstance(img, Image):
         return _lib.ClkImage(img.ptr, int(img.dtype), 0, int(img.width), int(img.height), int(img.depth))
     codes = _torch_codes()
     if img.dtype not in codes:
         raise TypeError("unsupported pixel type %s (char/uchar/short/ushort/float)" % img.dtype)
-    if img.dim() != 3:
-        raise ValueError("images are 3D tensors shaped (depth, height, width)")
+    if img.dim() not in (2, 3):
+        raise ValueError("images are tensors shaped (depth, height, width) or (height, width)")
     if not img.is_contiguous():
         raise ValueError("images must be contiguous")
     if img.device.type != "cuda":
         raise ValueError("images must live on the GPU")
-    d, h, w = img.shape
+    d, h, w = img.shape if img.dim() == 3 else (1,) + tuple(img.shape)
     return _lib.ClkImage(img.data_ptr(), codes[img.dtype], 0, w, h, d)
 
 
@@ -111,6 +118,30 @@ def median_box_3d(dst, src, Nx, Ny, Nz, stream=None):
     return dst
 
 
+def gaussian_blur_separable_2d(dst, src, dim, N, s, stream=None):
+    """gaussian_blur_separable_2d(dst, src, dim, N, s) on images of depth 1."""
+    return _sep("gaussian_blur_separable_2d", dst, src, dim, N, s, stream)
+
+
+def mean_separable_2d(dst, src, dim, N, s=0.0, stream=None):
+    return _sep("mean_separable_2d", dst, src, dim, N, s, stream)
+
+
+def minimum_separable_2d(dst, src, dim, N, s=0.0, stream=None):
+    return _sep("minimum_separable_2d", dst, src, dim, N, s, stream)
+
+
+def maximum_separable_2d(dst, src, dim, N, s=0.0, stream=None):
+    return _sep("maximum_separable_2d", dst, src, dim, N, s, stream)
+
+
+def median_box_2d(dst, src, Nx, Ny, stream=None):
+    """median_box_2d(dst, src, Nx, Ny): integer values narrowed by a C cast."""
+    d, sr = to_clk_image(dst), to_clk_image(src)
+    check(lib().clk_median_box_2d(_stream_ptr(stream), ctypes.byref(d), ctypes.byref(sr), int(Nx), int(Ny)))
+    return dst
+
+
 def execute(kernel_name, parameters, stream=None):
     """Launch a kernel by name with parameters bound by name, as a CLIJ host's
     execute(kernelName, parameterMap).  The name may also be the program file
@@ -146,6 +177,32 @@ def gaussian_blur_3d(src, dst, sigma_x, sigma_y, sigma_z, stream=None):
     check(lib().clk_gaussian_blur_3d(_stream_ptr(stream), ctypes.byref(d), ctypes.byref(sr),
                                      float(sigma_x), float(sigma_y), float(sigma_z)))
     return dst
+
+
+def gaussian_blur_2d(src, dst, sigma_x, sigma_y, stream=None):
+    """2D Gaussian (depth-1 images) as two single-axis passes."""
+    d, sr = to_clk_image(dst), to_clk_image(src)
+    check(lib().clk_gaussian_blur_2d(_stream_ptr(stream), ctypes.byref(d), ctypes.byref(sr),
+                                     float(sigma_x), float(sigma_y)))
+    return dst
+
+
+def _box2(name, src, dst, rx, ry, stream):
+    d, sr = to_clk_image(dst), to_clk_image(src)
+    check(getattr(lib(), "clk_" + name)(_stream_ptr(stream), ctypes.byref(d), ctypes.byref(sr), int(rx), int(ry)))
+    return dst
+
+
+def mean_box_2d(src, dst, radius_x, radius_y, stream=None):
+    return _box2("mean_box_2d", src, dst, radius_x, radius_y, stream)
+
+
+def minimum_box_2d(src, dst, radius_x, radius_y, stream=None):
+    return _box2("minimum_box_2d", src, dst, radius_x, radius_y, stream)
+
+
+def maximum_box_2d(src, dst, radius_x, radius_y, stream=None):
+    return _box2("maximum_box_2d", src, dst, radius_x, radius_y, stream)
 
 
 def _box(name, src, dst, rx, ry, rz, stream):

@@ -32,6 +32,9 @@ EXPORTED = (
     "clk_free", "clk_memcpy_h2d", "clk_memcpy_d2h", "clk_stream_create", "clk_stream_destroy",
     "clk_stream_sync", "clk_dtype_size", "clk_last_error_string", "clk_version",
     "clk_batch_filter_3d", "clk_set_tuning", "clk_get_tuning", "clk_autotune_save", "clk_autotune_load",
+    "clk_gaussian_blur_separable_2d", "clk_mean_separable_2d", "clk_minimum_separable_2d",
+    "clk_maximum_separable_2d", "clk_median_box_2d", "clk_gaussian_blur_2d", "clk_mean_box_2d",
+    "clk_minimum_box_2d", "clk_maximum_box_2d",
 )
 
 CLK_JOB_GAUSSIAN_BLUR_3D, CLK_JOB_MEAN_BOX_3D, CLK_JOB_MINIMUM_BOX_3D, CLK_JOB_MAXIMUM_BOX_3D, \
@@ -100,6 +103,19 @@ def lib():
             fn.restype = i32
         L.clk_median_box_3d.argtypes = [vp, img, img, i32, i32, i32]
         L.clk_median_box_3d.restype = i32
+        for name in ("clk_gaussian_blur_separable_2d", "clk_mean_separable_2d",
+                     "clk_minimum_separable_2d", "clk_maximum_separable_2d"):
+            fn = getattr(L, name)
+            fn.argtypes = [vp, img, img, i32, i32, f32]
+            fn.restype = i32
+        L.clk_median_box_2d.argtypes = [vp, img, img, i32, i32]
+        L.clk_median_box_2d.restype = i32
+        L.clk_gaussian_blur_2d.argtypes = [vp, img, img, f32, f32]
+        L.clk_gaussian_blur_2d.restype = i32
+        for name in ("clk_mean_box_2d", "clk_minimum_box_2d", "clk_maximum_box_2d"):
+            fn = getattr(L, name)
+            fn.argtypes = [vp, img, img, i32, i32]
+            fn.restype = i32
         L.clk_execute.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ClkArg), i32]
         L.clk_execute.restype = i32
         L.clk_gaussian_blur_3d.argtypes = [vp, img, img, f32, f32, f32]

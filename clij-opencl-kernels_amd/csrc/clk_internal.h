@@ -67,9 +67,10 @@ int32_t prepare_separable(int op, const clk_image& dst, const clk_image& src, in
                           float s, SepParams& p, int* axis_is_x, bool* fill_nan, std::string* err);
 int32_t launch_fill_nan(hipStream_t st, const clk_image& dst, std::string* err);
 
-// median_box_3d launch; arguments validated by the caller.
+// median_box_3d launch (median_box_2d with wrap = true and Nz = 1: the 2D
+// kernel's C cast of integer values); arguments validated by the caller.
 int32_t launch_median(hipStream_t st, const clk_image& dst, const clk_image& src,
-                      int32_t Nx, int32_t Ny, int32_t Nz, std::string* err);
+                      int32_t Nx, int32_t Ny, int32_t Nz, std::string* err, bool wrap = false);
 
 // Run a batch of 3D filter jobs stage by stage, one launch per kernel class
 // and stage (batch.hip).

@@ -12,6 +12,8 @@ struct MedParams {
   int32_t ex, ey, ez;
   int32_t n;
   uint32_t gx, gy;  // grid of the chosen kernel in x and y (batched launches)
+  int32_t wrap;     // 1: median_box_2d's C cast of integer values (see gather_value)
+  int32_t reserved;
 };
 
 }  // namespace clk

@@ -63,6 +63,31 @@ int32_t separable(int op, const char* name, void* stream, const clk_image* dst, 
   return ok();
 }
 
+// The 2D kernels (…_2d_x.cl) index (i, j) only: an image of depth 1.
+int32_t check_2d(const char* kernel, const clk_image* dst, const clk_image* src) {
+  if ((dst && dst->depth != 1) || (src && src->depth != 1))
+    return fail(CLK_INVALID_VALUE, std::string(kernel) + ": 2D kernels take images of depth 1");
+  return CLK_SUCCESS;
+}
+
+// gaussian/mean/minimum/maximum_separable_2d: the 3D pass on a W x H x 1
+// volume.  dir = (dim==0, dim==1) in 2D and the z pass of a one-plane volume
+// reads the centre at every tap, so every dim gives the 2D kernel's result;
+// the 2D minimum/maximum skip v = 0 (…_separable_2d_x.cl:21), which cannot
+// change the result: res starts at the centre and only ever decreases
+// (increases), so min(res, centre) keeps res.
+int32_t separable_2d(int op, const char* name, void* stream, const clk_image* dst, const clk_image* src,
+                     int32_t dim, int32_t N, float s) {
+  int32_t rc = check_pair(name, dst, src);
+  if (rc) return rc;
+  rc = check_2d(name, dst, src);
+  if (rc) return rc;
+  std::string err;
+  rc = clk::launch_separable(op, (hipStream_t)stream, *dst, *src, dim, N, s, &err);
+  if (rc) return fail(rc, std::string(name) + ": " + err);
+  return ok();
+}
+
 // ---- host chain -------------------------------------------------------------
 
 // Kernel size of a Gaussian sigma: int(8*sigma), made odd.  The reference
@@ -203,23 +228,63 @@ int32_t clk_median_box_3d(void* stream, const clk_image* dst, const clk_image* s
   return ok();
 }
 
+int32_t clk_gaussian_blur_separable_2d(void* stream, const clk_image* dst, const clk_image* src,
+                                       int32_t dim, int32_t N, float s) {
+  return separable_2d(clk::OP_GAUSS, "gaussian_blur_separable_2d", stream, dst, src, dim, N, s);
+}
+
+int32_t clk_mean_separable_2d(void* stream, const clk_image* dst, const clk_image* src, int32_t dim,
+                              int32_t N, float s) {
+  return separable_2d(clk::OP_MEAN, "mean_separable_2d", stream, dst, src, dim, N, s);
+}
+
+int32_t clk_minimum_separable_2d(void* stream, const clk_image* dst, const clk_image* src,
+                                 int32_t dim, int32_t N, float s) {
+  return separable_2d(clk::OP_MIN, "minimum_separable_2d", stream, dst, src, dim, N, s);
+}
+
+int32_t clk_maximum_separable_2d(void* stream, const clk_image* dst, const clk_image* src,
+                                 int32_t dim, int32_t N, float s) {
+  return separable_2d(clk::OP_MAX, "maximum_separable_2d", stream, dst, src, dim, N, s);
+}
+
+int32_t clk_median_box_2d(void* stream, const clk_image* dst, const clk_image* src, int32_t Nx,
+                          int32_t Ny) {
+  int32_t rc = check_pair("median_box_2d", dst, src);
+  if (rc) return rc;
+  rc = check_2d("median_box_2d", dst, src);
+  if (rc) return rc;
+  std::string err;
+  rc = clk::launch_median((hipStream_t)stream, *dst, *src, Nx, Ny, 1, &err, true);
+  if (rc) return fail(rc, err);
+  return ok();
+}
+
 int32_t clk_execute(void* stream, const char* kernel_name, const clk_arg* args, int32_t nargs) {
   if (!kernel_name) return fail(CLK_INVALID_KERNEL_NAME, "kernel name is NULL");
   if (nargs < 0 || (nargs > 0 && !args)) return fail(CLK_INVALID_KERNEL_ARGS, "invalid argument list");
   std::string k;
   strip_program_name(kernel_name, &k);
   int op = -1;
-  if (k == "gaussian_blur_separable_3d") op = clk::OP_GAUSS;
-  else if (k == "mean_separable_3d") op = clk::OP_MEAN;
-  else if (k == "minimum_separable_3d") op = clk::OP_MIN;
-  else if (k == "maximum_separable_3d") op = clk::OP_MAX;
-  else if (k == "median_box_3d") op = 4;
+  bool two_d = false;
+  std::string base = k;
+  if (k.size() > 3 && k.compare(k.size() - 3, 3, "_2d") == 0) {
+    two_d = true;
+    base = k.substr(0, k.size() - 3) + "_3d";
+  }
+  if (base == "gaussian_blur_separable_3d") op = clk::OP_GAUSS;
+  else if (base == "mean_separable_3d") op = clk::OP_MEAN;
+  else if (base == "minimum_separable_3d") op = clk::OP_MIN;
+  else if (base == "maximum_separable_3d") op = clk::OP_MAX;
+  else if (base == "median_box_3d") op = 4;
   if (op < 0) return fail(CLK_INVALID_KERNEL_NAME, "unknown kernel: " + k);
   // parameter names in declaration order (gaussian_blur_separable_3d_x.cl:8-11,
-  // median_box_3d_x.cl:25-29)
+  // median_box_3d_x.cl:25-29; the 2D kernels: gaussian_blur_separable_2d_x.cl:8-12,
+  // median_box_2d_x.cl:24-27, no Nz)
   static const char* sep_names[3] = {"dim", "N", "s"};
   static const char* med_names[3] = {"Nx", "Ny", "Nz"};
   const char** names = (op == 4) ? med_names : sep_names;
+  const int nscalar = (op == 4 && two_d) ? 2 : 3;
   NamedArgs na;
   for (int32_t i = 0; i < nargs; ++i) {
     const clk_arg& a = args[i];
@@ -231,7 +296,7 @@ int32_t clk_execute(void* stream, const char* kernel_name, const clk_arg* args, 
       continue;
     }
     int slot = -1;
-    for (int j = 0; j < 3; ++j)
+    for (int j = 0; j < nscalar; ++j)
       if (nm == names[j]) slot = j;
     if (slot < 0) return fail(CLK_INVALID_KERNEL_ARGS, k + ": unknown parameter " + nm);
     if (a.kind == CLK_ARG_IMAGE) return fail(CLK_INVALID_KERNEL_ARGS, k + ": " + nm + " must be a number");
@@ -242,13 +307,15 @@ int32_t clk_execute(void* stream, const char* kernel_name, const clk_arg* args, 
   }
   if (!na.dst || !na.src) return fail(CLK_INVALID_KERNEL_ARGS, k + ": dst and src are required");
   if (op == 4) {
-    for (int j = 0; j < 3; ++j)
+    for (int j = 0; j < nscalar; ++j)
       if (!na.has[j]) return fail(CLK_INVALID_KERNEL_ARGS, k + ": missing parameter " + names[j]);
+    if (two_d) return clk_median_box_2d(stream, na.dst, na.src, na.iv[0], na.iv[1]);
     return clk_median_box_3d(stream, na.dst, na.src, na.iv[0], na.iv[1], na.iv[2]);
   }
   if (!na.has[0] || !na.has[1])
     return fail(CLK_INVALID_KERNEL_ARGS, k + ": missing parameter dim or N");
   if (op == clk::OP_GAUSS && !na.has[2]) return fail(CLK_INVALID_KERNEL_ARGS, k + ": missing parameter s");
+  if (two_d) return separable_2d(op, k.c_str(), stream, na.dst, na.src, na.iv[0], na.iv[1], na.fv);
   return separable(op, k.c_str(), stream, na.dst, na.src, na.iv[0], na.iv[1], na.fv);
 }
 
@@ -272,6 +339,32 @@ int32_t clk_minimum_box_3d(void* stream, const clk_image* dst, const clk_image* 
 int32_t clk_maximum_box_3d(void* stream, const clk_image* dst, const clk_image* src, int32_t rx,
                            int32_t ry, int32_t rz) {
   return box_chain(clk::OP_MAX, "maximum_box_3d", stream, dst, src, rx, ry, rz);
+}
+
+int32_t clk_gaussian_blur_2d(void* stream, const clk_image* dst, const clk_image* src, float sx, float sy) {
+  const int32_t rc = check_2d("gaussian_blur_2d", dst, src);
+  if (rc) return rc;
+  const PassSpec ps[3] = {{sx > 0, sigma_to_n(sx), sx}, {sy > 0, sigma_to_n(sy), sy}, {false, 1, 0.f}};
+  return chain3(clk::OP_GAUSS, "gaussian_blur_2d", stream, dst, src, ps);
+}
+
+static int32_t box_chain_2d(int op, const char* name, void* stream, const clk_image* dst, const clk_image* src,
+                            int32_t rx, int32_t ry) {
+  const int32_t rc = check_2d(name, dst, src);
+  if (rc) return rc;
+  return box_chain(op, name, stream, dst, src, rx, ry, 0);
+}
+
+int32_t clk_mean_box_2d(void* stream, const clk_image* dst, const clk_image* src, int32_t rx, int32_t ry) {
+  return box_chain_2d(clk::OP_MEAN, "mean_box_2d", stream, dst, src, rx, ry);
+}
+
+int32_t clk_minimum_box_2d(void* stream, const clk_image* dst, const clk_image* src, int32_t rx, int32_t ry) {
+  return box_chain_2d(clk::OP_MIN, "minimum_box_2d", stream, dst, src, rx, ry);
+}
+
+int32_t clk_maximum_box_2d(void* stream, const clk_image* dst, const clk_image* src, int32_t rx, int32_t ry) {
+  return box_chain_2d(clk::OP_MAX, "maximum_box_2d", stream, dst, src, rx, ry);
 }
 
 int32_t clk_batch_filter_3d(void* stream, const clk_filter_job* jobs, int32_t njobs) {

@@ -38,11 +38,17 @@ namespace clk {
 
 inline bool hist8_enabled() { return tuning(TUNE_NO_HIST8) == 0; }
 
-// (dst_t)(float)READ(src): C conversion of the float to the dst pixel type.
-// In range it truncates toward zero; out of range the reference's C
-// conversion is undefined and this one saturates (NaN -> 0).
+// A source value as the reference stores it in the median's dst_t array.
+// median_box_3d (wrap = 0): (dst_t)(float)READ(src), a C conversion of the
+// float; in range it truncates toward zero, out of range C leaves it
+// undefined and this one saturates (NaN -> 0).  median_box_2d (wrap = 1):
+// (dst_t)READ(src).x, a C cast of the widened int/uint, which for an integer
+// dst keeps the low bits (two's complement); float sources convert as above.
 template <class TS, class TD>
-__device__ __forceinline__ float gather_value(TS x) {
+__device__ __forceinline__ float gather_value(TS x, int32_t wrap) {
+  if constexpr (Pix<TS>::kInt && Pix<TD>::kInt) {
+    if (wrap) return static_cast<float>(static_cast<TD>(static_cast<int32_t>(x)));
+  }
   return static_cast<float>(convert_sat<TD>(static_cast<float>(x)));
 }
 
@@ -95,7 +101,7 @@ __device__ __forceinline__ void median_generic_body(const MedParams& p, uint32_t
         const TS* rowp = src + (zz * p.H + yy) * p.W;
         for (int dx = -p.ex; dx <= p.ex; ++dx) {
           const int64_t xx = clamp64(x + dx, 0, p.W - 1);
-          cnt += value_key<TD>(gather_value<TS, TD>(rowp[xx])) <= mid;
+          cnt += value_key<TD>(gather_value<TS, TD>(rowp[xx], p.wrap)) <= mid;
         }
       }
     }
@@ -164,7 +170,7 @@ __device__ __forceinline__ void median_27_pk8_body(const MedParams& p, uint32_t 
   constexpr int kBias = -static_cast<int>(Pix<TD>::lo);  // signed values shift to 0..2^bits-1
   auto val = [&](const TS* row, int64_t x) -> uint32_t {
     x = x < 0 ? 0 : (x >= W ? W - 1 : x);
-    return (uint32_t)((int)gather_value<TS, TD>(row[x]) + kBias);
+    return (uint32_t)((int)gather_value<TS, TD>(row[x], p.wrap) + kBias);
   };
 #pragma unroll
   for (int r = 0; r < 9; ++r) {
@@ -213,7 +219,7 @@ __device__ __forceinline__ void median_27_body(const MedParams& p, uint32_t bx, 
 #pragma unroll
       for (int dx = -1; dx <= 1; ++dx) {
         const int64_t xx = clamp64(x + dx, 0, p.W - 1);
-        a[i++] = gather_value<TS, TD>(rowp[xx]);
+        a[i++] = gather_value<TS, TD>(rowp[xx], p.wrap);
       }
     }
   }
@@ -262,7 +268,7 @@ __device__ __forceinline__ void median_hist8_body(const MedParams& p, uint32_t b
   auto bin_at = [&](const TS* row, int dx) -> int {
     int xx = xi + dx;
     xx = xx < 0 ? 0 : (xx >= W ? W - 1 : xx);
-    return bin8<TD>(gather_value<TS, TD>(row[xx]));
+    return bin8<TD>(gather_value<TS, TD>(row[xx], p.wrap));
   };
   auto word = [&](int b) -> uint32_t* { return h + (((b >> 2) + lane) & 63); };
   auto add = [&](int b, bool inc) {

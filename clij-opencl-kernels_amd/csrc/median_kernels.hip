@@ -1,4 +1,7 @@
-// median_box_3d single launches; the kernels are in median_impl.h.
+// median_box_3d / median_box_2d single launches; the kernels are in
+// median_impl.h.
+#include <string.h>
+
 #include "median_impl.h"
 
 namespace clk {
@@ -39,8 +42,10 @@ static int32_t median_dst(hipStream_t st, MedParams& p, int dst_t, std::string* 
 }
 
 int32_t launch_median(hipStream_t st, const clk_image& dst, const clk_image& src, int32_t Nx,
-                      int32_t Ny, int32_t Nz, std::string* err) {
+                      int32_t Ny, int32_t Nz, std::string* err, bool wrap) {
   MedParams p;
+  memset(&p, 0, sizeof(p));
+  p.wrap = wrap ? 1 : 0;
   p.src = src.ptr;
   p.dst = dst.ptr;
   p.W = dst.width;

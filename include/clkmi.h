@@ -102,6 +102,34 @@ int32_t clk_median_box_3d(void* stream, const clk_image* dst,
                           const clk_image* src, int32_t Nx, int32_t Ny,
                           int32_t Nz);
 
+/* ---- the 2D siblings (images of depth 1) -------------------------------- */
+
+/* gaussian_blur_separable_2d / mean_separable_2d / minimum_separable_2d /
+ * maximum_separable_2d (dst, src, dim, N, s) replace
+ * kernels/gaussian_blur_separable_2d_x.cl:6-13, mean_separable_2d_x.cl:3-10,
+ * minimum_separable_2d_x.cl:3-10, maximum_separable_2d_x.cl:3-10: the same
+ * per-pixel arithmetic as the 3D kernels at depth 1.  Images must have
+ * depth 1 (CLK_INVALID_VALUE otherwise). */
+int32_t clk_gaussian_blur_separable_2d(void* stream, const clk_image* dst,
+                                       const clk_image* src, int32_t dim,
+                                       int32_t N, float s);
+int32_t clk_mean_separable_2d(void* stream, const clk_image* dst,
+                              const clk_image* src, int32_t dim, int32_t N,
+                              float s);
+int32_t clk_minimum_separable_2d(void* stream, const clk_image* dst,
+                                 const clk_image* src, int32_t dim, int32_t N,
+                                 float s);
+int32_t clk_maximum_separable_2d(void* stream, const clk_image* dst,
+                                 const clk_image* src, int32_t dim, int32_t N,
+                                 float s);
+
+/* median_box_2d(dst, src, Nx, Ny) replaces kernels/median_box_2d_x.cl:22-28.
+ * Unlike median_box_3d it stores (dst_t)READ(src).x, a C cast of the widened
+ * integer: an integer narrowed to a smaller integer dst keeps its low bits
+ * (e.g. short -5 -> uchar 251) instead of saturating. */
+int32_t clk_median_box_2d(void* stream, const clk_image* dst,
+                          const clk_image* src, int32_t Nx, int32_t Ny);
+
 /* ---- by-name dispatch (CLIJ execute(kernelName, parameterMap)) --------- */
 
 typedef enum clk_arg_kind {
@@ -120,7 +148,8 @@ typedef struct clk_arg {
 } clk_arg;
 
 /* Launch a kernel by its reference name ("gaussian_blur_separable_3d", or the
- * program file name "gaussian_blur_separable_3d_x.cl") with arguments bound
+ * program file name "gaussian_blur_separable_3d_x.cl"; the ten names are the
+ * five 3D kernels and their _2d siblings) with arguments bound
  * by parameter name, as a CLIJ host binds them (reference README.md:26).
  * An int argument is accepted for a float parameter and vice versa, as the
  * reference hosts convert Java/Python numbers.  Unknown names or missing
@@ -148,6 +177,17 @@ int32_t clk_minimum_box_3d(void* stream, const clk_image* dst,
 int32_t clk_maximum_box_3d(void* stream, const clk_image* dst,
                            const clk_image* src, int32_t rx, int32_t ry,
                            int32_t rz);
+
+/* The same chains for 2D images (depth 1): passes x and y. */
+int32_t clk_gaussian_blur_2d(void* stream, const clk_image* dst,
+                             const clk_image* src, float sigma_x,
+                             float sigma_y);
+int32_t clk_mean_box_2d(void* stream, const clk_image* dst,
+                        const clk_image* src, int32_t rx, int32_t ry);
+int32_t clk_minimum_box_2d(void* stream, const clk_image* dst,
+                           const clk_image* src, int32_t rx, int32_t ry);
+int32_t clk_maximum_box_2d(void* stream, const clk_image* dst,
+                           const clk_image* src, int32_t rx, int32_t ry);
 
 /* ---- batches: many 3D filters in a few launches ------------------------ */
 

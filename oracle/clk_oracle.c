@@ -1,7 +1,7 @@
 /*
  * clk_oracle.c -- CPU restatement of the five 3D neighbourhood kernels of
- * clEsperanto/clij-opencl-kernels, voxel by voxel, in the reference's own
- * operation order.
+ * clEsperanto/clij-opencl-kernels and their five 2D siblings, voxel by
+ * voxel, in the reference's own operation order.
  *
  * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/,
  * __graft_entry__.smoke() and the cpu_baseline leg of bench.py may load it.
@@ -210,6 +210,127 @@ int clko_median_box_3d(void* dst, int dst_t, const void* src, int src_t, int64_t
           const float res = median_of(arr, count); /* :63 */
           write_px(dst, dst_t, i + j * W + k * W * H, res); /* :64, no CONVERT */
         }
+      }
+    }
+    free(arr);
+  }
+  return rc;
+}
+
+/* ---- the 2D siblings: images of one plane, coordinates (i, j) ---------- */
+
+/* gaussian_blur_separable_2d_x.cl:15-34 for one work-item (i, j). */
+static float gaussian_pixel_2d(const img_t* src, int64_t i, int64_t j, int dim, int N, float s) {
+  const int64_t di = (dim == 0), dj = (dim == 1); /* :19 */
+  const int c = (N - 1) / 2;                       /* :22 */
+  const float n = -2 * s * s;                      /* :24 */
+  float res = 0, hsum = 0;                         /* :26-27 */
+  for (int v = -c; v <= c; v++) {                  /* :28 */
+    const float h = clko_expf((float)(v * v) / n); /* :29, (v*v)/n with v*v an int */
+    res = fmaf(h, read_px(src, i + v * di, j + v * dj, 0), res); /* :30 */
+    hsum += h;                                     /* :31 */
+  }
+  return res / hsum; /* :33 */
+}
+
+/* mean_separable_2d_x.cl:12-27. */
+static float mean_pixel_2d(const img_t* src, int64_t i, int64_t j, int dim, int N) {
+  const int64_t di = (dim == 0), dj = (dim == 1); /* :15 */
+  const int c = (N - 1) / 2;                       /* :18 */
+  float res = 0, count = 0;                        /* :20-21 */
+  for (int v = -c; v <= c; v++) {                  /* :22 */
+    res += read_px(src, i + v * di, j + v * dj, 0); /* :23 */
+    count += 1;                                    /* :24 */
+  }
+  return res / count; /* :26 */
+}
+
+/* minimum_separable_2d_x.cl:12-25 / maximum_separable_2d_x.cl:12-25: unlike
+ * the 3D kernels the centre is not applied a second time (v != 0, :21). */
+static float minmax_pixel_2d(const img_t* src, int64_t i, int64_t j, int dim, int N, int is_max) {
+  const int64_t di = (dim == 0), dj = (dim == 1); /* :14 */
+  const int c = (N - 1) / 2;                       /* :17 */
+  float res = read_px(src, i, j, 0);               /* :19 */
+  for (int v = -c; v <= c; v++) {                  /* :20 */
+    if (v != 0) {                                  /* :21 */
+      const float x = read_px(src, i + v * di, j + v * dj, 0);
+      if (is_max) res = (res < x) ? x : res;       /* :22 max */
+      else res = (x < res) ? x : res;              /* :22 min */
+    }
+  }
+  return res;
+}
+
+/* The four separable 2D kernels; images W x H (one plane). */
+int clko_separable_2d(int op, void* dst, int dst_t, const void* src, int src_t, int64_t W, int64_t H, int dim,
+                      int N, float s, int nthreads) {
+  if (op < 0 || op > 3 || src_t < 0 || src_t > 4 || dst_t < 0 || dst_t > 4) return -30;
+  const img_t im = {src, src_t, W, H, 1};
+  if (nthreads < 1) nthreads = 1;
+  int64_t j;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+  for (j = 0; j < H; j++) {
+    for (int64_t i = 0; i < W; i++) {
+      float r;
+      switch (op) {
+        case 0: r = gaussian_pixel_2d(&im, i, j, dim, N, s); break;
+        case 1: r = mean_pixel_2d(&im, i, j, dim, N); break;
+        case 2: r = minmax_pixel_2d(&im, i, j, dim, N, 0); break;
+        default: r = minmax_pixel_2d(&im, i, j, dim, N, 1); break;
+      }
+      write_px(dst, dst_t, i + j * W, clko_convert(r, dst_t)); /* WRITE(CONVERT(res)) */
+    }
+  }
+  return 0;
+}
+
+/* (IMAGE_dst_PIXEL_TYPE)READ_src_IMAGE(...).x of median_box_2d_x.cl:45: a C
+ * cast of the widened pixel (int for char/short, uint for uchar/ushort,
+ * float).  An integer narrowed to a smaller integer type keeps its low bits
+ * (two's complement, as C does on every target OpenCL runs on); a float
+ * going to an integer type is undefined out of range in C and saturates here,
+ * as in median_box_3d. */
+static float cast_2d(float v, int src_t, int dst_t) {
+  if (src_t != T_FLOAT && dst_t != T_FLOAT) {
+    const int32_t x = (int32_t)v; /* exact: every <=16-bit integer */
+    switch (dst_t) {
+      case T_CHAR: return (float)(int8_t)(uint8_t)(uint32_t)x;
+      case T_UCHAR: return (float)(uint8_t)(uint32_t)x;
+      case T_SHORT: return (float)(int16_t)(uint16_t)(uint32_t)x;
+      default: return (float)(uint16_t)(uint32_t)x;
+    }
+  }
+  return to_dst_type(v, dst_t);
+}
+
+/* median_box_2d_x.cl:29-53 (sort/median :3-20 as in 3D).  Returns -30 when
+ * a negative N leaves the box empty. */
+int clko_median_box_2d(void* dst, int dst_t, const void* src, int src_t, int64_t W, int64_t H, int Nx, int Ny,
+                       int nthreads) {
+  if (src_t < 0 || src_t > 4 || dst_t < 0 || dst_t > 4) return -30;
+  const int ex = (Nx - 1) / 2, ey = (Ny - 1) / 2; /* :36 */
+  if (ex < 0 || ey < 0) return -30;
+  const int n = (2 * ex + 1) * (2 * ey + 1);
+  const img_t im = {src, src_t, W, H, 1};
+  if (nthreads < 1) nthreads = 1;
+  int rc = 0;
+#pragma omp parallel num_threads(nthreads)
+  {
+    float* arr = (float*)malloc(sizeof(float) * (size_t)n);
+    if (!arr) {
+#pragma omp atomic write
+      rc = -5;
+    }
+    int64_t j;
+#pragma omp for schedule(static)
+    for (j = 0; j < H; j++) {
+      if (!arr) continue;
+      for (int64_t i = 0; i < W; i++) {
+        int count = 0;
+        for (int x = -ex; x <= ex; x++)   /* :43 */
+          for (int y = -ey; y <= ey; y++) /* :44 */
+            arr[count++] = cast_2d(read_px(&im, i + x, j + y, 0), src_t, dst_t); /* :45 */
+        write_px(dst, dst_t, i + j * W, median_of(arr, count)); /* :52-53 */
       }
     }
     free(arr);

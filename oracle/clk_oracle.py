@@ -53,6 +53,10 @@ def lib():
         L.clko_separable.restype = i32
         L.clko_median_box_3d.argtypes = [vp, i32, vp, i32, i64, i64, i64, i32, i32, i32, i32]
         L.clko_median_box_3d.restype = i32
+        L.clko_separable_2d.argtypes = [i32, vp, i32, vp, i32, i64, i64, i32, i32, f32, i32]
+        L.clko_separable_2d.restype = i32
+        L.clko_median_box_2d.argtypes = [vp, i32, vp, i32, i64, i64, i32, i32, i32]
+        L.clko_median_box_2d.restype = i32
         L.clko_expf.argtypes = [f32]
         L.clko_expf.restype = f32
         L.clko_convert.argtypes = [f32, i32]
@@ -116,6 +120,35 @@ def median_box_3d(src, dst_dtype, Nx, Ny, Nz, nthreads=None):
     rc = lib().clko_median_box_3d(
         dst.ctypes.data, DTYPE_CODES[dst.dtype], src.ctypes.data, DTYPE_CODES[src.dtype],
         W, H, D, int(Nx), int(Ny), int(Nz), int(nthreads or default_threads()))
+    if rc != 0:
+        raise ValueError("oracle rejected arguments (%d)" % rc)
+    return dst
+
+
+def separable_2d(op, src, dst_dtype, dim, N, s=0.0, nthreads=None):
+    """The 2D separable kernels (…_separable_2d_x.cl) on a (H, W) image."""
+    if isinstance(op, str):
+        op = OP_NAMES[op.replace("_2d", "_3d")]
+    src = np.ascontiguousarray(src)
+    assert src.ndim == 2
+    dst = np.empty(src.shape, dtype=np.dtype(dst_dtype))
+    H, W = src.shape
+    rc = lib().clko_separable_2d(op, dst.ctypes.data, DTYPE_CODES[dst.dtype], src.ctypes.data,
+                                 DTYPE_CODES[src.dtype], W, H, int(dim), int(N), float(s),
+                                 int(nthreads or default_threads()))
+    if rc != 0:
+        raise ValueError("oracle rejected arguments (%d)" % rc)
+    return dst
+
+
+def median_box_2d(src, dst_dtype, Nx, Ny, nthreads=None):
+    """median_box_2d_x.cl on a (H, W) image (C-cast narrowing of integers)."""
+    src = np.ascontiguousarray(src)
+    assert src.ndim == 2
+    dst = np.empty(src.shape, dtype=np.dtype(dst_dtype))
+    H, W = src.shape
+    rc = lib().clko_median_box_2d(dst.ctypes.data, DTYPE_CODES[dst.dtype], src.ctypes.data,
+                                  DTYPE_CODES[src.dtype], W, H, int(Nx), int(Ny), int(nthreads or default_threads()))
     if rc != 0:
         raise ValueError("oracle rejected arguments (%d)" % rc)
     return dst

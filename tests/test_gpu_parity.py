@@ -331,3 +331,99 @@ def test_static_lds_x_pass(gpu, shape, sub, tasks):
     finally:
         for k, v in old.items():
             clkmi.set_tuning(k, v)
+
+
+# ---- the 2D siblings (images of depth 1) ----------------------------------------
+
+SEP_OPS_2D = ["gaussian_blur_separable_2d", "mean_separable_2d", "minimum_separable_2d", "maximum_separable_2d"]
+
+
+def _image_2d(a, dtype=None):
+    """Device image of depth 1 from a (H, W) array (or an empty one)."""
+    if dtype is None:
+        return to_device(a[None])
+    return empty_device((1,) + tuple(a), dtype)
+
+
+def _specials_2d(dt, shape, seed):
+    a = random_volume(shape, dt, seed)
+    if np.dtype(dt).kind == "f":
+        flat = a.reshape(-1)
+        flat[::7] = np.nan
+        flat[3::11] = np.inf
+        flat[2::17] = -0.0
+    return a
+
+
+@pytest.mark.parametrize("name", SEP_OPS_2D)
+@pytest.mark.parametrize("src_t", DTYPES, ids=lambda t: DTYPE_NAMES[t])
+@pytest.mark.parametrize("dst_t", DTYPES, ids=lambda t: DTYPE_NAMES[t])
+def test_2d_separable_all_type_pairs(gpu, name, src_t, dst_t):
+    """clk_*_separable_2d against the oracle's restatement of the 2D kernels
+    (every dim incl. out of range, odd/even N, wide Gaussians)."""
+    import clkmi
+    a = _specials_2d(src_t, (37, 70), seed=zlib.crc32((name + DTYPE_NAMES[src_t]).encode()))
+    ts, si = _image_2d(a)
+    for dim in (0, 1, 2, -1):
+        for N, s in ((3, 0.8), (4, 1.2), (7, 2.0), (25, 3.0)):
+            td, di = _image_2d(a.shape, dst_t)
+            getattr(clkmi, name)(di, si, dim, N, s)
+            got = from_device(td, a.shape, dst_t)
+            want = O.separable_2d(name, a, dst_t, dim, N, s)
+            assert_parity(got, want, "%s dim=%d N=%d" % (name, dim, N))
+
+
+@pytest.mark.parametrize("src_t", DTYPES, ids=lambda t: DTYPE_NAMES[t])
+@pytest.mark.parametrize("dst_t", DTYPES, ids=lambda t: DTYPE_NAMES[t])
+def test_2d_median_all_type_pairs(gpu, src_t, dst_t):
+    """clk_median_box_2d (C-cast narrowing of integers) against the oracle,
+    through the packed 3x3 path, the histogram path and the generic one."""
+    import clkmi
+    a = random_volume((29, 77), src_t, seed=41)
+    ts, si = _image_2d(a)
+    for Nx, Ny in ((3, 3), (5, 5), (1, 7), (3, 1), (4, 2)):
+        td, di = _image_2d(a.shape, dst_t)
+        clkmi.median_box_2d(di, si, Nx, Ny)
+        got = from_device(td, a.shape, dst_t)
+        want = O.median_box_2d(a, dst_t, Nx, Ny)
+        assert_parity(got, want, "median_box_2d %dx%d" % (Nx, Ny))
+
+
+def test_2d_by_name_chains_and_errors(gpu):
+    """execute() with the _2d names and program file names, the 2D host
+    chains, and the depth-1 rule."""
+    import clkmi
+    a = random_volume((40, 90), np.uint16, seed=3)
+    ts, si = _image_2d(a)
+    td1, d1 = _image_2d(a.shape, np.uint8)
+    td2, d2 = _image_2d(a.shape, np.uint8)
+    clkmi.median_box_2d(d1, si, 3, 3)
+    clkmi.execute("median_box_2d_x.cl", {"dst": d2, "src": si, "Nx": 3, "Ny": 3})
+    assert np.array_equal(from_device(td1, a.shape, np.uint8), from_device(td2, a.shape, np.uint8))
+    tf, df = _image_2d(a.shape, np.float32)
+    clkmi.execute("gaussian_blur_separable_2d", {"dst": df, "src": si, "dim": 1, "N": 9, "s": 1.5})
+    assert_parity(from_device(tf, a.shape, np.float32),
+                  O.separable_2d("gaussian_blur_separable_2d", a, np.float32, 1, 9, 1.5), "execute 2d")
+    # 2D chains: x then y with float temporaries (min/max: the source type)
+    tg, dg = _image_2d(a.shape, np.float32)
+    clkmi.gaussian_blur_2d(si, dg, 1.5, 0.7)
+    n1, n2 = O.sigma_to_n(1.5), O.sigma_to_n(0.7)
+    want = O.separable_2d("gaussian_blur_separable_2d",
+                          O.separable_2d("gaussian_blur_separable_2d", a, np.float32, 0, n1, 1.5), np.float32, 1, n2, 0.7)
+    assert_parity(from_device(tg, a.shape, np.float32), want, "gaussian_blur_2d")
+    tm, dm = _image_2d(a.shape, np.uint16)
+    clkmi.maximum_box_2d(si, dm, 2, 1)
+    want = O.separable_2d("maximum_separable_2d", O.separable_2d("maximum_separable_2d", a, np.uint16, 0, 5), np.uint16, 1, 3)
+    assert_parity(from_device(tm, a.shape, np.uint16), want, "maximum_box_2d")
+    # a 2D kernel needs depth 1
+    t3, s3 = to_device(random_volume((2, 8, 8), np.uint16, seed=1))
+    t4, d4 = empty_device((2, 8, 8), np.uint16)
+    with pytest.raises(clkmi.ClkError) as e:
+        clkmi.median_box_2d(d4, s3, 3, 3)
+    assert e.value.code == -30
+    with pytest.raises(clkmi.ClkError) as e:
+        clkmi.execute("mean_separable_2d", {"dst": d4, "src": s3, "dim": 0, "N": 3, "s": 0.0})
+    assert e.value.code == -30
+    with pytest.raises(clkmi.ClkError) as e:  # median_box_2d has no Nz
+        clkmi.execute("median_box_2d", {"dst": d1, "src": si, "Nx": 3, "Ny": 3, "Nz": 3})
+    assert e.value.code == -52

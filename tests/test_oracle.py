@@ -10,7 +10,9 @@ import pytest
 from scipy import ndimage
 
 import clk_oracle as O
-from helpers import DTYPES, random_volume
+import zlib
+
+from helpers import DTYPES, random_volume, same_bits_or_value
 
 SHAPES = [(11, 13, 17), (5, 1, 9), (1, 1, 23)]
 
@@ -272,3 +274,66 @@ def test_div_uniform_equals_ieee_divide():
             got = np.float32(L.clko_div_uniform(float(a), float(b), float(r)))
             want = np.float32(a) / np.float32(b)
             assert (np.isnan(got) and np.isnan(want)) or got.tobytes() == want.tobytes(), (a, b)
+
+
+# ---- the 2D siblings -----------------------------------------------------------
+
+def _specials_2d(dt, shape, seed):
+    """Seeded 2D data; float images also hold NaN, +-inf and +-0 so that the
+    2D kernels' centre skip (min/max) meets order-sensitive values."""
+    a = random_volume(shape, dt, seed)
+    if np.dtype(dt).kind == "f":
+        flat = a.reshape(-1)
+        flat[::7] = np.nan
+        flat[3::11] = np.inf
+        flat[5::13] = -np.inf
+        flat[2::17] = -0.0
+        flat[4::19] = 0.0
+    return a
+
+
+@pytest.mark.parametrize("op", ["gaussian_blur_separable_2d", "mean_separable_2d", "minimum_separable_2d",
+                                "maximum_separable_2d"])
+@pytest.mark.parametrize("src_t", DTYPES)
+@pytest.mark.parametrize("dst_t", [np.uint8, np.int16, np.float32])
+def test_2d_separable_equals_3d_at_depth_one(op, src_t, dst_t):
+    """The 2D kernels restated from their own text (dir = (dim==0, dim==1),
+    min/max without the second centre tap) give the 3D kernels' results on a
+    one-plane volume for every dim, N and value, NaN and infinities included:
+    the equivalence the library's 2D entry points rely on."""
+    a = _specials_2d(src_t, (9, 23), seed=zlib.crc32((op + str(src_t)).encode()))
+    op3 = op.replace("_2d", "_3d")
+    for dim in (0, 1, 2, 5, -1):
+        for N, s in ((1, 1.0), (3, 0.8), (4, 1.3), (7, 2.0), (-3, 1.0)):
+            got = O.separable_2d(op, a, dst_t, dim, N, s)
+            want = O.separable(op3, a[None], dst_t, dim, N, s)[0]
+            assert same_bits_or_value(got, want), (op, dim, N)
+
+
+def test_2d_median_casts_integers_like_c():
+    """median_box_2d stores (dst_t)READ(src).x: integers narrow to their low
+    bits (numpy's astype is the same two's-complement cast), unlike the 3D
+    kernel's saturating path."""
+    a = np.array([[-5, 300, 7, -129], [70000 % 65536 - 32768, 2, 3, 255]], dtype=np.int16)
+    got = O.median_box_2d(a, np.uint8, 1, 1)
+    assert got.tolist() == a.astype(np.uint8).tolist()
+    assert got[0, 0] == 251 and got[0, 1] == 44
+    assert O.median_box_3d(a[None], np.uint8, 1, 1, 1)[0, 0, 0] == 0  # 3D saturates
+
+
+@pytest.mark.parametrize("src_t", [np.int8, np.uint8, np.int16, np.uint16])
+@pytest.mark.parametrize("dst_t", [np.int8, np.uint8, np.int16, np.uint16])
+@pytest.mark.parametrize("Ns", [(3, 3), (5, 3), (1, 7), (4, 2)])
+def test_2d_median_equals_cast_then_scipy(src_t, dst_t, Ns):
+    """Integer 2D medians = scipy's median filter of the C-cast image
+    (brute force by an independent library)."""
+    a = random_volume((11, 17), src_t, seed=Ns[0] * 10 + Ns[1])
+    Nx, Ny = Ns
+    ex, ey = (Nx - 1) // 2, (Ny - 1) // 2
+    want = ndimage.median_filter(a.astype(dst_t), size=(2 * ey + 1, 2 * ex + 1), mode="nearest")
+    assert np.array_equal(O.median_box_2d(a, dst_t, Nx, Ny), want)
+
+
+def test_2d_median_float_source_saturates_like_3d():
+    a = np.array([[-5.7, 300.2, 7.9, np.nan]], dtype=np.float32)
+    assert np.array_equal(O.median_box_2d(a, np.uint8, 1, 1), O.median_box_3d(a[None], np.uint8, 1, 1, 1)[0])
