@@ -362,18 +362,83 @@ def bench_median(args, torch, clkmi):
     print(json.dumps(out))
 
 
+def bench_neighbourhood(args, torch, clkmi):
+    """The widened rows: ellipsoid filters, slice median and converting copy
+    on the uint8 512x512x256 volume of the median workload, and the 2D
+    siblings on a uint16 8192x8192 image.  Per launch: ms, Gvoxel/s and
+    algorithmic HBM bytes (src + dst once) against 8 TB/s."""
+    W, H, D = 512, 512, 256
+    V = W * H * D
+    torch.cuda.init()
+    src = make_c3_volume(torch, D, H, W)
+    dst = torch.empty_like(src)
+    dstf = torch.empty((D, H, W), dtype=torch.float32, device="cuda")
+    si = clkmi.Image(src.data_ptr(), CLK_UCHAR, W, H, D)
+    di = clkmi.Image(dst.data_ptr(), CLK_UCHAR, W, H, D)
+    dfi = clkmi.Image(dstf.data_ptr(), CLK_FLOAT, W, H, D)
+    W2 = H2 = 8192
+    img = make_c2_volume(torch, 1, H2, W2, seed=7, nblobs=50)
+    out2 = torch.empty((1, H2, W2), dtype=torch.float32, device="cuda")
+    outu = torch.empty_like(img)
+    i2 = clkmi.Image(img.data_ptr(), CLK_USHORT, W2, H2, 1)
+    o2 = clkmi.Image(out2.data_ptr(), CLK_FLOAT, W2, H2, 1)
+    ou = clkmi.Image(outu.data_ptr(), CLK_USHORT, W2, H2, 1)
+    stream = torch.cuda.current_stream()
+    V2 = W2 * H2
+    plan = []
+    for n in (3, 5):
+        for k in ("minimum_sphere_3d", "maximum_sphere_3d", "mean_sphere_3d", "median_sphere_3d"):
+            plan.append(("%s %dx%dx%d u8" % (k, n, n, n), lambda k=k, n=n: getattr(clkmi, k)(di, si, n, n, n, stream=stream), V, 2))
+    plan.append(("median_slice_by_slice_box_3d 3x3 u8",
+                 lambda: clkmi.median_slice_by_slice_box_3d(di, si, 3, 3, stream=stream), V, 2))
+    plan.append(("copy_3d u8->f32", lambda: clkmi.copy_3d(dfi, si, stream=stream), V, 5))
+    for dim in (0, 1):
+        plan.append(("gaussian_blur_separable_2d dim=%d u16->f32 N=25" % dim,
+                     lambda dim=dim: clkmi.gaussian_blur_separable_2d(o2, i2, dim, 25, 3.0, stream=stream), V2, 6))
+        plan.append(("maximum_separable_2d dim=%d u16 N=7" % dim,
+                     lambda dim=dim: clkmi.maximum_separable_2d(ou, i2, dim, 7, 0.0, stream=stream), V2, 4))
+    plan.append(("median_box_2d 3x3 u16", lambda: clkmi.median_box_2d(ou, i2, 3, 3, stream=stream), V2, 4))
+    res = []
+    for label, fn, vox, bpv in plan:
+        for _ in range(max(1, args.warmup)):
+            fn()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record(stream)
+        for _ in range(args.steps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / args.steps
+        gbs = vox * bpv / (ms * 1e-3) / 1e9
+        res.append({"launch": label, "ms": round(ms, 4), "Gvoxel_per_s": round(vox / (ms * 1e-3) / 1e9, 2),
+                    "GBps": round(gbs, 1), "pct_hbm": round(100 * gbs / HBM_PEAK_GBS, 1)})
+    tot = sum(r["ms"] for r in res)
+    out = {
+        "metric": "Gvoxel/s, neighbourhood filters beyond the separable/median path", "unit": "Gvoxel/s",
+        "value": round(sum(vox for _, _, vox, _ in plan) / (tot * 1e-3) / 1e9, 3),
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(tot, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8/u16/f32",
+        "data": "synthetic: uint8 512x512x256 as the median workload (seed 3); uint16 8192x8192 Poisson + blobs (seed 7)",
+        "config": {"workload": "sphere filters r=1,2, slice median, copy_3d, 2D siblings"},
+        "launches": res,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", default=None, help="WxHxD (default: the workload's own size)")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "c6"],
                     help="c2: separable suite on uint16 1024x1024x512 (the headline); "
                          "c3: median_box_3d r=1 and r=2 on uint8 512x512x256; "
                          "c4: 256 uneven uint16 volumes, one random 3D filter each, "
                          "batched vs one call per volume; c5: served chains "
-                         "(Gaussian -> max -> median) under Poisson arrivals")
+                         "(Gaussian -> max -> median) under Poisson arrivals; c6: sphere filters, slice "
+                         "median, copy_3d and the 2D siblings")
     ap.add_argument("--requests", type=int, default=48, help="c5: requests per arrival rate")
     ap.add_argument("--no-autotune", action="store_true", help="keep default launch shapes (no first-launch tuning)")
     ap.add_argument("--tune-cache", default=None, help="file of tuned launch shapes to load before and save after tuning")
@@ -395,6 +460,8 @@ def main():
     import torch
     import clkmi
 
+    if args.workload == "c6":
+        return bench_neighbourhood(args, torch, clkmi)
     if args.workload == "c3":
         return bench_median(args, torch, clkmi)
     if args.workload == "c4":

@@ -23,6 +23,8 @@ __all__ = [
     "FilterJob", "batch_filter_3d", "set_tuning", "get_tuning", "autotune_save", "autotune_load",
     "gaussian_blur_separable_2d", "mean_separable_2d", "minimum_separable_2d", "maximum_separable_2d",
     "median_box_2d", "gaussian_blur_2d", "mean_box_2d", "minimum_box_2d", "maximum_box_2d",
+    "minimum_sphere_3d", "maximum_sphere_3d", "mean_sphere_3d", "median_sphere_3d",
+    "median_slice_by_slice_box_3d", "copy_3d",
 ]
 
 # Kernel name -> declared parameters (reference .cl declarations).
@@ -37,6 +39,12 @@ KERNELS = {
     "minimum_separable_2d": ("dst", "src", "dim", "N", "s"),        # minimum_separable_2d_x.cl:5-9
     "maximum_separable_2d": ("dst", "src", "dim", "N", "s"),        # maximum_separable_2d_x.cl:5-9
     "median_box_2d": ("dst", "src", "Nx", "Ny"),                    # median_box_2d_x.cl:24-27
+    "minimum_sphere_3d": ("dst", "src", "Nx", "Ny", "Nz"),          # minimum_sphere_3d_x.cl:5-9
+    "maximum_sphere_3d": ("dst", "src", "Nx", "Ny", "Nz"),          # maximum_sphere_3d_x.cl:5-9
+    "mean_sphere_3d": ("dst", "src", "Nx", "Ny", "Nz"),             # mean_sphere_3d_x.cl:5-9
+    "median_sphere_3d": ("dst", "src", "Nx", "Ny", "Nz"),           # median_sphere_3d_x.cl:24-28
+    "median_slice_by_slice_box_3d": ("dst", "src", "Nx", "Ny"),     # median_slice_by_slice_box_3d_x.cl:24-27
+    "copy_3d": ("dst", "src"),                                      # copy_3d_x.cl:2-3
 }
 
 Image = namedtuple("Image", "ptr dtype width height depth")
@@ -140,6 +148,38 @@ def median_box_2d(dst, src, Nx, Ny, stream=None):
     d, sr = to_clk_image(dst), to_clk_image(src)
     check(lib().clk_median_box_2d(_stream_ptr(stream), ctypes.byref(d), ctypes.byref(sr), int(Nx), int(Ny)))
     return dst
+
+
+def _nbh(name, dst, src, sizes, stream):
+    d, sr = to_clk_image(dst), to_clk_image(src)
+    check(getattr(lib(), "clk_" + name)(_stream_ptr(stream), ctypes.byref(d), ctypes.byref(sr),
+                                        *[int(v) for v in sizes]))
+    return dst
+
+
+def minimum_sphere_3d(dst, src, Nx, Ny, Nz, stream=None):
+    return _nbh("minimum_sphere_3d", dst, src, (Nx, Ny, Nz), stream)
+
+
+def maximum_sphere_3d(dst, src, Nx, Ny, Nz, stream=None):
+    return _nbh("maximum_sphere_3d", dst, src, (Nx, Ny, Nz), stream)
+
+
+def mean_sphere_3d(dst, src, Nx, Ny, Nz, stream=None):
+    return _nbh("mean_sphere_3d", dst, src, (Nx, Ny, Nz), stream)
+
+
+def median_sphere_3d(dst, src, Nx, Ny, Nz, stream=None):
+    return _nbh("median_sphere_3d", dst, src, (Nx, Ny, Nz), stream)
+
+
+def median_slice_by_slice_box_3d(dst, src, Nx, Ny, stream=None):
+    return _nbh("median_slice_by_slice_box_3d", dst, src, (Nx, Ny), stream)
+
+
+def copy_3d(dst, src, stream=None):
+    """dst = CONVERT(src) (copy_3d_x.cl)."""
+    return _nbh("copy_3d", dst, src, (), stream)
 
 
 def execute(kernel_name, parameters, stream=None):

@@ -34,7 +34,8 @@ EXPORTED = (
     "clk_batch_filter_3d", "clk_set_tuning", "clk_get_tuning", "clk_autotune_save", "clk_autotune_load",
     "clk_gaussian_blur_separable_2d", "clk_mean_separable_2d", "clk_minimum_separable_2d",
     "clk_maximum_separable_2d", "clk_median_box_2d", "clk_gaussian_blur_2d", "clk_mean_box_2d",
-    "clk_minimum_box_2d", "clk_maximum_box_2d",
+    "clk_minimum_box_2d", "clk_maximum_box_2d", "clk_minimum_sphere_3d", "clk_maximum_sphere_3d",
+    "clk_mean_sphere_3d", "clk_median_sphere_3d", "clk_median_slice_by_slice_box_3d", "clk_copy_3d",
 )
 
 CLK_JOB_GAUSSIAN_BLUR_3D, CLK_JOB_MEAN_BOX_3D, CLK_JOB_MINIMUM_BOX_3D, CLK_JOB_MAXIMUM_BOX_3D, \
@@ -108,6 +109,15 @@ def lib():
             fn = getattr(L, name)
             fn.argtypes = [vp, img, img, i32, i32, f32]
             fn.restype = i32
+        for name in ("clk_minimum_sphere_3d", "clk_maximum_sphere_3d", "clk_mean_sphere_3d",
+                     "clk_median_sphere_3d"):
+            fn = getattr(L, name)
+            fn.argtypes = [vp, img, img, i32, i32, i32]
+            fn.restype = i32
+        L.clk_median_slice_by_slice_box_3d.argtypes = [vp, img, img, i32, i32]
+        L.clk_median_slice_by_slice_box_3d.restype = i32
+        L.clk_copy_3d.argtypes = [vp, img, img]
+        L.clk_copy_3d.restype = i32
         L.clk_median_box_2d.argtypes = [vp, img, img, i32, i32]
         L.clk_median_box_2d.restype = i32
         L.clk_gaussian_blur_2d.argtypes = [vp, img, img, f32, f32]

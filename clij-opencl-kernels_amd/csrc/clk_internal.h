@@ -67,6 +67,11 @@ int32_t prepare_separable(int op, const clk_image& dst, const clk_image& src, in
                           float s, SepParams& p, int* axis_is_x, bool* fill_nan, std::string* err);
 int32_t launch_fill_nan(hipStream_t st, const clk_image& dst, std::string* err);
 
+// minimum/maximum/mean/median_sphere_3d (op 0..3); arguments validated by
+// the caller.
+int32_t launch_sphere(int op, hipStream_t st, const clk_image& dst, const clk_image& src, int32_t Nx,
+                      int32_t Ny, int32_t Nz, std::string* err);
+
 // median_box_3d launch (median_box_2d with wrap = true and Nz = 1: the 2D
 // kernel's C cast of integer values); arguments validated by the caller.
 int32_t launch_median(hipStream_t st, const clk_image& dst, const clk_image& src,

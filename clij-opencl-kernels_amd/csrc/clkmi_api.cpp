@@ -260,6 +260,57 @@ int32_t clk_median_box_2d(void* stream, const clk_image* dst, const clk_image* s
   return ok();
 }
 
+static int32_t sphere(int op, const char* name, void* stream, const clk_image* dst, const clk_image* src,
+                      int32_t Nx, int32_t Ny, int32_t Nz) {
+  int32_t rc = check_pair(name, dst, src);
+  if (rc) return rc;
+  std::string err;
+  rc = clk::launch_sphere(op, (hipStream_t)stream, *dst, *src, Nx, Ny, Nz, &err);
+  if (rc) return fail(rc, std::string(name) + ": " + err);
+  return ok();
+}
+
+int32_t clk_minimum_sphere_3d(void* stream, const clk_image* dst, const clk_image* src, int32_t Nx, int32_t Ny,
+                              int32_t Nz) {
+  return sphere(0, "minimum_sphere_3d", stream, dst, src, Nx, Ny, Nz);
+}
+
+int32_t clk_maximum_sphere_3d(void* stream, const clk_image* dst, const clk_image* src, int32_t Nx, int32_t Ny,
+                              int32_t Nz) {
+  return sphere(1, "maximum_sphere_3d", stream, dst, src, Nx, Ny, Nz);
+}
+
+int32_t clk_mean_sphere_3d(void* stream, const clk_image* dst, const clk_image* src, int32_t Nx, int32_t Ny,
+                           int32_t Nz) {
+  return sphere(2, "mean_sphere_3d", stream, dst, src, Nx, Ny, Nz);
+}
+
+int32_t clk_median_sphere_3d(void* stream, const clk_image* dst, const clk_image* src, int32_t Nx, int32_t Ny,
+                             int32_t Nz) {
+  return sphere(3, "median_sphere_3d", stream, dst, src, Nx, Ny, Nz);
+}
+
+int32_t clk_median_slice_by_slice_box_3d(void* stream, const clk_image* dst, const clk_image* src, int32_t Nx,
+                                         int32_t Ny) {
+  int32_t rc = check_pair("median_slice_by_slice_box_3d", dst, src);
+  if (rc) return rc;
+  std::string err;
+  // the 2D box per z slice, values stored by the same C cast as median_box_2d
+  // (median_slice_by_slice_box_3d_x.cl:46)
+  rc = clk::launch_median((hipStream_t)stream, *dst, *src, Nx, Ny, 1, &err, true);
+  if (rc) return fail(rc, "median_slice_by_slice_box_3d: " + err);
+  return ok();
+}
+
+int32_t clk_copy_3d(void* stream, const clk_image* dst, const clk_image* src) {
+  int32_t rc = check_pair("copy_3d", dst, src);
+  if (rc) return rc;
+  std::string err;
+  rc = clk::launch_convert_copy((hipStream_t)stream, *dst, *src, &err);  // copy_3d_x.cl:13-14
+  if (rc) return fail(rc, "copy_3d: " + err);
+  return ok();
+}
+
 int32_t clk_execute(void* stream, const char* kernel_name, const clk_arg* args, int32_t nargs) {
   if (!kernel_name) return fail(CLK_INVALID_KERNEL_NAME, "kernel name is NULL");
   if (nargs < 0 || (nargs > 0 && !args)) return fail(CLK_INVALID_KERNEL_ARGS, "invalid argument list");
@@ -277,6 +328,47 @@ int32_t clk_execute(void* stream, const char* kernel_name, const clk_arg* args, 
   else if (base == "minimum_separable_3d") op = clk::OP_MIN;
   else if (base == "maximum_separable_3d") op = clk::OP_MAX;
   else if (base == "median_box_3d") op = 4;
+  else if (!two_d) {
+    // kernels with (dst, src[, Nx, Ny[, Nz]]): spheres, slice median, copy
+    // (minimum_sphere_3d_x.cl:5-9, median_slice_by_slice_box_3d_x.cl:24-28,
+    // copy_3d_x.cl:1-4)
+    int kind = -1, nsc = 3;
+    if (k == "minimum_sphere_3d") kind = 0;
+    else if (k == "maximum_sphere_3d") kind = 1;
+    else if (k == "mean_sphere_3d") kind = 2;
+    else if (k == "median_sphere_3d") kind = 3;
+    else if (k == "median_slice_by_slice_box_3d") { kind = 4; nsc = 2; }
+    else if (k == "copy_3d") { kind = 5; nsc = 0; }
+    if (kind >= 0) {
+      static const char* nn[3] = {"Nx", "Ny", "Nz"};
+      const clk_image *d = nullptr, *sr = nullptr;
+      int32_t v[3] = {0, 0, 0};
+      bool has[3] = {false, false, false};
+      for (int32_t i = 0; i < nargs; ++i) {
+        const clk_arg& a = args[i];
+        if (!a.name) return fail(CLK_INVALID_KERNEL_ARGS, k + ": argument without a name");
+        const std::string nm(a.name);
+        if (nm == "dst" || nm == "src") {
+          if (a.kind != CLK_ARG_IMAGE) return fail(CLK_INVALID_KERNEL_ARGS, k + ": " + nm + " must be an image");
+          (nm == "dst" ? d : sr) = a.image;
+          continue;
+        }
+        int slot = -1;
+        for (int j = 0; j < nsc; ++j)
+          if (nm == nn[j]) slot = j;
+        if (slot < 0) return fail(CLK_INVALID_KERNEL_ARGS, k + ": unknown parameter " + nm);
+        if (a.kind == CLK_ARG_IMAGE) return fail(CLK_INVALID_KERNEL_ARGS, k + ": " + nm + " must be a number");
+        v[slot] = (a.kind == CLK_ARG_INT) ? a.i : (int32_t)a.f;
+        has[slot] = true;
+      }
+      if (!d || !sr) return fail(CLK_INVALID_KERNEL_ARGS, k + ": dst and src are required");
+      for (int j = 0; j < nsc; ++j)
+        if (!has[j]) return fail(CLK_INVALID_KERNEL_ARGS, k + ": missing parameter " + nn[j]);
+      if (kind <= 3) return sphere(kind, k.c_str(), stream, d, sr, v[0], v[1], v[2]);
+      if (kind == 4) return clk_median_slice_by_slice_box_3d(stream, d, sr, v[0], v[1]);
+      return clk_copy_3d(stream, d, sr);
+    }
+  }
   if (op < 0) return fail(CLK_INVALID_KERNEL_NAME, "unknown kernel: " + k);
   // parameter names in declaration order (gaussian_blur_separable_3d_x.cl:8-11,
   // median_box_3d_x.cl:25-29; the 2D kernels: gaussian_blur_separable_2d_x.cl:8-12,

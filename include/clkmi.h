@@ -130,6 +130,37 @@ int32_t clk_maximum_separable_2d(void* stream, const clk_image* dst,
 int32_t clk_median_box_2d(void* stream, const clk_image* dst,
                           const clk_image* src, int32_t Nx, int32_t Ny);
 
+/* ---- ellipsoid neighbourhoods, slice medians, converting copy ---------- */
+
+/* minimum_sphere_3d / maximum_sphere_3d / mean_sphere_3d / median_sphere_3d
+ * (dst, src, Nx, Ny, Nz) replace kernels/minimum_sphere_3d_x.cl:3-9,
+ * maximum_sphere_3d_x.cl:3-9, mean_sphere_3d_x.cl:3-9 and
+ * median_sphere_3d_x.cl:22-28: the box of half widths (N-1)/2 restricted to
+ * x^2/a^2 + y^2/b^2 + z^2/c^2 <= 1 (FLT_MIN for a zero half width).  Half
+ * widths up to 31 per axis; median_sphere_3d rejects a negative N. */
+int32_t clk_minimum_sphere_3d(void* stream, const clk_image* dst,
+                              const clk_image* src, int32_t Nx, int32_t Ny,
+                              int32_t Nz);
+int32_t clk_maximum_sphere_3d(void* stream, const clk_image* dst,
+                              const clk_image* src, int32_t Nx, int32_t Ny,
+                              int32_t Nz);
+int32_t clk_mean_sphere_3d(void* stream, const clk_image* dst,
+                           const clk_image* src, int32_t Nx, int32_t Ny,
+                           int32_t Nz);
+int32_t clk_median_sphere_3d(void* stream, const clk_image* dst,
+                             const clk_image* src, int32_t Nx, int32_t Ny,
+                             int32_t Nz);
+
+/* median_slice_by_slice_box_3d(dst, src, Nx, Ny) replaces
+ * kernels/median_slice_by_slice_box_3d_x.cl:22-28: median_box_2d on every z
+ * slice (values stored by the same C cast). */
+int32_t clk_median_slice_by_slice_box_3d(void* stream, const clk_image* dst,
+                                         const clk_image* src, int32_t Nx,
+                                         int32_t Ny);
+
+/* copy_3d(dst, src) replaces kernels/copy_3d_x.cl:1-15: dst = CONVERT(src). */
+int32_t clk_copy_3d(void* stream, const clk_image* dst, const clk_image* src);
+
 /* ---- by-name dispatch (CLIJ execute(kernelName, parameterMap)) --------- */
 
 typedef enum clk_arg_kind {
@@ -148,8 +179,9 @@ typedef struct clk_arg {
 } clk_arg;
 
 /* Launch a kernel by its reference name ("gaussian_blur_separable_3d", or the
- * program file name "gaussian_blur_separable_3d_x.cl"; the ten names are the
- * five 3D kernels and their _2d siblings) with arguments bound
+ * program file name "gaussian_blur_separable_3d_x.cl"; the names are the five
+ * 3D kernels, their _2d siblings, the four _sphere_3d filters,
+ * median_slice_by_slice_box_3d and copy_3d) with arguments bound
  * by parameter name, as a CLIJ host binds them (reference README.md:26).
  * An int argument is accepted for a float parameter and vice versa, as the
  * reference hosts convert Java/Python numbers.  Unknown names or missing

@@ -338,6 +338,97 @@ int clko_median_box_2d(void* dst, int dst_t, const void* src, int src_t, int64_t
   return rc;
 }
 
+/* ---- ellipsoid neighbourhoods (…_sphere_3d_x.cl) ------------------------ */
+
+/* aSquared etc. of minimum_sphere_3d_x.cl:16-28: e*e as an int converted to
+ * float, FLT_MIN when zero. */
+static float sq_or_min(int e) {
+  float a = (float)(e * e);
+  if (a == 0) a = 1.17549435e-38f; /* FLT_MIN */
+  return a;
+}
+
+/* op: 0 minimum_sphere_3d_x.cl:11-53, 1 maximum_sphere_3d_x.cl (the same with
+ * >), 2 mean_sphere_3d_x.cl:11-53, 3 median_sphere_3d_x.cl:29-71.  Returns
+ * -30 for bad arguments, including an empty median neighbourhood. */
+int clko_sphere_3d(int op, void* dst, int dst_t, const void* src, int src_t, int64_t W, int64_t H, int64_t D,
+                   int Nx, int Ny, int Nz, int nthreads) {
+  if (op < 0 || op > 3 || src_t < 0 || src_t > 4 || dst_t < 0 || dst_t > 4) return -30;
+  const int ex = (Nx - 1) / 2, ey = (Ny - 1) / 2, ez = (Nz - 1) / 2; /* :14 */
+  if (op == 3 && (ex < 0 || ey < 0 || ez < 0)) return -30;
+  const float a2 = sq_or_min(ex), b2 = sq_or_min(ey), c2 = sq_or_min(ez);
+  const int64_t cap = (ex >= 0 && ey >= 0 && ez >= 0) ? (int64_t)(2 * ex + 1) * (2 * ey + 1) * (2 * ez + 1) : 1;
+  const img_t im = {src, src_t, W, H, D};
+  if (nthreads < 1) nthreads = 1;
+  int rc = 0;
+#pragma omp parallel num_threads(nthreads)
+  {
+    float* arr = (float*)malloc(sizeof(float) * (size_t)cap);
+    if (!arr) {
+#pragma omp atomic write
+      rc = -5;
+    }
+    int64_t k, j;
+#pragma omp for collapse(2) schedule(static)
+    for (k = 0; k < D; k++) {
+      for (j = 0; j < H; j++) {
+        if (!arr) continue;
+        for (int64_t i = 0; i < W; i++) {
+          float ext = read_px(&im, i, j, k); /* min/max: the centre first (:15) */
+          float sum = 0;                     /* mean (:17) */
+          int count = 0;
+          for (int x = -ex; x <= ex; x++) {
+            const float xs = (float)(x * x);
+            for (int y = -ey; y <= ey; y++) {
+              const float ys = (float)(y * y);
+              for (int z = -ez; z <= ez; z++) {
+                const float zs = (float)(z * z);
+                if (xs / a2 + ys / b2 + zs / c2 <= 1.0f) { /* :37 */
+                  const float v = read_px(&im, i + x, j + y, k + z);
+                  if (op == 0) { if (v < ext) ext = v; }      /* :44-46 */
+                  else if (op == 1) { if (v > ext) ext = v; }
+                  else if (op == 2) sum = sum + v;            /* mean :44 */
+                  else arr[count] = to_dst_type(v, dst_t);    /* median :65 */
+                  count++;
+                }
+              }
+            }
+          }
+          float r;
+          if (op <= 1) r = clko_convert(ext, dst_t);
+          else if (op == 2) r = clko_convert(sum / (float)count, dst_t); /* :50 */
+          else r = median_of(arr, count);                                /* :70 */
+          write_px(dst, dst_t, i + j * W + k * W * H, r);
+        }
+      }
+    }
+    free(arr);
+  }
+  return rc;
+}
+
+/* median_slice_by_slice_box_3d_x.cl:29-53: median_box_2d on every z slice,
+ * with the same C cast of the values (:46). */
+int clko_median_slice_by_slice_box_3d(void* dst, int dst_t, const void* src, int src_t, int64_t W, int64_t H,
+                                      int64_t D, int Nx, int Ny, int nthreads) {
+  const int64_t plane_in = W * H * (src_t == T_FLOAT ? 4 : (src_t >= T_SHORT ? 2 : 1));
+  const int64_t plane_out = W * H * (dst_t == T_FLOAT ? 4 : (dst_t >= T_SHORT ? 2 : 1));
+  for (int64_t k = 0; k < D; k++) {
+    const int rc = clko_median_box_2d((char*)dst + k * plane_out, dst_t, (const char*)src + k * plane_in, src_t, W,
+                                      H, Nx, Ny, nthreads);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+/* copy_3d_x.cl:11-14: WRITE(CONVERT((float)READ(src))). */
+int clko_copy_3d(void* dst, int dst_t, const void* src, int src_t, int64_t n) {
+  if (src_t < 0 || src_t > 4 || dst_t < 0 || dst_t > 4) return -30;
+  const img_t im = {src, src_t, n, 1, 1};
+  for (int64_t i = 0; i < n; i++) write_px(dst, dst_t, i, clko_convert(read_px(&im, i, 0, 0), dst_t));
+  return 0;
+}
+
 /* Correctly rounded a/b through RN(1/b) and one FMA residual step, the
  * formula the GPU kernels use for a launch-uniform divisor; exported so the
  * tests can check it against the IEEE divide on many inputs. */

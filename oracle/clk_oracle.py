@@ -53,6 +53,12 @@ def lib():
         L.clko_separable.restype = i32
         L.clko_median_box_3d.argtypes = [vp, i32, vp, i32, i64, i64, i64, i32, i32, i32, i32]
         L.clko_median_box_3d.restype = i32
+        L.clko_sphere_3d.argtypes = [i32, vp, i32, vp, i32, i64, i64, i64, i32, i32, i32, i32]
+        L.clko_sphere_3d.restype = i32
+        L.clko_median_slice_by_slice_box_3d.argtypes = [vp, i32, vp, i32, i64, i64, i64, i32, i32, i32]
+        L.clko_median_slice_by_slice_box_3d.restype = i32
+        L.clko_copy_3d.argtypes = [vp, i32, vp, i32, i64]
+        L.clko_copy_3d.restype = i32
         L.clko_separable_2d.argtypes = [i32, vp, i32, vp, i32, i64, i64, i32, i32, f32, i32]
         L.clko_separable_2d.restype = i32
         L.clko_median_box_2d.argtypes = [vp, i32, vp, i32, i64, i64, i32, i32, i32]
@@ -149,6 +155,43 @@ def median_box_2d(src, dst_dtype, Nx, Ny, nthreads=None):
     H, W = src.shape
     rc = lib().clko_median_box_2d(dst.ctypes.data, DTYPE_CODES[dst.dtype], src.ctypes.data,
                                   DTYPE_CODES[src.dtype], W, H, int(Nx), int(Ny), int(nthreads or default_threads()))
+    if rc != 0:
+        raise ValueError("oracle rejected arguments (%d)" % rc)
+    return dst
+
+
+SPHERE_OPS = {"minimum_sphere_3d": 0, "maximum_sphere_3d": 1, "mean_sphere_3d": 2, "median_sphere_3d": 3}
+
+
+def sphere_3d(op, src, dst_dtype, Nx, Ny, Nz, nthreads=None):
+    """minimum/maximum/mean/median_sphere_3d on a (D, H, W) volume."""
+    op = SPHERE_OPS[op] if isinstance(op, str) else int(op)
+    src = _vol(src)
+    dst = np.empty(src.shape, dtype=np.dtype(dst_dtype))
+    D, H, W = src.shape
+    rc = lib().clko_sphere_3d(op, dst.ctypes.data, DTYPE_CODES[dst.dtype], src.ctypes.data, DTYPE_CODES[src.dtype],
+                              W, H, D, int(Nx), int(Ny), int(Nz), int(nthreads or default_threads()))
+    if rc != 0:
+        raise ValueError("oracle rejected arguments (%d)" % rc)
+    return dst
+
+
+def median_slice_by_slice_box_3d(src, dst_dtype, Nx, Ny, nthreads=None):
+    src = _vol(src)
+    dst = np.empty(src.shape, dtype=np.dtype(dst_dtype))
+    D, H, W = src.shape
+    rc = lib().clko_median_slice_by_slice_box_3d(dst.ctypes.data, DTYPE_CODES[dst.dtype], src.ctypes.data,
+                                                 DTYPE_CODES[src.dtype], W, H, D, int(Nx), int(Ny),
+                                                 int(nthreads or default_threads()))
+    if rc != 0:
+        raise ValueError("oracle rejected arguments (%d)" % rc)
+    return dst
+
+
+def copy_3d(src, dst_dtype):
+    src = np.ascontiguousarray(src)
+    dst = np.empty(src.shape, dtype=np.dtype(dst_dtype))
+    rc = lib().clko_copy_3d(dst.ctypes.data, DTYPE_CODES[dst.dtype], src.ctypes.data, DTYPE_CODES[src.dtype], src.size)
     if rc != 0:
         raise ValueError("oracle rejected arguments (%d)" % rc)
     return dst

@@ -427,3 +427,63 @@ def test_2d_by_name_chains_and_errors(gpu):
     with pytest.raises(clkmi.ClkError) as e:  # median_box_2d has no Nz
         clkmi.execute("median_box_2d", {"dst": d1, "src": si, "Nx": 3, "Ny": 3, "Nz": 3})
     assert e.value.code == -52
+
+
+# ---- ellipsoid neighbourhoods, slice medians, copy -----------------------------
+
+SPHERE_OPS = ["minimum_sphere_3d", "maximum_sphere_3d", "mean_sphere_3d", "median_sphere_3d"]
+
+
+@pytest.mark.parametrize("name", SPHERE_OPS)
+@pytest.mark.parametrize("src_t", DTYPES, ids=lambda t: DTYPE_NAMES[t])
+@pytest.mark.parametrize("dst_t", DTYPES, ids=lambda t: DTYPE_NAMES[t])
+def test_sphere_all_type_pairs(gpu, name, src_t, dst_t):
+    import clkmi
+    a = random_volume((6, 9, 70), src_t, seed=zlib.crc32((name + DTYPE_NAMES[src_t]).encode()))
+    ts, si = to_device(a)
+    for Ns in ((3, 3, 3), (5, 5, 5), (5, 3, 1), (1, 1, 1), (7, 5, 3), (4, 2, 6)):
+        td, di = empty_device(a.shape, dst_t)
+        getattr(clkmi, name)(di, si, *Ns)
+        got = from_device(td, a.shape, dst_t)
+        assert_parity(got, O.sphere_3d(name, a, dst_t, *Ns), "%s %s" % (name, Ns))
+
+
+@pytest.mark.parametrize("src_t", DTYPES, ids=lambda t: DTYPE_NAMES[t])
+@pytest.mark.parametrize("dst_t", DTYPES, ids=lambda t: DTYPE_NAMES[t])
+def test_slice_median_and_copy_all_type_pairs(gpu, src_t, dst_t):
+    import clkmi
+    a = random_volume((5, 11, 37), src_t, seed=77)
+    ts, si = to_device(a)
+    for Nx, Ny in ((3, 3), (5, 1), (1, 7)):
+        td, di = empty_device(a.shape, dst_t)
+        clkmi.median_slice_by_slice_box_3d(di, si, Nx, Ny)
+        assert_parity(from_device(td, a.shape, dst_t), O.median_slice_by_slice_box_3d(a, dst_t, Nx, Ny),
+                      "slice median %dx%d" % (Nx, Ny))
+    td, di = empty_device(a.shape, dst_t)
+    clkmi.copy_3d(di, si)
+    assert_parity(from_device(td, a.shape, dst_t), O.copy_3d(a, dst_t), "copy_3d")
+
+
+def test_sphere_by_name_and_errors(gpu):
+    import clkmi
+    a = random_volume((8, 16, 40), np.uint16, seed=5)
+    ts, si = to_device(a)
+    t1, d1 = empty_device(a.shape, np.uint16)
+    t2, d2 = empty_device(a.shape, np.uint16)
+    clkmi.median_sphere_3d(d1, si, 5, 5, 3)
+    clkmi.execute("median_sphere_3d_x.cl", {"dst": d2, "src": si, "Nx": 5, "Ny": 5, "Nz": 3})
+    assert np.array_equal(from_device(t1, a.shape, np.uint16), from_device(t2, a.shape, np.uint16))
+    clkmi.execute("copy_3d", {"dst": d2, "src": si})
+    assert np.array_equal(from_device(t2, a.shape, np.uint16), a)
+    clkmi.execute("median_slice_by_slice_box_3d", {"dst": d2, "src": si, "Nx": 3, "Ny": 3})
+    assert np.array_equal(from_device(t2, a.shape, np.uint16), O.median_slice_by_slice_box_3d(a, np.uint16, 3, 3))
+    # degenerate sizes as the reference: empty neighbourhood -> centre / 0/0
+    clkmi.minimum_sphere_3d(d1, si, -1, -1, -1)
+    assert np.array_equal(from_device(t1, a.shape, np.uint16), a)
+    for bad in ((-1, 3, 3), (65, 3, 3)):
+        with pytest.raises(clkmi.ClkError) as e:
+            clkmi.median_sphere_3d(d1, si, *bad)
+        assert e.value.code == -30
+    with pytest.raises(clkmi.ClkError) as e:
+        clkmi.execute("mean_sphere_3d", {"dst": d1, "src": si, "Nx": 3, "Ny": 3})
+    assert e.value.code == -52

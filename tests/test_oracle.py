@@ -337,3 +337,106 @@ def test_2d_median_equals_cast_then_scipy(src_t, dst_t, Ns):
 def test_2d_median_float_source_saturates_like_3d():
     a = np.array([[-5.7, 300.2, 7.9, np.nan]], dtype=np.float32)
     assert np.array_equal(O.median_box_2d(a, np.uint8, 1, 1), O.median_box_3d(a[None], np.uint8, 1, 1, 1)[0])
+
+
+# ---- ellipsoid neighbourhoods, slice medians, copy -------------------------------
+
+def _sphere_footprint(Nx, Ny, Nz):
+    """The reference's mask (minimum_sphere_3d_x.cl:16-37) in float32: a
+    (z, y, x) footprint for scipy."""
+    ex, ey, ez = (Nx - 1) // 2, (Ny - 1) // 2, (Nz - 1) // 2
+    f32 = np.float32
+
+    def sq(e):
+        a = f32(e * e)
+        return a if a != 0 else f32(np.finfo(np.float32).tiny)
+    a2, b2, c2 = sq(ex), sq(ey), sq(ez)
+    fp = np.zeros((2 * ez + 1, 2 * ey + 1, 2 * ex + 1), dtype=bool)
+    for x in range(-ex, ex + 1):
+        for y in range(-ey, ey + 1):
+            for z in range(-ez, ez + 1):
+                t = f32(x * x) / a2 + f32(y * y) / b2
+                t = t + f32(z * z) / c2
+                fp[z + ez, y + ey, x + ex] = t <= f32(1.0)
+    return fp
+
+
+@pytest.mark.parametrize("Ns,count", [((3, 3, 3), 7), ((5, 5, 5), 33), ((3, 1, 1), 3), ((1, 1, 1), 1),
+                                      ((5, 3, 1), 7)])
+def test_sphere_neighbourhood_sizes(Ns, count):
+    """Closed forms: the lattice points of the ball of radius 1 (7) and 2
+    (33), a segment, a point, an ellipse with semi-axes 2 and 1 (7) -- via
+    the maximum of an impulse, which marks exactly the neighbourhood."""
+    a = np.zeros((9, 9, 9), dtype=np.float32)
+    a[4, 4, 4] = 1
+    assert int(O.sphere_3d("maximum_sphere_3d", a, np.float32, *Ns).sum()) == count
+    assert int(_sphere_footprint(*Ns).sum()) == count
+
+
+@pytest.mark.parametrize("dt", DTYPES)
+@pytest.mark.parametrize("Ns", [(3, 3, 3), (5, 5, 5), (7, 5, 3), (1, 3, 5)])
+def test_sphere_minmax_equals_scipy(dt, Ns):
+    a = random_volume((9, 11, 13), dt, seed=sum(Ns))
+    fp = _sphere_footprint(*Ns)
+    for op, ref in (("minimum_sphere_3d", ndimage.minimum_filter), ("maximum_sphere_3d", ndimage.maximum_filter)):
+        want = ref(a, footprint=fp, mode="nearest")
+        assert np.array_equal(O.sphere_3d(op, a, dt, *Ns), want), op
+
+
+@pytest.mark.parametrize("src_t", [np.int8, np.uint8, np.int16, np.uint16])
+@pytest.mark.parametrize("dst_t", DTYPES)
+def test_sphere_mean_integer_source_exact(src_t, dst_t):
+    """Integer sums are exact in float32 here, so the mean is the float32
+    quotient of scipy's sum by the neighbourhood size, then CONVERT."""
+    a = random_volume((7, 9, 11), src_t, seed=5)
+    Ns = (5, 3, 5)
+    fp = _sphere_footprint(*Ns)
+    s = ndimage.correlate(a.astype(np.float64), fp.astype(np.float64), mode="nearest")
+    q = (s.astype(np.float32) / np.float32(fp.sum())).astype(np.float32)
+    want = O.convert(q, dst_t)
+    assert same_bits_or_value(O.sphere_3d("mean_sphere_3d", a, dst_t, *Ns), want)
+
+
+def test_sphere_mean_float_source_close_to_fp64():
+    a = random_volume((7, 9, 11), np.float32, seed=6)
+    Ns = (5, 5, 3)
+    fp = _sphere_footprint(*Ns)
+    want = ndimage.correlate(a.astype(np.float64), fp.astype(np.float64), mode="nearest") / fp.sum()
+    got = O.sphere_3d("mean_sphere_3d", a, np.float32, *Ns)
+    assert np.max(np.abs(got - want)) <= 1e-5 * np.max(np.abs(a))
+
+
+@pytest.mark.parametrize("src_t", DTYPES)
+@pytest.mark.parametrize("dst_t", [np.uint8, np.int16, np.float32])
+@pytest.mark.parametrize("Ns", [(3, 3, 3), (5, 5, 5), (5, 3, 1)])
+def test_sphere_median_equals_scipy(src_t, dst_t, Ns):
+    """Rank count/2 of the dst-converted values in the neighbourhood."""
+    a = random_volume((7, 9, 11), src_t, seed=11)
+    fp = _sphere_footprint(*Ns)
+    want = ndimage.median_filter(O.convert(a.astype(np.float32), dst_t), footprint=fp, mode="nearest")
+    assert np.array_equal(O.sphere_3d("median_sphere_3d", a, dst_t, *Ns), want)
+
+
+def test_sphere_degenerate_sizes():
+    a = random_volume((4, 5, 6), np.int16, seed=2)
+    # negative N: an empty neighbourhood; min/max keep the centre, the mean is 0/0
+    assert np.array_equal(O.sphere_3d("minimum_sphere_3d", a, np.int16, -3, -3, -3), a)
+    assert np.array_equal(O.sphere_3d("mean_sphere_3d", a, np.int16, -3, -3, -3), np.zeros_like(a))
+    with pytest.raises(ValueError):
+        O.sphere_3d("median_sphere_3d", a, np.int16, -1, 3, 3)
+
+
+@pytest.mark.parametrize("src_t", [np.int16, np.uint16, np.float32])
+@pytest.mark.parametrize("dst_t", [np.uint8, np.int8])
+def test_slice_median_equals_cast_then_scipy(src_t, dst_t):
+    a = random_volume((4, 9, 13), src_t, seed=9)
+    cast = a.astype(dst_t) if np.dtype(src_t).kind != "f" else O.convert(a, dst_t)
+    want = ndimage.median_filter(cast, size=(1, 3, 5), mode="nearest")
+    assert np.array_equal(O.median_slice_by_slice_box_3d(a, dst_t, 5, 3), want)
+
+
+@pytest.mark.parametrize("src_t", DTYPES)
+@pytest.mark.parametrize("dst_t", DTYPES)
+def test_copy_is_convert(src_t, dst_t):
+    a = random_volume((3, 4, 5), src_t, seed=1)
+    assert same_bits_or_value(O.copy_3d(a, dst_t), O.convert(a.astype(np.float32), dst_t))
