@@ -58,6 +58,7 @@ __global__ __launch_bounds__(256) void batch_median_kernel(const MedParams* __re
   const uint32_t r = t.vblk / p.gx;
   const uint32_t by = r % p.gy, bz = r / p.gy;
   if constexpr (KIND == MED_PK8) median_27_pk8_body<TS, TD>(p, bx, by, bz);
+  else if constexpr (KIND == MED_PK9) median_27_pk8_body<TS, TD, 1>(p, bx, by, bz);
   else if constexpr (KIND == MED_27) median_27_body<TS, TD>(p, bx, by, bz);
   else median_generic_body<TS, TD>(p, bx, by, bz);
 }
@@ -134,6 +135,7 @@ void launch_median_class(hipStream_t st, const ClassKey& k, const ClassWork& w, 
   const MedParams* ms = reinterpret_cast<const MedParams*>(dev + w.params_off);
   switch (k.op) {
     case MED_PK8: hipLaunchKernelGGL((batch_median_kernel<TS, TD, MED_PK8>), dim3(nt), dim3(256), 0, st, ms, tiles); break;
+    case MED_PK9: hipLaunchKernelGGL((batch_median_kernel<TS, TD, MED_PK9>), dim3(nt), dim3(256), 0, st, ms, tiles); break;
     case MED_HIST8: hipLaunchKernelGGL((batch_hist8_kernel<TS, TD>), dim3(nt), dim3(64), 0, st, ms, tiles); break;
     case MED_27: hipLaunchKernelGGL((batch_median_kernel<TS, TD, MED_27>), dim3(nt), dim3(256), 0, st, ms, tiles); break;
     default: hipLaunchKernelGGL((batch_median_kernel<TS, TD, MED_GENERIC>), dim3(nt), dim3(256), 0, st, ms, tiles); break;

@@ -150,18 +150,21 @@ __device__ __forceinline__ T forgetful_median(T (&a)[N]) {
   return vmax(vmin(x0, x1), vmin(vmax(x0, x1), x2));
 }
 
-// 3x3x3 median, 8- or 16-bit integer dst: a block filters 512 consecutive x
-// of one (y, z).  The 9 source rows (x-1 .. x+512, clamped), converted to
+// 3x3x3 (ZN = 3 planes) or 3x3x1 (ZN = 1: median_box_2d, slice medians)
+// median, 8- or 16-bit integer dst: a block filters 512 consecutive x of one
+// (y, z).  The 3*ZN source rows (x-1 .. x+512, clamped), converted to
 // the dst type and biased to unsigned 16-bit keys (signed types shift by
 // -lo, which keeps their order), are staged in LDS as 16-bit value pairs
 // twice, pairs starting at even and at odd positions, so the three pairs a
 // thread needs per row (its two outputs shifted by -1, 0, +1) are three
 // aligned 32-bit reads; the selection network then runs on two voxels per
 // instruction (v_pk_min_u16 / v_pk_max_u16).
-template <class TS, class TD>
+template <class TS, class TD, int ZN = 3>
 __device__ __forceinline__ void median_27_pk8_body(const MedParams& p, uint32_t bx, uint32_t by, uint32_t bz) {
-  __shared__ uint32_t pe[9][257];
-  __shared__ uint32_t po[9][256];
+  static_assert(ZN == 1 || ZN == 3, "one or three planes");
+  constexpr int R = 3 * ZN;  // staged rows
+  __shared__ uint32_t pe[R][257];
+  __shared__ uint32_t po[R][256];
   const int t = threadIdx.x;
   const int64_t xt = (int64_t)bx * 512;
   const int64_t y = by, z = bz;
@@ -173,8 +176,8 @@ __device__ __forceinline__ void median_27_pk8_body(const MedParams& p, uint32_t 
     return (uint32_t)((int)gather_value<TS, TD>(row[x], p.wrap) + kBias);
   };
 #pragma unroll
-  for (int r = 0; r < 9; ++r) {
-    const int64_t zz = clamp64(z + r / 3 - 1, 0, p.D - 1);
+  for (int r = 0; r < R; ++r) {
+    const int64_t zz = clamp64(z + r / 3 - (ZN == 3 ? 1 : 0), 0, p.D - 1);
     const int64_t yy = clamp64(y + r % 3 - 1, 0, p.H - 1);
     const TS* row = src + (zz * p.H + yy) * p.W;
     // v[i] = value at x = xt - 1 + i
@@ -185,15 +188,15 @@ __device__ __forceinline__ void median_27_pk8_body(const MedParams& p, uint32_t 
     if (t == 255) pe[r][256] = v2 | (val(row, xb + 3) << 16);
   }
   __syncthreads();
-  us2 a[27];
+  us2 a[3 * R];
 #pragma unroll
-  for (int r = 0; r < 9; ++r) {
+  for (int r = 0; r < R; ++r) {
     const uint32_t w0 = pe[r][t], w1 = po[r][t], w2 = pe[r][t + 1];
     __builtin_memcpy(&a[3 * r + 0], &w0, 4);
     __builtin_memcpy(&a[3 * r + 1], &w1, 4);
     __builtin_memcpy(&a[3 * r + 2], &w2, 4);
   }
-  const us2 m = forgetful_median<27>(a);
+  const us2 m = forgetful_median<3 * R>(a);
   const int64_t x = xt + 2 * t;
   TD* __restrict__ dst = static_cast<TD*>(p.dst) + (z * p.H + y) * p.W;
   if (x < W) dst[x] = static_cast<TD>((int)m.x - kBias);
@@ -327,13 +330,19 @@ CLK_MEDIAN_KERNEL(median_27, 256)
 CLK_MEDIAN_KERNEL(median_hist8, 64)
 #undef CLK_MEDIAN_KERNEL
 
-enum MedKind : int { MED_PK8 = 0, MED_HIST8 = 1, MED_27 = 2, MED_GENERIC = 3 };
+template <class TS, class TD>
+__global__ __launch_bounds__(256) void median_9_pk8_kernel(const MedParams p) {
+  median_27_pk8_body<TS, TD, 1>(p, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+enum MedKind : int { MED_PK8 = 0, MED_HIST8 = 1, MED_27 = 2, MED_GENERIC = 3, MED_PK9 = 4 };
 
 // Kernel and grid for one median launch; sets p.gx, p.gy.
 inline MedKind plan_median(MedParams& p, int td_size, dim3* grid, int* threads) {
   MedKind k;
-  if ((td_size == 1 || td_size == 2) && p.ex == 1 && p.ey == 1 && p.ez == 1) {
-    k = MED_PK8;  // integer dst of 8 or 16 bits (float dst is 4 bytes)
+  if ((td_size == 1 || td_size == 2) && p.ex == 1 && p.ey == 1 && (p.ez == 1 || p.ez == 0)) {
+    // integer dst of 8 or 16 bits (float dst is 4 bytes); 3x3x3 or 3x3x1
+    k = p.ez == 1 ? MED_PK8 : MED_PK9;
     *grid = dim3((uint32_t)((p.W + 511) / 512), (uint32_t)p.H, (uint32_t)p.D);
     *threads = 256;
   } else if (td_size == 1 && p.n <= 255 && hist8_enabled()) {

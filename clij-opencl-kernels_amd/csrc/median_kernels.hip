@@ -16,6 +16,7 @@ static int32_t median_typed(hipStream_t st, MedParams& p, std::string* err) {
   int threads;
   switch (plan_median(p, (int)sizeof(TD), &grid, &threads)) {
     case MED_PK8: hipLaunchKernelGGL((median_27_pk8_kernel<TS, TD>), grid, dim3(threads), 0, st, p); break;
+    case MED_PK9: hipLaunchKernelGGL((median_9_pk8_kernel<TS, TD>), grid, dim3(threads), 0, st, p); break;
     case MED_HIST8: hipLaunchKernelGGL((median_hist8_kernel<TS, TD>), grid, dim3(threads), 0, st, p); break;
     case MED_27: hipLaunchKernelGGL((median_27_kernel<TS, TD>), grid, dim3(threads), 0, st, p); break;
     default: hipLaunchKernelGGL((median_generic_kernel<TS, TD>), grid, dim3(threads), 0, st, p); break;

@@ -487,3 +487,20 @@ def test_sphere_by_name_and_errors(gpu):
     with pytest.raises(clkmi.ClkError) as e:
         clkmi.execute("mean_sphere_3d", {"dst": d1, "src": si, "Nx": 3, "Ny": 3})
     assert e.value.code == -52
+
+
+def test_copy_3d_unaligned_and_ragged(gpu):
+    """The vector copy needs 8-pixel alignment; offset pointers take the
+    element path, and a ragged tail is finished element by element."""
+    import clkmi
+    import torch
+    a = random_volume((3, 7, 13), np.int16, seed=4)
+    n = a.size
+    src = torch.from_numpy(np.concatenate([np.zeros(1, np.int16), a.reshape(-1)])).cuda()
+    dst = torch.zeros(n + 1, dtype=torch.float32, device="cuda")
+    si = clkmi.Image(src.data_ptr() + 2, 2, 13, 7, 3)        # one pixel in: unaligned
+    di = clkmi.Image(dst.data_ptr() + 4, 4, 13, 7, 3)
+    clkmi.copy_3d(di, si)
+    got = dst.cpu().numpy()[1:].reshape(a.shape)
+    assert_parity(got, O.copy_3d(a, np.float32), "unaligned copy")
+    assert dst.cpu().numpy()[0] == 0
