@@ -42,6 +42,14 @@ __global__ __launch_bounds__(256) void batch_x_kernel(const SepParams* __restric
   sep_x_body<OP, TS, TD, 1>(ps[t.job], t.vblk, lds);
 }
 
+template <int OP, class TS, class TD>
+__global__ __launch_bounds__(256) void batch_xpack_kernel(const SepParams* __restrict__ ps,
+                                                          const BatchTile* __restrict__ tiles) {
+  extern __shared__ uint32_t lds[];
+  const BatchTile t = tiles[blockIdx.x];
+  sep_xpack_body<OP, TS, TD>(ps[t.job], t.vblk, lds);
+}
+
 template <int OP, class TS, class TD, int COLS>
 __global__ __launch_bounds__(256) void batch_strided_kernel(const SepParams* __restrict__ ps,
                                                             const BatchTile* __restrict__ tiles) {
@@ -75,7 +83,7 @@ __global__ __launch_bounds__(64) void batch_hist8_kernel(const MedParams* __rest
 
 namespace {
 
-enum ClassKind : int { K_X = 0, K_STRIDED4 = 1, K_STRIDED1 = 2, K_MEDIAN = 3 };
+enum ClassKind : int { K_X = 0, K_STRIDED4 = 1, K_STRIDED1 = 2, K_MEDIAN = 3, K_XPACK = 4 };
 
 // Pixel-type pairs with batched kernels: same type, to float, from float.
 bool batch_pair(int ts, int td) { return ts == td || td == CLK_FLOAT || ts == CLK_FLOAT; }
@@ -122,6 +130,8 @@ void launch_class_typed(hipStream_t st, const ClassKey& k, const ClassWork& w, c
   const SepParams* ps = reinterpret_cast<const SepParams*>(dev + w.params_off);
   if (k.kind == K_X)
     hipLaunchKernelGGL((batch_x_kernel<OP, TS, TD>), dim3(nt), dim3(256), w.lds, st, ps, tiles);
+  else if (k.kind == K_XPACK)
+    hipLaunchKernelGGL((batch_xpack_kernel<OP, TS, TD>), dim3(nt), dim3(256), w.lds, st, ps, tiles);
   else if (k.kind == K_STRIDED4)
     hipLaunchKernelGGL((batch_strided_kernel<OP, TS, TD, 4>), dim3(nt), dim3(256), 0, st, ps, tiles);
   else
@@ -310,7 +320,12 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
       ClassKey key{0, ps.op, ps.src.dtype, ps.dst.dtype};
       uint32_t blocks = 0;
       size_t lds = 0;
-      if (axis_is_x && p.c <= kMaxCX) {
+      if (axis_is_x && p.L <= 256 && p.c <= 256 && tuning(TUNE_NO_XPACK) == 0) {
+        key.kind = K_XPACK;  // short rows: several per wave
+        plan_xpack(p);
+        blocks = (uint32_t)((p.ntasks + 3) / 4);
+        lds = (size_t)4 * p.lds_stride * 4;
+      } else if (axis_is_x && p.c <= kMaxCX) {
         key.kind = K_X;
         const int64_t W = p.L;
         p.nseg = (int32_t)((W + kXSeg - 1) / kXSeg);

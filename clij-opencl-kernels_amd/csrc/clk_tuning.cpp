@@ -15,8 +15,9 @@ namespace {
 const char* kNames[TUNE_COUNT] = {"strided_rounds", "x_sub", "batch_seg",
                                   "no_xreg", "strided_variant", "no_hist8",
                                   "xreg_maxc", "ystatic_cols", "strided_order",
-                                  "autotune", "xreg_units", "x_dynamic", "x_tasks"};
-const int32_t kDefaults[TUNE_COUNT] = {16, 0, 64, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+                                  "autotune", "xreg_units", "x_dynamic", "x_tasks", "sphere_direct",
+                                  "no_xpack"};
+const int32_t kDefaults[TUNE_COUNT] = {16, 0, 64, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 std::atomic<int32_t> g_val[TUNE_COUNT];
 std::once_flag g_once;
 

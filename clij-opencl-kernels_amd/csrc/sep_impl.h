@@ -235,6 +235,45 @@ __host__ __device__ __forceinline__ int lds_pad(int e) { return e + e / kXK; }
 #include "sep_ystatic.h"
 namespace clk {
 
+// The taps of 8 consecutive outputs whose windows start at lb (LDS, one
+// pad word per 8): a ring of 8 window registers, refilled as the taps
+// advance, so each LDS value is read once per lane.
+template <int OP, class TS, class VT>
+__device__ __forceinline__ void x_ring(const VT* lb, const int c, const int ntaps, const SepParams& p,
+                                       VT (&acc)[kXK]) {
+  constexpr bool kFloatSrc = !Pix<TS>::kInt;
+  constexpr int K = kXK;
+  VT win[K];
+#pragma unroll
+  for (int t = 0; t < K; ++t) win[t] = lb[lds_pad(t)];
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    if constexpr (OP == OP_GAUSS || OP == OP_MEAN) acc[q] = 0.f;
+    else if constexpr (kFloatSrc) acc[q] = lb[lds_pad(q + c)];
+    else acc[q] = win[q];
+  }
+  int vv = 0;
+  for (; vv + K <= ntaps; vv += K) {
+    const VT* lv = lb + (K + 1) * (vv / K);  // lds_pad(vv + r + K) = (K+1)*(vv/K) + r + K + 1
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+      const float h = (OP == OP_GAUSS) ? p.w[vv + r] : 0.f;
+#pragma unroll
+      for (int q = 0; q < K; ++q) acc[q] = tap_apply<OP, kFloatSrc, VT>(acc[q], win[(q + r) % K], h);
+      win[r] = lv[r + K + 1];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < K - 1; ++r) {
+    if (vv + r < ntaps) {
+      const float h = (OP == OP_GAUSS) ? p.w[vv + r] : 0.f;
+#pragma unroll
+      for (int q = 0; q < K; ++q) acc[q] = tap_apply<OP, kFloatSrc, VT>(acc[q], win[(q + r) % K], h);
+      if (vv + r + 1 < ntaps) win[r] = lb[lds_pad(vv + r + K)];
+    }
+  }
+}
+
 // One wave filters SUB*512 consecutive outputs of one row: SUB vector loads
 // per lane (all in flight together) plus c halo pixels per side, staged in
 // LDS, then SUB blocks of 8 outputs per lane through the register ring.  The
@@ -242,7 +281,6 @@ namespace clk {
 template <int OP, class TS, class TD, int SUB>
 __device__ __forceinline__ void sep_x_body(const SepParams& p, const uint32_t vblk, uint32_t* lds_raw) {
   using VT = vt_t<OP, TS>;
-  constexpr bool kFloatSrc = !Pix<TS>::kInt;
   constexpr int K = kXK;
   constexpr int TSEG = kXSeg * SUB;
   const int wv = threadIdx.x >> 6;
@@ -286,36 +324,8 @@ __device__ __forceinline__ void sep_x_body(const SepParams& p, const uint32_t vb
   for (int m = 0; m < SUB; ++m) {
     // lds_pad(512*m + K*lane + t) = 576*m + (K+1)*lane + lds_pad(t)
     const VT* lb = buf + (kXSeg + kXSeg / K) * m + (K + 1) * lane;
-    VT win[K];
     VT acc[K];
-#pragma unroll
-    for (int t = 0; t < K; ++t) win[t] = lb[lds_pad(t)];
-#pragma unroll
-    for (int q = 0; q < K; ++q) {
-      if constexpr (OP == OP_GAUSS || OP == OP_MEAN) acc[q] = 0.f;
-      else if constexpr (kFloatSrc) acc[q] = lb[lds_pad(q + c)];
-      else acc[q] = win[q];
-    }
-    int vv = 0;
-    for (; vv + K <= ntaps; vv += K) {
-      const VT* lv = lb + (K + 1) * (vv / K);  // lds_pad(vv + r + K) = (K+1)*(vv/K) + r + K + 1
-#pragma unroll
-      for (int r = 0; r < K; ++r) {
-        const float h = (OP == OP_GAUSS) ? p.w[vv + r] : 0.f;
-#pragma unroll
-        for (int q = 0; q < K; ++q) acc[q] = tap_apply<OP, kFloatSrc, VT>(acc[q], win[(q + r) % K], h);
-        win[r] = lv[r + K + 1];
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < K - 1; ++r) {
-      if (vv + r < ntaps) {
-        const float h = (OP == OP_GAUSS) ? p.w[vv + r] : 0.f;
-#pragma unroll
-        for (int q = 0; q < K; ++q) acc[q] = tap_apply<OP, kFloatSrc, VT>(acc[q], win[(q + r) % K], h);
-        if (vv + r + 1 < ntaps) win[r] = lb[lds_pad(vv + r + K)];
-      }
-    }
+    x_ring<OP, TS, VT>(lb, c, ntaps, p, acc);
     const int64_t xs = x0 + kXSeg * m + (int64_t)K * lane;
     if (xs < W) {
       TD* __restrict__ dp = static_cast<TD*>(p.dst) + (int64_t)row * W + xs;
@@ -337,6 +347,72 @@ template <int OP, class TS, class TD, int SUB>
 __global__ __launch_bounds__(256) void sep_x_kernel(const SepParams p) {
   extern __shared__ uint32_t lds_raw[];
   sep_x_body<OP, TS, TD, SUB>(p, blockIdx.x, lds_raw);
+}
+
+// x pass for rows of at most 512 pixels, several rows per wave.  A task is
+// k = nseg consecutive rows, staged in LDS as padded rows (c edge pixels,
+// the row, c edge pixels; k (W + 2c) <= 512 + 2c words, one pad word per 8).
+// Lane l computes the 8 padded positions c + 8l .. c + 8l + 7 with the
+// register ring; positions that fall in a halo are computed and dropped.
+// Each wave uses its own LDS rows, so waves synchronise with themselves.
+template <int OP, class TS, class TD>
+__device__ __forceinline__ void sep_xpack_body(const SepParams& p, const uint32_t vblk, uint32_t* lds_raw) {
+  using VT = vt_t<OP, TS>;
+  constexpr int K = kXK;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  const uint32_t task = vblk * 4u + (uint32_t)wv;
+  if (task >= (uint32_t)p.ntasks) return;
+  VT* const buf = reinterpret_cast<VT*>(lds_raw) + wv * p.lds_stride;
+  const int W = (int)p.L;
+  const int c = p.c;
+  const int PW = W + 2 * c;
+  const int64_t row0 = (int64_t)task * p.nseg;
+  const int k = (int)(p.outer - row0 < p.nseg ? p.outer - row0 : p.nseg);
+  const TS* __restrict__ rp0 = static_cast<const TS*>(p.src) + row0 * W;
+  for (int r = 0; r < k; ++r) {
+    const TS* __restrict__ rp = rp0 + (int64_t)r * W;
+    for (int e = lane; e < PW; e += 64) {
+      int x = e - c;
+      x = x < 0 ? 0 : (x >= W ? W - 1 : x);
+      buf[lds_pad(r * PW + e)] = static_cast<VT>(rp[x]);
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  VT acc[K];
+  x_ring<OP, TS, VT>(buf + (K + 1) * lane, c, p.ntaps, p, acc);
+  // position i = c + 8 lane + q is x = (i mod PW) - c of padded row i / PW
+  const int i0 = c + K * lane;
+  int r = i0 / PW;
+  int x = i0 - r * PW - c;
+  TD* __restrict__ dp = static_cast<TD*>(p.dst) + row0 * W;
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    if (x >= W + c) {
+      ++r;
+      x -= PW;
+    }
+    if (r < k && x >= 0 && x < W) dp[(int64_t)r * W + x] = finish<OP, TS, TD, VT>(acc[q], p);
+    ++x;
+  }
+}
+
+template <int OP, class TS, class TD>
+__global__ __launch_bounds__(256) void sep_xpack_kernel(const SepParams p) {
+  extern __shared__ uint32_t lds_raw[];
+  sep_xpack_body<OP, TS, TD>(p, blockIdx.x, lds_raw);
+}
+
+// Rows per packed task and the LDS words per wave for half width c (W <= 512).
+inline void plan_xpack(SepParams& p) {
+  const int64_t PW = p.L + 2 * (int64_t)p.c;
+  int64_t k = (kXSeg + 2 * (int64_t)p.c) / PW;
+  if (k < 1) k = 1;
+  p.nseg = (int32_t)k;
+  p.ntasks = (p.outer + k - 1) / k;
+  p.lds_stride = lds_pad(kXSeg + 2 * p.c + kXK) + 1;
 }
 
 }  // namespace clk
@@ -574,6 +650,16 @@ int32_t launch_sep_typed(hipStream_t st, SepParams& p, int axis_is_x, std::strin
       return CLK_INVALID_VALUE;
     }
     rc = launch_xreg<OP, TS, TD>(st, p, err);
+  } else if (axis_is_x && p.L <= 256 && p.c <= 256 && tuning(TUNE_NO_XPACK) == 0) {
+    // short rows: several rows per wave
+    plan_xpack(p);
+    if (p.ntasks > 0x7ffffff0LL) {
+      *err = "volume too large for the x pass grid";
+      return CLK_INVALID_VALUE;
+    }
+    const size_t shmem = (size_t)4 * p.lds_stride * 4;
+    hipLaunchKernelGGL((sep_xpack_kernel<OP, TS, TD>), dim3((uint32_t)((p.ntasks + 3) / 4)), dim3(256), shmem, st,
+                       p);
   } else if (axis_is_x && p.c <= kMaxCX) {
     const int64_t W = p.L;
     // larger tasks for long rows and narrow halos (more loads in flight per

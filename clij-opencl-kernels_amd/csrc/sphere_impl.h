@@ -86,4 +86,76 @@ __global__ __launch_bounds__(256) void sphere_kernel(const SphereParams p) {
   }
 }
 
+// LDS-tiled form for small neighbourhoods: a block filters 256 consecutive x
+// of one (y, z).  It stages every source row (y+dy, z+dz) that holds a
+// neighbourhood offset, x0-ex .. x0+255+ex clamped, once in LDS, then each
+// thread walks the offset list (host-built, in the reference's x, y, z loop
+// order) reading LDS.  Values are staged as floats (min/max/mean) or as
+// order-preserving keys of the dst-typed value (median, by bisection).
+constexpr int kSphereTileW = 256;
+constexpr int kSphereTileMaxRows = 49;
+constexpr int kSphereTileMaxOff = 400;
+
+struct SphereTileParams {
+  const void* src;
+  void* dst;
+  int64_t W, H, D;
+  int32_t ex, nrows, noff, n;
+  int8_t rdy[kSphereTileMaxRows], rdz[kSphereTileMaxRows];
+  uint32_t off[kSphereTileMaxOff];  // (dx + 128) | row << 8
+};
+
+template <class TS, class TD, int OP>
+__global__ __launch_bounds__(256) void sphere_tile_kernel(const SphereTileParams p) {
+  extern __shared__ uint32_t tile[];
+  const int t = threadIdx.x;
+  const int64_t x0 = (int64_t)blockIdx.x * kSphereTileW;
+  const int64_t y = blockIdx.y, z = blockIdx.z;
+  const int rowlen = kSphereTileW + 2 * p.ex;
+  const TS* __restrict__ src = static_cast<const TS*>(p.src);
+  for (int r = 0; r < p.nrows; ++r) {
+    const int64_t yy = clamp64(y + p.rdy[r], 0, p.H - 1), zz = clamp64(z + p.rdz[r], 0, p.D - 1);
+    const TS* row = src + (zz * p.H + yy) * p.W;
+    for (int e = t; e < rowlen; e += kSphereTileW) {
+      const TS v = row[clamp64(x0 - p.ex + e, 0, p.W - 1)];
+      uint32_t w;
+      if constexpr (OP == SPH_MEDIAN) w = value_key<TD>(gather_value<TS, TD>(v, 0));
+      else w = __float_as_uint(static_cast<float>(v));
+      tile[r * rowlen + e] = w;
+    }
+  }
+  __syncthreads();
+  const int64_t x = x0 + t;
+  if (x >= p.W) return;
+  auto at = [&](uint32_t o) -> uint32_t {
+    return tile[(int)(o >> 8) * rowlen + t + p.ex + ((int)(o & 0xff) - 128)];
+  };
+  TD* const out = static_cast<TD*>(p.dst) + (z * p.H + y) * p.W + x;
+  if constexpr (OP == SPH_MEDIAN) {
+    const int k = p.n / 2;
+    uint32_t lo = 0;
+    uint32_t hi = (KeyBits<TD>::v == 32) ? 0xffffffffu : ((1u << KeyBits<TD>::v) - 1u);
+    while (lo < hi) {
+      const uint32_t mid = lo + ((hi - lo) >> 1);
+      int cnt = 0;
+      for (int i = 0; i < p.noff; ++i) cnt += at(p.off[i]) <= mid;
+      if (cnt >= k + 1) hi = mid; else lo = mid + 1;
+    }
+    *out = static_cast<TD>(key_value<TD>(lo));
+  } else if constexpr (OP == SPH_MEAN) {
+    float sum = 0.f;
+    for (int i = 0; i < p.noff; ++i) sum = sum + __uint_as_float(at(p.off[i]));
+    *out = convert_sat<TD>(sum / static_cast<float>(p.noff));
+  } else {
+    // the centre is an offset of every neighbourhood that is not empty; an
+    // empty one (negative N) never takes this kernel
+    float m = static_cast<float>(src[(z * p.H + y) * p.W + x]);
+    for (int i = 0; i < p.noff; ++i) {
+      const float v = __uint_as_float(at(p.off[i]));
+      if (OP == SPH_MIN ? (v < m) : (v > m)) m = v;
+    }
+    *out = convert_sat<TD>(m);
+  }
+}
+
 }  // namespace clk

@@ -504,3 +504,46 @@ def test_copy_3d_unaligned_and_ragged(gpu):
     got = dst.cpu().numpy()[1:].reshape(a.shape)
     assert_parity(got, O.copy_3d(a, np.float32), "unaligned copy")
     assert dst.cpu().numpy()[0] == 0
+
+
+@pytest.mark.parametrize("direct", [0, 1])
+def test_sphere_tiled_and_direct_paths(gpu, direct):
+    """Both ellipsoid kernels (LDS row tiles; per-voxel gathers) against the
+    oracle, including a neighbourhood too large for the tiles (r = 7)."""
+    import clkmi
+    a = random_volume((9, 12, 300), np.int16, seed=12 + direct)
+    ts, si = to_device(a)
+    old = clkmi.get_tuning("sphere_direct")
+    try:
+        clkmi.set_tuning("sphere_direct", direct)
+        for name in SPHERE_OPS:
+            for Ns in ((3, 3, 3), (7, 7, 7), (15, 15, 15), (9, 1, 3)):
+                if name == "median_sphere_3d" and Ns == (15, 15, 15):
+                    continue  # a 1400-value bisection per voxel: slow, covered by (7, 7, 7)
+                td, di = empty_device(a.shape, np.float32)
+                getattr(clkmi, name)(di, si, *Ns)
+                assert_parity(from_device(td, a.shape, np.float32), O.sphere_3d(name, a, np.float32, *Ns),
+                              "%s %s direct=%d" % (name, Ns, direct))
+    finally:
+        clkmi.set_tuning("sphere_direct", old)
+
+
+@pytest.mark.parametrize("pack", [0, 1])
+@pytest.mark.parametrize("shape", [(3, 5, 7), (2, 9, 37), (4, 3, 144), (2, 5, 250), (1, 1, 3), (6, 7, 256)])
+def test_packed_short_rows_x_pass(gpu, shape, pack):
+    """Rows of at most 256 pixels several per wave (padded rows in LDS,
+    halo positions dropped) against the oracle, and against the one-row-per-
+    wave pass; half widths up to beyond the row length."""
+    import clkmi
+    old = clkmi.get_tuning("no_xpack")
+    try:
+        clkmi.set_tuning("no_xpack", 1 - pack)
+        for src_t, dst_t in ((np.uint16, np.float32), (np.float32, np.float32), (np.int8, np.int8), (np.uint16, np.uint16)):
+            a = random_volume(shape, src_t, seed=shape[2] + pack)
+            for name in SEP_OPS:
+                for c in (1, 3, 5, 16, 40):
+                    got = gpu_separable(name, a, dst_t, 0, 2 * c + 1, 0.3 * c + 0.5)
+                    want = O.separable(name, a, dst_t, 0, 2 * c + 1, 0.3 * c + 0.5)
+                    assert_parity(got, want, "%s %s c=%d pack=%d" % (name, shape, c, pack))
+    finally:
+        clkmi.set_tuning("no_xpack", old)
