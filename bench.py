@@ -392,6 +392,10 @@ def bench_neighbourhood(args, torch, clkmi):
     plan.append(("median_slice_by_slice_box_3d 3x3 u8",
                  lambda: clkmi.median_slice_by_slice_box_3d(di, si, 3, 3, stream=stream), V, 2))
     plan.append(("copy_3d u8->f32", lambda: clkmi.copy_3d(dfi, si, stream=stream), V, 5))
+    plan.append(("detect_maxima_3d u8->u8", lambda: clkmi.detect_maxima_3d(si, di, stream=stream), V, 2))
+    proj = torch.empty((1, H, W), dtype=torch.float32, device="cuda")
+    pi = clkmi.Image(proj.data_ptr(), CLK_FLOAT, W, H, 1)
+    plan.append(("maximum_z_projection u8->f32", lambda: clkmi.maximum_z_projection(pi, si, stream=stream), V, 1))
     for dim in (0, 1):
         plan.append(("gaussian_blur_separable_2d dim=%d u16->f32 N=25" % dim,
                      lambda dim=dim: clkmi.gaussian_blur_separable_2d(o2, i2, dim, 25, 3.0, stream=stream), V2, 6))
@@ -420,7 +424,8 @@ def bench_neighbourhood(args, torch, clkmi):
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(tot, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8/u16/f32",
         "data": "synthetic: uint8 512x512x256 as the median workload (seed 3); uint16 8192x8192 Poisson + blobs (seed 7)",
-        "config": {"workload": "sphere filters r=1,2, slice median, copy_3d, 2D siblings"},
+        "config": {"workload": "sphere filters r=1,2, slice median, copy_3d, detect_maxima_3d, "
+                               "maximum_z_projection, 2D siblings"},
         "launches": res,
     }
     print(json.dumps(out))

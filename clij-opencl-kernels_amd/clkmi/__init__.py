@@ -24,7 +24,7 @@ __all__ = [
     "gaussian_blur_separable_2d", "mean_separable_2d", "minimum_separable_2d", "maximum_separable_2d",
     "median_box_2d", "gaussian_blur_2d", "mean_box_2d", "minimum_box_2d", "maximum_box_2d",
     "minimum_sphere_3d", "maximum_sphere_3d", "mean_sphere_3d", "median_sphere_3d",
-    "median_slice_by_slice_box_3d", "copy_3d",
+    "median_slice_by_slice_box_3d", "copy_3d", "detect_maxima_3d", "maximum_z_projection",
 ]
 
 # Kernel name -> declared parameters (reference .cl declarations).
@@ -45,6 +45,8 @@ KERNELS = {
     "median_sphere_3d": ("dst", "src", "Nx", "Ny", "Nz"),           # median_sphere_3d_x.cl:24-28
     "median_slice_by_slice_box_3d": ("dst", "src", "Nx", "Ny"),     # median_slice_by_slice_box_3d_x.cl:24-27
     "copy_3d": ("dst", "src"),                                      # copy_3d_x.cl:2-3
+    "detect_maxima_3d": ("src", "dst"),                             # detect_maxima_3d_x.cl:3-4
+    "maximum_z_projection": ("dst_max", "src"),                     # maximum_z_projection_x.cl:3-4
 }
 
 Image = namedtuple("Image", "ptr dtype width height depth")
@@ -182,6 +184,20 @@ def copy_3d(dst, src, stream=None):
     return _nbh("copy_3d", dst, src, (), stream)
 
 
+def detect_maxima_3d(src, dst, stream=None):
+    """detect_maxima_3d(src, dst): 1 at local maxima of the 3x3x3 box (note src first)."""
+    d, sr = to_clk_image(dst), to_clk_image(src)
+    check(lib().clk_detect_maxima_3d(_stream_ptr(stream), ctypes.byref(sr), ctypes.byref(d)))
+    return dst
+
+
+def maximum_z_projection(dst_max, src, stream=None):
+    """maximum_z_projection(dst_max, src): maximum over z into a W x H x 1 image."""
+    d, sr = to_clk_image(dst_max), to_clk_image(src)
+    check(lib().clk_maximum_z_projection(_stream_ptr(stream), ctypes.byref(d), ctypes.byref(sr)))
+    return dst_max
+
+
 def execute(kernel_name, parameters, stream=None):
     """Launch a kernel by name with parameters bound by name, as a CLIJ host's
     execute(kernelName, parameterMap).  The name may also be the program file
@@ -201,7 +217,7 @@ def execute(kernel_name, parameters, stream=None):
             a.kind = _lib.CLK_ARG_IMAGE
             a.image = ctypes.pointer(im)
     check(lib().clk_execute(_stream_ptr(stream), kernel_name.encode(), args, len(parameters)))
-    return parameters.get("dst")
+    return parameters.get("dst", parameters.get("dst_max"))
 
 
 def sigma_to_kernel_size(sigma):

@@ -36,6 +36,7 @@ EXPORTED = (
     "clk_maximum_separable_2d", "clk_median_box_2d", "clk_gaussian_blur_2d", "clk_mean_box_2d",
     "clk_minimum_box_2d", "clk_maximum_box_2d", "clk_minimum_sphere_3d", "clk_maximum_sphere_3d",
     "clk_mean_sphere_3d", "clk_median_sphere_3d", "clk_median_slice_by_slice_box_3d", "clk_copy_3d",
+    "clk_detect_maxima_3d", "clk_maximum_z_projection",
 )
 
 CLK_JOB_GAUSSIAN_BLUR_3D, CLK_JOB_MEAN_BOX_3D, CLK_JOB_MINIMUM_BOX_3D, CLK_JOB_MAXIMUM_BOX_3D, \
@@ -118,6 +119,10 @@ def lib():
         L.clk_median_slice_by_slice_box_3d.restype = i32
         L.clk_copy_3d.argtypes = [vp, img, img]
         L.clk_copy_3d.restype = i32
+        L.clk_detect_maxima_3d.argtypes = [vp, img, img]
+        L.clk_detect_maxima_3d.restype = i32
+        L.clk_maximum_z_projection.argtypes = [vp, img, img]
+        L.clk_maximum_z_projection.restype = i32
         L.clk_median_box_2d.argtypes = [vp, img, img, i32, i32]
         L.clk_median_box_2d.restype = i32
         L.clk_gaussian_blur_2d.argtypes = [vp, img, img, f32, f32]

@@ -67,6 +67,10 @@ int32_t prepare_separable(int op, const clk_image& dst, const clk_image& src, in
                           float s, SepParams& p, int* axis_is_x, bool* fill_nan, std::string* err);
 int32_t launch_fill_nan(hipStream_t st, const clk_image& dst, std::string* err);
 
+// detect_maxima_3d (detect = true; dst and src of one size) or
+// maximum_z_projection (dst W x H x 1); arguments validated by the caller.
+int32_t launch_consumer(bool detect, hipStream_t st, const clk_image& dst, const clk_image& src, std::string* err);
+
 // minimum/maximum/mean/median_sphere_3d (op 0..3); arguments validated by
 // the caller.
 int32_t launch_sphere(int op, hipStream_t st, const clk_image& dst, const clk_image& src, int32_t Nx,

@@ -311,6 +311,33 @@ int32_t clk_copy_3d(void* stream, const clk_image* dst, const clk_image* src) {
   return ok();
 }
 
+int32_t clk_detect_maxima_3d(void* stream, const clk_image* src, const clk_image* dst) {
+  int32_t rc = check_pair("detect_maxima_3d", dst, src);
+  if (rc) return rc;
+  std::string err;
+  rc = clk::launch_consumer(true, (hipStream_t)stream, *dst, *src, &err);
+  if (rc) return fail(rc, "detect_maxima_3d: " + err);
+  return ok();
+}
+
+int32_t clk_maximum_z_projection(void* stream, const clk_image* dst_max, const clk_image* src) {
+  int32_t rc = check_image(dst_max, "dst_max");
+  if (rc) return rc;
+  rc = check_image(src, "src");
+  if (rc) return rc;
+  if (dst_max->width != src->width || dst_max->height != src->height || dst_max->depth != 1)
+    return fail(CLK_INVALID_VALUE, "maximum_z_projection: dst_max must be width x height x 1 of src");
+  const uintptr_t s0 = (uintptr_t)src->ptr, d0 = (uintptr_t)dst_max->ptr;
+  const uintptr_t s1 = s0 + (uintptr_t)(voxels(*src) * clk_dtype_size(src->dtype));
+  const uintptr_t d1 = d0 + (uintptr_t)(voxels(*dst_max) * clk_dtype_size(dst_max->dtype));
+  if (voxels(*dst_max) > 0 && voxels(*src) > 0 && s0 < d1 && d0 < s1)
+    return fail(CLK_MEM_COPY_OVERLAP, "maximum_z_projection: src and dst_max overlap");
+  std::string err;
+  rc = clk::launch_consumer(false, (hipStream_t)stream, *dst_max, *src, &err);
+  if (rc) return fail(rc, "maximum_z_projection: " + err);
+  return ok();
+}
+
 int32_t clk_execute(void* stream, const char* kernel_name, const clk_arg* args, int32_t nargs) {
   if (!kernel_name) return fail(CLK_INVALID_KERNEL_NAME, "kernel name is NULL");
   if (nargs < 0 || (nargs > 0 && !args)) return fail(CLK_INVALID_KERNEL_ARGS, "invalid argument list");
@@ -329,9 +356,10 @@ int32_t clk_execute(void* stream, const char* kernel_name, const clk_arg* args, 
   else if (base == "maximum_separable_3d") op = clk::OP_MAX;
   else if (base == "median_box_3d") op = 4;
   else if (!two_d) {
-    // kernels with (dst, src[, Nx, Ny[, Nz]]): spheres, slice median, copy
-    // (minimum_sphere_3d_x.cl:5-9, median_slice_by_slice_box_3d_x.cl:24-28,
-    // copy_3d_x.cl:1-4)
+    // kernels with (dst, src[, Nx, Ny[, Nz]]): spheres, slice median, copy,
+    // and the consumers (minimum_sphere_3d_x.cl:5-9,
+    // median_slice_by_slice_box_3d_x.cl:24-28, copy_3d_x.cl:1-4,
+    // detect_maxima_3d_x.cl:2-5, maximum_z_projection_x.cl:2-5)
     int kind = -1, nsc = 3;
     if (k == "minimum_sphere_3d") kind = 0;
     else if (k == "maximum_sphere_3d") kind = 1;
@@ -339,6 +367,8 @@ int32_t clk_execute(void* stream, const char* kernel_name, const clk_arg* args, 
     else if (k == "median_sphere_3d") kind = 3;
     else if (k == "median_slice_by_slice_box_3d") { kind = 4; nsc = 2; }
     else if (k == "copy_3d") { kind = 5; nsc = 0; }
+    else if (k == "detect_maxima_3d") { kind = 6; nsc = 0; }
+    else if (k == "maximum_z_projection") { kind = 7; nsc = 0; }
     if (kind >= 0) {
       static const char* nn[3] = {"Nx", "Ny", "Nz"};
       const clk_image *d = nullptr, *sr = nullptr;
@@ -348,9 +378,11 @@ int32_t clk_execute(void* stream, const char* kernel_name, const clk_arg* args, 
         const clk_arg& a = args[i];
         if (!a.name) return fail(CLK_INVALID_KERNEL_ARGS, k + ": argument without a name");
         const std::string nm(a.name);
-        if (nm == "dst" || nm == "src") {
+        // maximum_z_projection names its output dst_max (maximum_z_projection_x.cl:3)
+        const std::string dname = kind == 7 ? "dst_max" : "dst";
+        if (nm == dname || nm == "src") {
           if (a.kind != CLK_ARG_IMAGE) return fail(CLK_INVALID_KERNEL_ARGS, k + ": " + nm + " must be an image");
-          (nm == "dst" ? d : sr) = a.image;
+          (nm == dname ? d : sr) = a.image;
           continue;
         }
         int slot = -1;
@@ -361,11 +393,13 @@ int32_t clk_execute(void* stream, const char* kernel_name, const clk_arg* args, 
         v[slot] = (a.kind == CLK_ARG_INT) ? a.i : (int32_t)a.f;
         has[slot] = true;
       }
-      if (!d || !sr) return fail(CLK_INVALID_KERNEL_ARGS, k + ": dst and src are required");
+      if (!d || !sr) return fail(CLK_INVALID_KERNEL_ARGS, k + ": both images are required");
       for (int j = 0; j < nsc; ++j)
         if (!has[j]) return fail(CLK_INVALID_KERNEL_ARGS, k + ": missing parameter " + nn[j]);
       if (kind <= 3) return sphere(kind, k.c_str(), stream, d, sr, v[0], v[1], v[2]);
       if (kind == 4) return clk_median_slice_by_slice_box_3d(stream, d, sr, v[0], v[1]);
+      if (kind == 6) return clk_detect_maxima_3d(stream, sr, d);
+      if (kind == 7) return clk_maximum_z_projection(stream, d, sr);
       return clk_copy_3d(stream, d, sr);
     }
   }

@@ -161,6 +161,22 @@ int32_t clk_median_slice_by_slice_box_3d(void* stream, const clk_image* dst,
 /* copy_3d(dst, src) replaces kernels/copy_3d_x.cl:1-15: dst = CONVERT(src). */
 int32_t clk_copy_3d(void* stream, const clk_image* dst, const clk_image* src);
 
+/* ---- consumers usually chained after smoothing ------------------------ */
+
+/* detect_maxima_3d(src, dst) replaces kernels/detect_maxima_3d_x.cl:2-5
+ * (note the reference's parameter order, src first): dst = 1 where the first
+ * strict maximum of the clamped 3x3x3 neighbourhood, in the loop order x, y,
+ * z and starting from centre - 1, lies at the centre itself, else 0.  The
+ * reference's debug printf from work-item (0,0,0) is not reproduced. */
+int32_t clk_detect_maxima_3d(void* stream, const clk_image* src,
+                             const clk_image* dst);
+
+/* maximum_z_projection(dst_max, src) replaces
+ * kernels/maximum_z_projection_x.cl:2-5: dst_max (width x height x 1) =
+ * CONVERT(maximum over z), the first value then strictly greater ones. */
+int32_t clk_maximum_z_projection(void* stream, const clk_image* dst_max,
+                                 const clk_image* src);
+
 /* ---- by-name dispatch (CLIJ execute(kernelName, parameterMap)) --------- */
 
 typedef enum clk_arg_kind {
@@ -181,7 +197,8 @@ typedef struct clk_arg {
 /* Launch a kernel by its reference name ("gaussian_blur_separable_3d", or the
  * program file name "gaussian_blur_separable_3d_x.cl"; the names are the five
  * 3D kernels, their _2d siblings, the four _sphere_3d filters,
- * median_slice_by_slice_box_3d and copy_3d) with arguments bound
+ * median_slice_by_slice_box_3d, copy_3d, detect_maxima_3d and
+ * maximum_z_projection) with arguments bound
  * by parameter name, as a CLIJ host binds them (reference README.md:26).
  * An int argument is accepted for a float parameter and vice versa, as the
  * reference hosts convert Java/Python numbers.  Unknown names or missing

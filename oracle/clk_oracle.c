@@ -429,6 +429,51 @@ int clko_copy_3d(void* dst, int dst_t, const void* src, int src_t, int64_t n) {
   return 0;
 }
 
+/* ---- consumers: detect_maxima_3d, maximum_z_projection ------------------ */
+
+/* detect_maxima_3d_x.cl:9-50 (src first in the reference's declaration).
+ * The position of the running maximum is the UNclamped neighbour position
+ * (:27, :37); reads are clamped by the sampler.  The debug printf (:15-17)
+ * is left out. */
+int clko_detect_maxima_3d(const void* src, int src_t, void* dst, int dst_t, int64_t W, int64_t H, int64_t D) {
+  if (src_t < 0 || src_t > 4 || dst_t < 0 || dst_t > 4) return -30;
+  const img_t im = {src, src_t, W, H, D};
+  const int radius = 1; /* :9 */
+  for (int64_t k = 0; k < D; k++)
+    for (int64_t j = 0; j < H; j++)
+      for (int64_t i = 0; i < W; i++) {
+        float local_max = read_px(&im, i, j, k) - 1;          /* :21 */
+        int64_t mx = i, my = j, mz = k;                       /* :22 */
+        for (int x = -radius; x < radius + 1; x++)           /* :24 */
+          for (int y = -radius; y < radius + 1; y++)         /* :26 */
+            for (int z = -radius; z < radius + 1; z++) {     /* :28 */
+              const float value = read_px(&im, i + x, j + y, k + z); /* :32 */
+              if (value > local_max) {                       /* :35 */
+                local_max = value;
+                mx = i + x; my = j + y; mz = k + z;          /* :37 */
+              }
+            }
+        write_px(dst, dst_t, i + j * W + k * W * H, (mx == i && my == j && mz == k) ? 1.f : 0.f); /* :45-50 */
+      }
+  return 0;
+}
+
+/* maximum_z_projection_x.cl:7-19: dst_max is W x H. */
+int clko_maximum_z_projection(void* dst, int dst_t, const void* src, int src_t, int64_t W, int64_t H, int64_t D) {
+  if (src_t < 0 || src_t > 4 || dst_t < 0 || dst_t > 4) return -30;
+  const img_t im = {src, src_t, W, H, D};
+  for (int64_t y = 0; y < H; y++)
+    for (int64_t x = 0; x < W; x++) {
+      float m = 0;                                  /* :9 */
+      for (int64_t z = 0; z < D; z++) {             /* :10 */
+        const float value = read_px(&im, x, y, z); /* :13 */
+        if (value > m || z == 0) m = value;         /* :14 */
+      }
+      write_px(dst, dst_t, x + y * W, clko_convert(m, dst_t)); /* :18 */
+    }
+  return 0;
+}
+
 /* Correctly rounded a/b through RN(1/b) and one FMA residual step, the
  * formula the GPU kernels use for a launch-uniform divisor; exported so the
  * tests can check it against the IEEE divide on many inputs. */

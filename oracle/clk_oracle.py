@@ -59,6 +59,10 @@ def lib():
         L.clko_median_slice_by_slice_box_3d.restype = i32
         L.clko_copy_3d.argtypes = [vp, i32, vp, i32, i64]
         L.clko_copy_3d.restype = i32
+        L.clko_detect_maxima_3d.argtypes = [vp, i32, vp, i32, i64, i64, i64]
+        L.clko_detect_maxima_3d.restype = i32
+        L.clko_maximum_z_projection.argtypes = [vp, i32, vp, i32, i64, i64, i64]
+        L.clko_maximum_z_projection.restype = i32
         L.clko_separable_2d.argtypes = [i32, vp, i32, vp, i32, i64, i64, i32, i32, f32, i32]
         L.clko_separable_2d.restype = i32
         L.clko_median_box_2d.argtypes = [vp, i32, vp, i32, i64, i64, i32, i32, i32]
@@ -192,6 +196,29 @@ def copy_3d(src, dst_dtype):
     src = np.ascontiguousarray(src)
     dst = np.empty(src.shape, dtype=np.dtype(dst_dtype))
     rc = lib().clko_copy_3d(dst.ctypes.data, DTYPE_CODES[dst.dtype], src.ctypes.data, DTYPE_CODES[src.dtype], src.size)
+    if rc != 0:
+        raise ValueError("oracle rejected arguments (%d)" % rc)
+    return dst
+
+
+def detect_maxima_3d(src, dst_dtype):
+    src = _vol(src)
+    dst = np.empty(src.shape, dtype=np.dtype(dst_dtype))
+    D, H, W = src.shape
+    rc = lib().clko_detect_maxima_3d(src.ctypes.data, DTYPE_CODES[src.dtype], dst.ctypes.data,
+                                     DTYPE_CODES[dst.dtype], W, H, D)
+    if rc != 0:
+        raise ValueError("oracle rejected arguments (%d)" % rc)
+    return dst
+
+
+def maximum_z_projection(src, dst_dtype):
+    """(D, H, W) -> (1, H, W)."""
+    src = _vol(src)
+    D, H, W = src.shape
+    dst = np.empty((1, H, W), dtype=np.dtype(dst_dtype))
+    rc = lib().clko_maximum_z_projection(dst.ctypes.data, DTYPE_CODES[dst.dtype], src.ctypes.data,
+                                         DTYPE_CODES[src.dtype], W, H, D)
     if rc != 0:
         raise ValueError("oracle rejected arguments (%d)" % rc)
     return dst

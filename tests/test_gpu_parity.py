@@ -547,3 +547,43 @@ def test_packed_short_rows_x_pass(gpu, shape, pack):
                     assert_parity(got, want, "%s %s c=%d pack=%d" % (name, shape, c, pack))
     finally:
         clkmi.set_tuning("no_xpack", old)
+
+
+# ---- consumers ------------------------------------------------------------------
+
+@pytest.mark.parametrize("src_t", DTYPES, ids=lambda t: DTYPE_NAMES[t])
+@pytest.mark.parametrize("dst_t", DTYPES, ids=lambda t: DTYPE_NAMES[t])
+def test_detect_maxima_and_projection_all_type_pairs(gpu, src_t, dst_t):
+    import clkmi
+    a = random_volume((6, 17, 300), src_t, seed=31)
+    if np.dtype(src_t).kind != "f":
+        a = (a % 5).astype(src_t)  # plateaus and ties
+    else:
+        a.reshape(-1)[::97] = np.nan
+    ts, si = to_device(a)
+    td, di = empty_device(a.shape, dst_t)
+    clkmi.detect_maxima_3d(si, di)
+    assert_parity(from_device(td, a.shape, dst_t), O.detect_maxima_3d(a, dst_t), "detect_maxima_3d")
+    tp, dp = empty_device((1,) + a.shape[1:], dst_t)
+    clkmi.maximum_z_projection(dp, si)
+    assert_parity(from_device(tp, (1,) + a.shape[1:], dst_t), O.maximum_z_projection(a, dst_t),
+                  "maximum_z_projection")
+
+
+def test_consumers_by_name_and_errors(gpu):
+    import clkmi
+    a = random_volume((4, 9, 33), np.uint16, seed=2)
+    ts, si = to_device(a)
+    td, di = empty_device(a.shape, np.uint8)
+    clkmi.execute("detect_maxima_3d_x.cl", {"src": si, "dst": di})
+    assert np.array_equal(from_device(td, a.shape, np.uint8), O.detect_maxima_3d(a, np.uint8))
+    tp, dp = empty_device((1, 9, 33), np.float32)
+    clkmi.execute("maximum_z_projection", {"dst_max": dp, "src": si})
+    assert_parity(from_device(tp, (1, 9, 33), np.float32), O.maximum_z_projection(a, np.float32), "by name")
+    bad_t, bad = empty_device((2, 9, 33), np.float32)   # depth must be 1
+    with pytest.raises(clkmi.ClkError) as e:
+        clkmi.maximum_z_projection(bad, si)
+    assert e.value.code == -30
+    with pytest.raises(clkmi.ClkError) as e:   # projection output is named dst_max
+        clkmi.execute("maximum_z_projection", {"dst": dp, "src": si})
+    assert e.value.code == -52

@@ -440,3 +440,60 @@ def test_slice_median_equals_cast_then_scipy(src_t, dst_t):
 def test_copy_is_convert(src_t, dst_t):
     a = random_volume((3, 4, 5), src_t, seed=1)
     assert same_bits_or_value(O.copy_3d(a, dst_t), O.convert(a.astype(np.float32), dst_t))
+
+
+# ---- consumers --------------------------------------------------------------------
+
+def _detect_maxima_brute(a):
+    """detect_maxima_3d by brute force in Python (small volumes): the first
+    strict maximum over offsets in x, y, z order, from centre - 1, with the
+    unclamped position."""
+    D, H, W = a.shape
+    out = np.zeros(a.shape, dtype=np.uint8)
+    f = a.astype(np.float64)
+    for k in range(D):
+        for j in range(H):
+            for i in range(W):
+                best, pos = np.float32(f[k, j, i]) - np.float32(1), (0, 0, 0)
+                for x in (-1, 0, 1):
+                    for y in (-1, 0, 1):
+                        for z in (-1, 0, 1):
+                            v = f[min(max(k + z, 0), D - 1), min(max(j + y, 0), H - 1), min(max(i + x, 0), W - 1)]
+                            if v > best:
+                                best, pos = v, (x, y, z)
+                out[k, j, i] = pos == (0, 0, 0)
+    return out
+
+
+@pytest.mark.parametrize("dt", DTYPES)
+def test_detect_maxima_equals_brute_force(dt):
+    a = random_volume((4, 5, 6), dt, seed=8, lo=0 if np.dtype(dt).kind == "f" else None,
+                      hi=5 if np.dtype(dt).kind == "f" else None)
+    if np.dtype(dt).kind != "f":
+        a = (a % 4).astype(dt)  # many ties
+    assert np.array_equal(O.detect_maxima_3d(a, np.uint8), _detect_maxima_brute(a))
+
+
+def test_detect_maxima_known_answers():
+    # a constant volume has no maxima: the first neighbour already equals it
+    assert not O.detect_maxima_3d(np.full((3, 4, 5), 7, np.uint16), np.uint8).any()
+    # one interior peak; a border voxel meets its own clamped value first
+    a = np.zeros((5, 5, 5), np.float32)
+    a[2, 2, 2] = 3
+    got = O.detect_maxima_3d(a, np.float32)
+    assert got[2, 2, 2] == 1 and got.sum() == 1
+    b = np.zeros((5, 5, 5), np.float32)
+    b[0, 0, 0] = 3
+    assert O.detect_maxima_3d(b, np.uint8)[0, 0, 0] == 0
+    # a NaN centre never sees a greater value: it stays a "maximum"
+    c = np.zeros((3, 3, 3), np.float32)
+    c[1, 1, 1] = np.nan
+    assert O.detect_maxima_3d(c, np.uint8)[1, 1, 1] == 1
+
+
+@pytest.mark.parametrize("src_t", DTYPES)
+@pytest.mark.parametrize("dst_t", DTYPES)
+def test_maximum_z_projection_equals_numpy(src_t, dst_t):
+    a = random_volume((7, 5, 6), src_t, seed=3)
+    want = O.convert(a.max(axis=0, keepdims=True).astype(np.float32), dst_t)
+    assert same_bits_or_value(O.maximum_z_projection(a, dst_t), want)
