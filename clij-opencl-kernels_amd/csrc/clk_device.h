@@ -162,4 +162,14 @@ __device__ __forceinline__ int64_t clamp64(int64_t v, int64_t lo, int64_t hi) {
   return v < lo ? lo : (v > hi ? hi : v);
 }
 
+// Whole-wave shifts by one lane (DPP wave_shr:1 / wave_shl:1): lane l gets
+// v of lane l-1 (from_left) or l+1 (from_right); the lane with no source
+// (0, resp. 63) keeps old.
+__device__ __forceinline__ uint32_t dpp_from_left(uint32_t old, uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ uint32_t dpp_from_right(uint32_t old, uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, 0x130, 0xf, 0xf, false);
+}
+
 }  // namespace clk

@@ -241,39 +241,75 @@ __device__ __forceinline__ int bin8(float v) {
 
 constexpr int kHistZSeg = 32;  // outputs per lane along z
 
-template <class TS, class TD>
+template <class TS, class TD, int EX = 0>
 __device__ __forceinline__ void median_hist8_body(const MedParams& p, uint32_t bx, uint32_t by, uint32_t bz) {
   // 256 byte counters per lane packed four to a dword; dword d of lane l is
-  // stored at l*64 + ((d + l) & 63) so equal values of different lanes fall
-  // in different banks.  Counters change by LDS atomic adds whose result is
-  // not returned (no dependency chain between updates); -1 on a byte is
-  // +(2^32 - 2^(8j)), which cannot borrow because a count never goes below 0.
+  // stored at d*64 + l, so every lane always uses its own bank.  Counters
+  // change by LDS atomic adds whose result is not returned (no dependency
+  // chain between updates); -1 on a byte is +(2^32 - 2^(8j)), which cannot
+  // borrow because a count never goes below 0.
+  // EX > 0 fixes the x half width at compile time: each lane then loads one
+  // value per window row and receives the bins of its x neighbours from the
+  // adjacent lanes by DPP wave shifts (lanes 0 and 63 load the clamped halo).
+  // EX == 0 loads every window position itself.
   __shared__ uint32_t hist[64 * 64];
   const int lane = threadIdx.x;
   const int64_t x = (int64_t)bx * 64 + lane;
   const int64_t y = by;
   const int64_t z0 = (int64_t)bz * kHistZSeg;
   const int64_t z1 = (z0 + kHistZSeg < p.D) ? z0 + kHistZSeg : p.D;
-  if (x >= p.W) return;  // lanes never share data: no barrier follows
-  uint32_t* const h = hist + lane * 64;
+  const bool valid = x < p.W;
+  if constexpr (EX == 0) {
+    if (!valid) return;  // lanes never share data on this path
+  }
+  uint32_t* const h = hist + lane;
 #pragma unroll 8
-  for (int i = 0; i < 64; ++i) h[i] = 0u;
+  for (int i = 0; i < 64; ++i) h[i * 64] = 0u;
   const TS* __restrict__ src = static_cast<const TS*>(p.src);
-  const int ex = p.ex, ey = p.ey, ez = p.ez;
+  const int ex = EX > 0 ? EX : p.ex;
+  const int ey = p.ey, ez = p.ez;
   const int W = (int)p.W;
-  const int xi = (int)x;
+  const int xi = valid ? (int)x : W - 1;  // lanes past the row end act as its last pixel
+  // the whole block's x window inside the row: no clamp, offsets from x
+  const bool interior = (int64_t)bx * 64 - ex >= 0 && (int64_t)bx * 64 + 63 + ex < p.W;
   // row pointer of a (z, y) row: uniform across the wave
   auto rowp = [&](int64_t zz, int64_t yy) -> const TS* {
     zz = clamp64(zz, 0, p.D - 1);
     yy = clamp64(yy, 0, p.H - 1);
     return src + (zz * p.H + yy) * p.W;
   };
+  // bin of a source value: for the same 8-bit type the value itself
+  auto bin_of = [&](TS v) -> int {
+    if constexpr (std::is_same<TS, TD>::value) return (int)v - static_cast<int>(Pix<TD>::lo);
+    else return bin8<TD>(gather_value<TS, TD>(v, p.wrap));
+  };
   auto bin_at = [&](const TS* row, int dx) -> int {
+    if (interior) return bin_of(row[xi + dx]);
     int xx = xi + dx;
     xx = xx < 0 ? 0 : (xx >= W ? W - 1 : xx);
-    return bin8<TD>(gather_value<TS, TD>(row[xx], p.wrap));
+    return bin_of(row[xx]);
   };
-  auto word = [&](int b) -> uint32_t* { return h + (((b >> 2) + lane) & 63); };
+  // bins of the 2*EX+1 window positions x-EX .. x+EX of one row
+  auto row_bins = [&](const TS* row, int (&b)[2 * EX + 1]) {
+    b[EX] = bin_of(row[xi]);
+    int edge[EX > 0 ? EX : 1];
+    if (lane == 0 || lane == 63) {
+      // lane 0: x-1-k, lane 63: x+1+k (clamped into the row)
+      const int step = lane == 0 ? -1 : 1;
+#pragma unroll
+      for (int k = 0; k < EX; ++k) {
+        int xx = xi + step * (k + 1);
+        xx = xx < 0 ? 0 : (xx >= W ? W - 1 : xx);
+        edge[k] = bin_of(row[xx]);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < EX; ++k) {
+      b[EX - 1 - k] = (int)dpp_from_left((uint32_t)edge[k], (uint32_t)b[EX - k]);
+      b[EX + 1 + k] = (int)dpp_from_right((uint32_t)edge[k], (uint32_t)b[EX + k]);
+    }
+  };
+  auto word = [&](int b) -> uint32_t* { return h + (b >> 2) * 64; };
   auto add = [&](int b, bool inc) {
     const uint32_t one = 1u << ((b & 3) * 8);
     __hip_atomic_fetch_add(word(b), inc ? one : 0u - one, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -283,7 +319,14 @@ __device__ __forceinline__ void median_hist8_body(const MedParams& p, uint32_t b
   for (int dz = -ez; dz <= ez; ++dz)
     for (int dy = -ey; dy <= ey; ++dy) {
       const TS* row = rowp(z0 + dz, y + dy);
-      for (int dx = -ex; dx <= ex; ++dx) add(bin_at(row, dx), true);
+      if constexpr (EX > 0) {
+        int b[2 * EX + 1];
+        row_bins(row, b);
+#pragma unroll
+        for (int i = 0; i <= 2 * EX; ++i) add(b[i], true);
+      } else {
+        for (int dx = -ex; dx <= ex; ++dx) add(bin_at(row, dx), true);
+      }
     }
   int med = 0, lt = 0;
   for (int c = count(0); lt + c <= k; c = count(med)) {
@@ -292,18 +335,30 @@ __device__ __forceinline__ void median_hist8_body(const MedParams& p, uint32_t b
   }
   TD* __restrict__ dst = static_cast<TD*>(p.dst);
   const int64_t plane = p.H * p.W;
-  dst[z0 * plane + y * p.W + x] = static_cast<TD>(med + static_cast<int>(Pix<TD>::lo));
+  if (valid) dst[z0 * plane + y * p.W + x] = static_cast<TD>(med + static_cast<int>(Pix<TD>::lo));
   for (int64_t z = z0 + 1; z < z1; ++z) {
     const int64_t zo = z - 1 - ez, zi = z + ez;
     for (int dy = -ey; dy <= ey; ++dy) {
       const TS* ro = rowp(zo, y + dy);
       const TS* ri = rowp(zi, y + dy);
-      for (int dx = -ex; dx <= ex; ++dx) {
-        const int a = bin_at(ro, dx);
-        const int b = bin_at(ri, dx);
-        add(a, false);
-        add(b, true);
-        lt += (b < med) - (a < med);
+      if constexpr (EX > 0) {
+        int a[2 * EX + 1], b[2 * EX + 1];
+        row_bins(ro, a);
+        row_bins(ri, b);
+#pragma unroll
+        for (int i = 0; i <= 2 * EX; ++i) {
+          add(a[i], false);
+          add(b[i], true);
+          lt += (b[i] < med) - (a[i] < med);
+        }
+      } else {
+        for (int dx = -ex; dx <= ex; ++dx) {
+          const int a = bin_at(ro, dx);
+          const int b = bin_at(ri, dx);
+          add(a, false);
+          add(b, true);
+          lt += (b < med) - (a < med);
+        }
       }
     }
     // median = the bin m with lt(m) <= k < lt(m) + count(m)
@@ -315,7 +370,7 @@ __device__ __forceinline__ void median_hist8_body(const MedParams& p, uint32_t b
       lt += c;
       ++med;
     }
-    dst[z * plane + y * p.W + x] = static_cast<TD>(med + static_cast<int>(Pix<TD>::lo));
+    if (valid) dst[z * plane + y * p.W + x] = static_cast<TD>(med + static_cast<int>(Pix<TD>::lo));
   }
 }
 
@@ -327,8 +382,12 @@ __device__ __forceinline__ void median_hist8_body(const MedParams& p, uint32_t b
 CLK_MEDIAN_KERNEL(median_generic, 256)
 CLK_MEDIAN_KERNEL(median_27_pk8, 256)
 CLK_MEDIAN_KERNEL(median_27, 256)
-CLK_MEDIAN_KERNEL(median_hist8, 64)
 #undef CLK_MEDIAN_KERNEL
+
+template <class TS, class TD, int EX>
+__global__ __launch_bounds__(64) void median_hist8_kernel(const MedParams p) {
+  median_hist8_body<TS, TD, EX>(p, blockIdx.x, blockIdx.y, blockIdx.z);
+}
 
 template <class TS, class TD>
 __global__ __launch_bounds__(256) void median_9_pk8_kernel(const MedParams p) {

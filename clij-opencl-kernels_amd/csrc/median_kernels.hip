@@ -17,7 +17,12 @@ static int32_t median_typed(hipStream_t st, MedParams& p, std::string* err) {
   switch (plan_median(p, (int)sizeof(TD), &grid, &threads)) {
     case MED_PK8: hipLaunchKernelGGL((median_27_pk8_kernel<TS, TD>), grid, dim3(threads), 0, st, p); break;
     case MED_PK9: hipLaunchKernelGGL((median_9_pk8_kernel<TS, TD>), grid, dim3(threads), 0, st, p); break;
-    case MED_HIST8: hipLaunchKernelGGL((median_hist8_kernel<TS, TD>), grid, dim3(threads), 0, st, p); break;
+    case MED_HIST8:
+      if (p.ex == 1) hipLaunchKernelGGL((median_hist8_kernel<TS, TD, 1>), grid, dim3(threads), 0, st, p);
+      else if (p.ex == 2) hipLaunchKernelGGL((median_hist8_kernel<TS, TD, 2>), grid, dim3(threads), 0, st, p);
+      else if (p.ex == 3) hipLaunchKernelGGL((median_hist8_kernel<TS, TD, 3>), grid, dim3(threads), 0, st, p);
+      else hipLaunchKernelGGL((median_hist8_kernel<TS, TD, 0>), grid, dim3(threads), 0, st, p);
+      break;
     case MED_27: hipLaunchKernelGGL((median_27_kernel<TS, TD>), grid, dim3(threads), 0, st, p); break;
     default: hipLaunchKernelGGL((median_generic_kernel<TS, TD>), grid, dim3(threads), 0, st, p); break;
   }

@@ -14,13 +14,6 @@
 
 namespace clk {
 
-__device__ __forceinline__ uint32_t dpp_from_left(uint32_t old, uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, 0x138, 0xf, 0xf, false);  // wave_shr:1
-}
-__device__ __forceinline__ uint32_t dpp_from_right(uint32_t old, uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, 0x130, 0xf, 0xf, false);  // wave_shl:1
-}
-
 // Pixel i (little-endian order) of a 32-bit word holding 4 / sizeof(TS)
 // pixels, by shifts: no byte addressing, so the words stay in registers.
 template <class TS>

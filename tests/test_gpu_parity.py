@@ -122,6 +122,19 @@ def test_median_salt_and_pepper(gpu):
         assert_parity(got, want, "median r=%d" % r)
 
 
+@pytest.mark.parametrize("src_t", (np.uint8, np.int8, np.float32), ids=lambda t: DTYPE_NAMES[t])
+def test_median_hist8_wide_rows(gpu, src_t):
+    # 8-bit dst with a window beyond 3x3x3 takes the histogram kernel: rows of
+    # 150 pixels span three 64-lane blocks (interior, halo lanes 0/63, a
+    # partial last block) and depth 70 spans three z segments
+    src = random_volume((70, 5, 150), src_t, seed=29)
+    for Ns in ((5, 5, 5), (3, 5, 7), (7, 3, 5), (9, 3, 3), (5, 5, 1)):
+        for dst_t in (np.uint8, np.int8):
+            got = gpu_median(src, dst_t, *Ns)
+            want = O.median_box_3d(src, dst_t, *Ns)
+            assert_parity(got, want, "median %s -> %s %s" % (DTYPE_NAMES[src_t], DTYPE_NAMES[dst_t], Ns))
+
+
 def test_execute_by_name_matches_typed_call(gpu):
     import clkmi
     src = random_volume((6, 7, 8), np.uint16, seed=1)
