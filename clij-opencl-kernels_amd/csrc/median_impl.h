@@ -241,127 +241,26 @@ __device__ __forceinline__ int bin8(float v) {
 
 constexpr int kHistZSeg = 32;  // outputs per lane along z
 
-template <class TS, class TD, int EX = 0>
-__device__ __forceinline__ void median_hist8_body(const MedParams& p, uint32_t bx, uint32_t by, uint32_t bz) {
-  // 256 byte counters per lane packed four to a dword; dword d of lane l is
-  // stored at d*64 + l, so every lane always uses its own bank.  Counters
-  // change by LDS atomic adds whose result is not returned (no dependency
-  // chain between updates); -1 on a byte is +(2^32 - 2^(8j)), which cannot
-  // borrow because a count never goes below 0.
-  // EX > 0 fixes the x half width at compile time: each lane then loads one
-  // value per window row and receives the bins of its x neighbours from the
-  // adjacent lanes by DPP wave shifts (lanes 0 and 63 load the clamped halo).
-  // EX == 0 loads every window position itself.
-  __shared__ uint32_t hist[64 * 64];
-  const int lane = threadIdx.x;
-  const int64_t x = (int64_t)bx * 64 + lane;
-  const int64_t y = by;
-  const int64_t z0 = (int64_t)bz * kHistZSeg;
-  const int64_t z1 = (z0 + kHistZSeg < p.D) ? z0 + kHistZSeg : p.D;
-  const bool valid = x < p.W;
-  if constexpr (EX == 0) {
-    if (!valid) return;  // lanes never share data on this path
-  }
-  uint32_t* const h = hist + lane;
+// Shared by the histogram median bodies: 256 byte counters per lane packed
+// four to a dword; dword d of lane l is stored at d*64 + l, so every lane
+// always uses its own bank.  Counters change by LDS atomic adds whose result
+// is not returned (no dependency chain between updates); -1 on a byte is
+// +(2^32 - 2^(8j)), which cannot borrow because a count never goes below 0.
+struct Hist8Lane {
+  uint32_t* h;  // this lane's column of the block's counters
+  __device__ __forceinline__ void clear() const {
 #pragma unroll 8
-  for (int i = 0; i < 64; ++i) h[i * 64] = 0u;
-  const TS* __restrict__ src = static_cast<const TS*>(p.src);
-  const int ex = EX > 0 ? EX : p.ex;
-  const int ey = p.ey, ez = p.ez;
-  const int W = (int)p.W;
-  const int xi = valid ? (int)x : W - 1;  // lanes past the row end act as its last pixel
-  // the whole block's x window inside the row: no clamp, offsets from x
-  const bool interior = (int64_t)bx * 64 - ex >= 0 && (int64_t)bx * 64 + 63 + ex < p.W;
-  // row pointer of a (z, y) row: uniform across the wave
-  auto rowp = [&](int64_t zz, int64_t yy) -> const TS* {
-    zz = clamp64(zz, 0, p.D - 1);
-    yy = clamp64(yy, 0, p.H - 1);
-    return src + (zz * p.H + yy) * p.W;
-  };
-  // bin of a source value: for the same 8-bit type the value itself
-  auto bin_of = [&](TS v) -> int {
-    if constexpr (std::is_same<TS, TD>::value) return (int)v - static_cast<int>(Pix<TD>::lo);
-    else return bin8<TD>(gather_value<TS, TD>(v, p.wrap));
-  };
-  auto bin_at = [&](const TS* row, int dx) -> int {
-    if (interior) return bin_of(row[xi + dx]);
-    int xx = xi + dx;
-    xx = xx < 0 ? 0 : (xx >= W ? W - 1 : xx);
-    return bin_of(row[xx]);
-  };
-  // bins of the 2*EX+1 window positions x-EX .. x+EX of one row
-  auto row_bins = [&](const TS* row, int (&b)[2 * EX + 1]) {
-    b[EX] = bin_of(row[xi]);
-    int edge[EX > 0 ? EX : 1];
-    if (lane == 0 || lane == 63) {
-      // lane 0: x-1-k, lane 63: x+1+k (clamped into the row)
-      const int step = lane == 0 ? -1 : 1;
-#pragma unroll
-      for (int k = 0; k < EX; ++k) {
-        int xx = xi + step * (k + 1);
-        xx = xx < 0 ? 0 : (xx >= W ? W - 1 : xx);
-        edge[k] = bin_of(row[xx]);
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < EX; ++k) {
-      b[EX - 1 - k] = (int)dpp_from_left((uint32_t)edge[k], (uint32_t)b[EX - k]);
-      b[EX + 1 + k] = (int)dpp_from_right((uint32_t)edge[k], (uint32_t)b[EX + k]);
-    }
-  };
-  auto word = [&](int b) -> uint32_t* { return h + (b >> 2) * 64; };
-  auto add = [&](int b, bool inc) {
+    for (int i = 0; i < 64; ++i) h[i * 64] = 0u;
+  }
+  __device__ __forceinline__ uint32_t* word(int b) const { return h + (b >> 2) * 64; }
+  __device__ __forceinline__ void add(int b, bool inc) const {
     const uint32_t one = 1u << ((b & 3) * 8);
     __hip_atomic_fetch_add(word(b), inc ? one : 0u - one, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-  };
-  auto count = [&](int b) -> int { return (int)((*word(b) >> ((b & 3) * 8)) & 255u); };
-  const int k = p.n / 2;
-  for (int dz = -ez; dz <= ez; ++dz)
-    for (int dy = -ey; dy <= ey; ++dy) {
-      const TS* row = rowp(z0 + dz, y + dy);
-      if constexpr (EX > 0) {
-        int b[2 * EX + 1];
-        row_bins(row, b);
-#pragma unroll
-        for (int i = 0; i <= 2 * EX; ++i) add(b[i], true);
-      } else {
-        for (int dx = -ex; dx <= ex; ++dx) add(bin_at(row, dx), true);
-      }
-    }
-  int med = 0, lt = 0;
-  for (int c = count(0); lt + c <= k; c = count(med)) {
-    lt += c;
-    ++med;
   }
-  TD* __restrict__ dst = static_cast<TD*>(p.dst);
-  const int64_t plane = p.H * p.W;
-  if (valid) dst[z0 * plane + y * p.W + x] = static_cast<TD>(med + static_cast<int>(Pix<TD>::lo));
-  for (int64_t z = z0 + 1; z < z1; ++z) {
-    const int64_t zo = z - 1 - ez, zi = z + ez;
-    for (int dy = -ey; dy <= ey; ++dy) {
-      const TS* ro = rowp(zo, y + dy);
-      const TS* ri = rowp(zi, y + dy);
-      if constexpr (EX > 0) {
-        int a[2 * EX + 1], b[2 * EX + 1];
-        row_bins(ro, a);
-        row_bins(ri, b);
-#pragma unroll
-        for (int i = 0; i <= 2 * EX; ++i) {
-          add(a[i], false);
-          add(b[i], true);
-          lt += (b[i] < med) - (a[i] < med);
-        }
-      } else {
-        for (int dx = -ex; dx <= ex; ++dx) {
-          const int a = bin_at(ro, dx);
-          const int b = bin_at(ri, dx);
-          add(a, false);
-          add(b, true);
-          lt += (b < med) - (a < med);
-        }
-      }
-    }
-    // median = the bin m with lt(m) <= k < lt(m) + count(m)
+  __device__ __forceinline__ int count(int b) const { return (int)((*word(b) >> ((b & 3) * 8)) & 255u); }
+  // median = the bin m with lt(m) <= k < lt(m) + count(m); lt = number of
+  // values below med on entry and on return
+  __device__ __forceinline__ void walk(int& med, int& lt, int k) const {
     while (lt > k) {
       --med;
       lt -= count(med);
@@ -370,7 +269,198 @@ __device__ __forceinline__ void median_hist8_body(const MedParams& p, uint32_t b
       lt += c;
       ++med;
     }
+  }
+};
+
+// bin of a source value for an 8-bit dst: for the same type the value itself
+template <class TS, class TD>
+__device__ __forceinline__ int hist8_bin(TS v, int wrap) {
+  if constexpr (std::is_same<TS, TD>::value) return (int)v - static_cast<int>(Pix<TD>::lo);
+  else return bin8<TD>(gather_value<TS, TD>(v, wrap));
+}
+
+// Any window (half widths at run time): each lane loads every window
+// position itself.
+template <class TS, class TD>
+__device__ __forceinline__ void median_hist8_body(const MedParams& p, uint32_t bx, uint32_t by, uint32_t bz) {
+  __shared__ uint32_t hist[64 * 64];
+  const int lane = threadIdx.x;
+  const int64_t x = (int64_t)bx * 64 + lane;
+  const int64_t y = by;
+  const int64_t z0 = (int64_t)bz * kHistZSeg;
+  const int64_t z1 = (z0 + kHistZSeg < p.D) ? z0 + kHistZSeg : p.D;
+  if (x >= p.W) return;  // lanes never share data: no barrier follows
+  const Hist8Lane hl{hist + lane};
+  hl.clear();
+  const TS* __restrict__ src = static_cast<const TS*>(p.src);
+  const int ex = p.ex, ey = p.ey, ez = p.ez;
+  const int W = (int)p.W;
+  const int xi = (int)x;
+  // the whole block's x window inside the row: no clamp, offsets from x
+  const bool interior = (int64_t)bx * 64 - ex >= 0 && (int64_t)bx * 64 + 63 + ex < p.W;
+  // row pointer of a (z, y) row: uniform across the wave
+  auto rowp = [&](int64_t zz, int64_t yy) -> const TS* {
+    zz = clamp64(zz, 0, p.D - 1);
+    yy = clamp64(yy, 0, p.H - 1);
+    return src + (zz * p.H + yy) * p.W;
+  };
+  auto bin_at = [&](const TS* row, int dx) -> int {
+    int xx = xi + dx;
+    if (!interior) xx = xx < 0 ? 0 : (xx >= W ? W - 1 : xx);
+    return hist8_bin<TS, TD>(row[xx], p.wrap);
+  };
+  const int k = p.n / 2;
+  for (int dz = -ez; dz <= ez; ++dz)
+    for (int dy = -ey; dy <= ey; ++dy) {
+      const TS* row = rowp(z0 + dz, y + dy);
+      for (int dx = -ex; dx <= ex; ++dx) hl.add(bin_at(row, dx), true);
+    }
+  int med = 0, lt = 0;
+  hl.walk(med, lt, k);
+  TD* __restrict__ dst = static_cast<TD*>(p.dst);
+  const int64_t plane = p.H * p.W;
+  dst[z0 * plane + y * p.W + x] = static_cast<TD>(med + static_cast<int>(Pix<TD>::lo));
+  for (int64_t z = z0 + 1; z < z1; ++z) {
+    const int64_t zo = z - 1 - ez, zi = z + ez;
+    for (int dy = -ey; dy <= ey; ++dy) {
+      const TS* ro = rowp(zo, y + dy);
+      const TS* ri = rowp(zi, y + dy);
+      for (int dx = -ex; dx <= ex; ++dx) {
+        const int a = bin_at(ro, dx);
+        const int b = bin_at(ri, dx);
+        hl.add(a, false);
+        hl.add(b, true);
+        lt += (b < med) - (a < med);
+      }
+    }
+    hl.walk(med, lt, k);
+    dst[z * plane + y * p.W + x] = static_cast<TD>(med + static_cast<int>(Pix<TD>::lo));
+  }
+}
+
+// x and y half widths fixed at compile time (z at run time).  Each lane
+// loads one value per window row; the bins of its x neighbours come from the
+// adjacent lanes by DPP wave shifts, lanes 0 and 63 loading the clamped halo
+// (x-1-k, resp. x+1+k).  The rows a z step needs (the plane leaving and the
+// plane entering the window) are loaded one step ahead into the other of two
+// register buffers, so their latency overlaps the previous step's counter
+// updates and median walk.  Lanes past the row end act as its last pixel
+// (the clamp) and store nothing.  Bins travel as keys b << 6: the byte
+// offset of b's counter dword is then key & 0x3f00, the byte shift inside it
+// (key >> 3) & 31, and keys order like bins.
+template <class TS, class TD, int EX, int EY>
+__device__ __forceinline__ void median_hist8_fixed_body(const MedParams& p, uint32_t bx, uint32_t by, uint32_t bz) {
+  constexpr int NX = 2 * EX + 1, NY = 2 * EY + 1;
+  // integer sources stay in registers extended to int by the load
+  using Raw = typename std::conditional<std::is_integral<TS>::value, int32_t, TS>::type;
+  __shared__ uint32_t hist[64 * 64];
+  const int lane = threadIdx.x;
+  const int64_t x = (int64_t)bx * 64 + lane;
+  const int64_t y = by;
+  const int64_t z0 = (int64_t)bz * kHistZSeg;
+  const int64_t z1 = (z0 + kHistZSeg < p.D) ? z0 + kHistZSeg : p.D;
+  const bool valid = x < p.W;
+  const Hist8Lane hl{hist + lane};
+  hl.clear();
+  char* const hbase = reinterpret_cast<char*>(hist + lane);
+  auto add_key = [&](int key, bool inc) {
+    const uint32_t one = 1u << ((key >> 3) & 31);
+    __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(hbase + (key & 0x3f00)), inc ? one : 0u - one,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  };
+  auto key_of = [&](Raw r) -> int {
+    if constexpr (std::is_same<TS, TD>::value && std::is_integral<TS>::value)
+      return (r - static_cast<int>(Pix<TD>::lo)) << 6;
+    else return hist8_bin<TS, TD>(static_cast<TS>(r), p.wrap) << 6;
+  };
+  const TS* __restrict__ src = static_cast<const TS*>(p.src);
+  const int W = (int)p.W;
+  const int ez = p.ez;
+  const int xi = valid ? (int)x : W - 1;
+  const bool edge_lane = lane == 0 || lane == 63;
+  int hx[EX];  // halo columns of lanes 0 and 63, clamped into the row
+#pragma unroll
+  for (int k = 0; k < EX; ++k) {
+    const int xx = xi + (lane == 0 ? -(k + 1) : k + 1);
+    hx[k] = xx < 0 ? 0 : (xx >= W ? W - 1 : xx);
+  }
+  int64_t rows[NY];  // element offsets of the window rows within a plane
+#pragma unroll
+  for (int j = 0; j < NY; ++j) rows[j] = clamp64(y + j - EY, 0, p.H - 1) * p.W;
+  struct Plane {
+    Raw own[NY];
+    Raw halo[NY][EX];
+  };
+  auto load = [&](int64_t zz, Plane& r) {
+    const TS* pl = src + clamp64(zz, 0, p.D - 1) * p.H * p.W;
+#pragma unroll
+    for (int j = 0; j < NY; ++j) r.own[j] = static_cast<Raw>(pl[rows[j] + xi]);
+#pragma unroll
+    for (int j = 0; j < NY; ++j)
+#pragma unroll
+      for (int k = 0; k < EX; ++k) r.halo[j][k] = Raw(0);
+    if (edge_lane) {
+#pragma unroll
+      for (int j = 0; j < NY; ++j)
+#pragma unroll
+        for (int k = 0; k < EX; ++k) r.halo[j][k] = static_cast<Raw>(pl[rows[j] + hx[k]]);
+    }
+  };
+  // keys of window row j at x-EX .. x+EX
+  auto keys = [&](const Plane& r, int j, int (&b)[NX]) {
+    b[EX] = key_of(r.own[j]);
+#pragma unroll
+    for (int k = 0; k < EX; ++k) {
+      const uint32_t e = (uint32_t)key_of(r.halo[j][k]);
+      b[EX - 1 - k] = (int)dpp_from_left(e, (uint32_t)b[EX - k]);
+      b[EX + 1 + k] = (int)dpp_from_right(e, (uint32_t)b[EX + k]);
+    }
+  };
+  const int k = p.n / 2;
+  for (int dz = -ez; dz <= ez; ++dz) {
+    Plane r;
+    load(z0 + dz, r);
+#pragma unroll
+    for (int j = 0; j < NY; ++j) {
+      int b[NX];
+      keys(r, j, b);
+#pragma unroll
+      for (int i = 0; i < NX; ++i) add_key(b[i], true);
+    }
+  }
+  int med = 0, lt = 0;
+  hl.walk(med, lt, k);
+  TD* __restrict__ dst = static_cast<TD*>(p.dst);
+  const int64_t plane = p.H * p.W;
+  if (valid) dst[z0 * plane + y * p.W + x] = static_cast<TD>(med + static_cast<int>(Pix<TD>::lo));
+  // step z removes plane z-1-ez and adds plane z+ez, and loads the planes of
+  // step z+1; loads past the volume clamp, so the prefetch after the last
+  // step stays in bounds
+  auto step = [&](int64_t z, const Plane& co, const Plane& ci, Plane& no, Plane& ni) {
+    load(z - ez, no);
+    load(z + 1 + ez, ni);
+    const int mkey = med << 6;
+#pragma unroll
+    for (int j = 0; j < NY; ++j) {
+      int a[NX], b[NX];
+      keys(co, j, a);
+      keys(ci, j, b);
+#pragma unroll
+      for (int i = 0; i < NX; ++i) {
+        add_key(a[i], false);
+        add_key(b[i], true);
+        lt += (b[i] < mkey) - (a[i] < mkey);
+      }
+    }
+    hl.walk(med, lt, k);
     if (valid) dst[z * plane + y * p.W + x] = static_cast<TD>(med + static_cast<int>(Pix<TD>::lo));
+  };
+  Plane ao, ai, bo, bi;
+  load(z0 - ez, ao);
+  load(z0 + 1 + ez, ai);
+  for (int64_t z = z0 + 1; z < z1; z += 2) {
+    step(z, ao, ai, bo, bi);
+    if (z + 1 < z1) step(z + 1, bo, bi, ao, ai);
   }
 }
 
@@ -384,9 +474,14 @@ CLK_MEDIAN_KERNEL(median_27_pk8, 256)
 CLK_MEDIAN_KERNEL(median_27, 256)
 #undef CLK_MEDIAN_KERNEL
 
-template <class TS, class TD, int EX>
+template <class TS, class TD>
 __global__ __launch_bounds__(64) void median_hist8_kernel(const MedParams p) {
-  median_hist8_body<TS, TD, EX>(p, blockIdx.x, blockIdx.y, blockIdx.z);
+  median_hist8_body<TS, TD>(p, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+template <class TS, class TD, int EX, int EY>
+__global__ __launch_bounds__(64) void median_hist8_fixed_kernel(const MedParams p) {
+  median_hist8_fixed_body<TS, TD, EX, EY>(p, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 template <class TS, class TD>

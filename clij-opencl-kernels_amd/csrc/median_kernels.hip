@@ -18,10 +18,19 @@ static int32_t median_typed(hipStream_t st, MedParams& p, std::string* err) {
     case MED_PK8: hipLaunchKernelGGL((median_27_pk8_kernel<TS, TD>), grid, dim3(threads), 0, st, p); break;
     case MED_PK9: hipLaunchKernelGGL((median_9_pk8_kernel<TS, TD>), grid, dim3(threads), 0, st, p); break;
     case MED_HIST8:
-      if (p.ex == 1) hipLaunchKernelGGL((median_hist8_kernel<TS, TD, 1>), grid, dim3(threads), 0, st, p);
-      else if (p.ex == 2) hipLaunchKernelGGL((median_hist8_kernel<TS, TD, 2>), grid, dim3(threads), 0, st, p);
-      else if (p.ex == 3) hipLaunchKernelGGL((median_hist8_kernel<TS, TD, 3>), grid, dim3(threads), 0, st, p);
-      else hipLaunchKernelGGL((median_hist8_kernel<TS, TD, 0>), grid, dim3(threads), 0, st, p);
+      // planned for 8-bit dst only
+      if constexpr (sizeof(TD) == 1) {
+#define CLK_HIST8_FIXED(EX, EY)                                                                    \
+  if (p.ex == EX && p.ey == EY) {                                                                  \
+    hipLaunchKernelGGL((median_hist8_fixed_kernel<TS, TD, EX, EY>), grid, dim3(threads), 0, st, p); \
+    break;                                                                                         \
+  }
+      CLK_HIST8_FIXED(1, 1) CLK_HIST8_FIXED(1, 2) CLK_HIST8_FIXED(1, 3)
+      CLK_HIST8_FIXED(2, 1) CLK_HIST8_FIXED(2, 2) CLK_HIST8_FIXED(2, 3)
+      CLK_HIST8_FIXED(3, 1) CLK_HIST8_FIXED(3, 2) CLK_HIST8_FIXED(3, 3)
+#undef CLK_HIST8_FIXED
+        hipLaunchKernelGGL((median_hist8_kernel<TS, TD>), grid, dim3(threads), 0, st, p);
+      }
       break;
     case MED_27: hipLaunchKernelGGL((median_27_kernel<TS, TD>), grid, dim3(threads), 0, st, p); break;
     default: hipLaunchKernelGGL((median_generic_kernel<TS, TD>), grid, dim3(threads), 0, st, p); break;
