@@ -328,7 +328,7 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
       } else if (axis_is_x && p.c <= kMaxCX) {
         key.kind = K_X;
         const int64_t W = p.L;
-        p.nseg = (int32_t)((W + kXSeg - 1) / kXSeg);
+        set_nseg(p, (W + kXSeg - 1) / kXSeg);
         p.ntasks = p.outer * p.nseg;
         p.lds_stride = lds_pad(kXSeg + 2 * p.c) + 1;
         const int ss = clk_dtype_size(ps.src.dtype), ds = clk_dtype_size(ps.dst.dtype);
@@ -349,7 +349,7 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
         if (seg > (p.L + 7) / 8 * 8) seg = (p.L + 7) / 8 * 8;
         const int64_t nseg = (p.L + seg - 1) / seg;
         p.seg = (int32_t)seg;
-        p.nseg = (int32_t)nseg;
+        set_nseg(p, nseg);
         p.nblk = (uint32_t)nblk;
         blocks = (uint32_t)(nblk * nseg);
       }

@@ -38,8 +38,22 @@ struct SepParams {
   int32_t fastdiv;     // numerators never need the guarded divide
   int32_t order;       // strided pass block order: bit 0 = segments of a column
                        // block adjacent, bit 1 = contiguous ranges per XCD
+  uint32_t nseg_m;     // n / nseg = (n * nseg_m) >> (31 + nseg_l) for n < 2^31
+  int32_t nseg_l;
   float w[kMaxTaps];   // Gaussian weights in tap order v = -c..c
 };
+
+// Sets p.nseg and the constants of an exact division by it: with
+// l = ceil(log2 nseg) and m = ceil(2^(31+l) / nseg) (m < 2^32 for
+// nseg < 2^31), floor(n * m / 2^(31+l)) = floor(n / nseg) for every
+// 0 <= n < 2^31 (Granlund & Montgomery, PLDI 1994, Thm 4.2 with N = 31).
+inline void set_nseg(SepParams& p, int64_t nseg) {
+  p.nseg = (int32_t)nseg;
+  int l = 0;
+  while ((int64_t(1) << l) < nseg) ++l;
+  p.nseg_l = l;
+  p.nseg_m = (uint32_t)(((uint64_t(1) << (31 + l)) + (uint64_t)nseg - 1) / (uint64_t)nseg);
+}
 
 struct Status {
   int32_t code = 0;

@@ -49,6 +49,12 @@
 
 namespace clk {
 
+// n / p.nseg for n < 2^31 by the multiplier set_nseg computed; for a
+// wave-uniform n this stays on the scalar unit
+__device__ __forceinline__ uint32_t div_nseg(const SepParams& p, uint32_t n) {
+  return (uint32_t)(((uint64_t)n * p.nseg_m) >> (31 + p.nseg_l));
+}
+
 template <int OP, class TS>
 using vt_t = std::conditional_t<((OP == OP_MIN || OP == OP_MAX) && Pix<TS>::kInt), int32_t, float>;
 
@@ -291,7 +297,7 @@ __device__ __forceinline__ void sep_x_body(const SepParams& p, const uint32_t vb
   const int ntaps = p.ntaps;
   const uint32_t task = vblk * 4u + (uint32_t)wv;
   const bool valid = task < (uint32_t)p.ntasks;
-  const uint32_t row = task / (uint32_t)p.nseg;
+  const uint32_t row = div_nseg(p, task);
   const int64_t x0 = (int64_t)(task - row * (uint32_t)p.nseg) * TSEG;
   const TS* __restrict__ rp = static_cast<const TS*>(p.src) + (int64_t)row * W;
 
@@ -467,7 +473,7 @@ inline void plan_segments(SepParams& p, int64_t nblk, int64_t target_blocks) {
   seg = (seg + K - 1) / K * K;
   nseg = (p.L + seg - 1) / seg;
   p.seg = (int32_t)seg;
-  p.nseg = (int32_t)nseg;
+  set_nseg(p, nseg);
   p.nblk = (uint32_t)nblk;
 }
 
@@ -667,7 +673,7 @@ int32_t launch_sep_typed(hipStream_t st, SepParams& p, int axis_is_x, std::strin
     int sub = tuning(TUNE_X_SUB);
     if (sub != 1 && sub != 2 && sub != 4) sub = (W >= 2 * kXSeg && p.c <= 256) ? 2 : 1;
     const int tseg = kXSeg * sub;
-    p.nseg = (int32_t)((W + tseg - 1) / tseg);
+    set_nseg(p, (W + tseg - 1) / tseg);
     p.ntasks = p.outer * p.nseg;
     if (p.ntasks > 0x7ffffff0LL) {
       *err = "volume too large for the x pass grid";

@@ -30,7 +30,7 @@ __device__ __forceinline__ void xs_load(const SepParams& p, const uint32_t task,
   const int64_t W = p.L;
   t.live = task < (uint32_t)p.ntasks;  // wave-uniform
   if (!t.live) return;
-  t.row = task / (uint32_t)p.nseg;
+  t.row = div_nseg(p, task);
   t.x0 = (int64_t)(task - t.row * (uint32_t)p.nseg) * TSEG;
   t.full = p.vec_ld && t.x0 + TSEG <= W;
   const TS* __restrict__ rp = static_cast<const TS*>(p.src) + (int64_t)t.row * W;
@@ -75,7 +75,12 @@ __device__ __forceinline__ void xs_compute(const SepParams& p, const int lane, c
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
   for (int m = 0; m < SUB; ++m) {
-    const VT* const lb = buf + 9 * (64 * m + lane);
+    // chunk m > 0 lies beyond the reach of ds_read2's 8-bit word offsets
+    // from the m = 0 address: form its base once (the empty asm keeps the
+    // compiler from folding it back into one add per read)
+    uint32_t off = 9u * (64u * m + (uint32_t)lane);
+    if (m > 0) asm volatile("" : "+v"(off));
+    const VT* const lb = buf + off;
     VT win[WIN];
 #pragma unroll
     for (int i = 0; i < WIN; ++i) win[i] = lb[lds_pad(i)];

@@ -29,7 +29,7 @@ __global__ __launch_bounds__(256) void sep_ystatic_kernel(const SepParams p) {
   }
   uint32_t sid, b;
   if (p.order & 1) {
-    b = bid / (uint32_t)p.nseg;
+    b = div_nseg(p, bid);
     sid = bid - b * (uint32_t)p.nseg;
   } else {
     sid = bid / p.nblk;
