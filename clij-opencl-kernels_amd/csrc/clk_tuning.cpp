@@ -84,6 +84,7 @@ extern "C" int32_t clk_set_tuning(const char* key, int32_t value) {
   if (k == clk::TUNE_STRIDED_ORDER && (value < 0 || value > 3)) return CLK_INVALID_VALUE;
   if (k == clk::TUNE_AUTOTUNE && value != 0 && value != 1) return CLK_INVALID_VALUE;
   if (k == clk::TUNE_X_TASKS && (value < 0 || value > 2)) return CLK_INVALID_VALUE;
+  if (k == clk::TUNE_SPHERE_DIRECT && (value < 0 || value > 2)) return CLK_INVALID_VALUE;
   if (k == clk::TUNE_XREG_UNITS && value != 0 && value != 1 && value != 2 && value != 4)
     return CLK_INVALID_VALUE;
   clk::g_val[k].store(value);

@@ -20,7 +20,7 @@ enum TuneKey : int {
   TUNE_XREG_UNITS = 10,      // 0 = auto; 1, 2 or 4 units of 512 outputs per wave in the register x pass
   TUNE_X_DYNAMIC = 11,       // 1 = LDS x pass always takes the half width at run time
   TUNE_X_TASKS = 12,         // 0/1 = one task per wave in the static LDS x pass, 2 = two (prefetched)
-  TUNE_SPHERE_DIRECT = 13,   // 1 = ellipsoid filters always gather per voxel (no LDS tiles)
+  TUNE_SPHERE_DIRECT = 13,   // ellipsoid filters: 0 = auto, 1 = per-voxel gathers, 2 = LDS row tiles (no z sliding)
   TUNE_NO_XPACK = 14,        // 1 = short rows take one row per wave in the LDS x pass
   TUNE_COUNT = 15
 };

@@ -47,7 +47,7 @@ int32_t sphere_dst(int op, hipStream_t st, const SphereParams& p, const SphereTi
 // empty or too large for it.
 bool plan_tile(const SphereParams& p, SphereTileParams* tp) {
   if (p.ex < 0 || p.ey < 0 || p.ez < 0 || p.ex > 100 || p.n == 0 || p.n > kSphereTileMaxOff ||
-      tuning(TUNE_SPHERE_DIRECT))
+      tuning(TUNE_SPHERE_DIRECT) == 1)
     return false;
   memset(tp, 0, sizeof(*tp));
   tp->src = p.src;
@@ -124,16 +124,20 @@ int32_t launch_sphere(int op, hipStream_t st, const clk_image& dst, const clk_im
       for (int y = -p.ey; y <= p.ey; ++y)
         for (int z = -p.ez; z <= p.ez; ++z)
           if (p.tx[x + p.ex] + p.ty[y + p.ey] + p.tz[z + p.ez] <= 1.0f) ++p.n;
-    SphereTileParams* tp = new (std::nothrow) SphereTileParams;
-    const SphereTileParams* use = (tp && plan_tile(p, tp)) ? tp : nullptr;
-    switch (src.dtype) {
-      case CLK_CHAR: rc = sphere_dst<int8_t>(op, st, p, use, dst.dtype); break;
-      case CLK_UCHAR: rc = sphere_dst<uint8_t>(op, st, p, use, dst.dtype); break;
-      case CLK_SHORT: rc = sphere_dst<int16_t>(op, st, p, use, dst.dtype); break;
-      case CLK_USHORT: rc = sphere_dst<uint16_t>(op, st, p, use, dst.dtype); break;
-      default: rc = sphere_dst<float>(op, st, p, use, dst.dtype); break;
+    // the sliding form where the neighbourhood fits it, else LDS row tiles,
+    // else per-voxel gathers
+    if (!launch_sphere_slide(op, st, p, src.dtype, dst.dtype)) {
+      SphereTileParams* tp = new (std::nothrow) SphereTileParams;
+      const SphereTileParams* use = (tp && plan_tile(p, tp)) ? tp : nullptr;
+      switch (src.dtype) {
+        case CLK_CHAR: rc = sphere_dst<int8_t>(op, st, p, use, dst.dtype); break;
+        case CLK_UCHAR: rc = sphere_dst<uint8_t>(op, st, p, use, dst.dtype); break;
+        case CLK_SHORT: rc = sphere_dst<int16_t>(op, st, p, use, dst.dtype); break;
+        case CLK_USHORT: rc = sphere_dst<uint16_t>(op, st, p, use, dst.dtype); break;
+        default: rc = sphere_dst<float>(op, st, p, use, dst.dtype); break;
+      }
+      delete tp;
     }
-    delete tp;
     const hipError_t e = hipGetLastError();
     if (rc == 0 && e != hipSuccess) {
       *err = std::string("kernel launch failed: ") + hipGetErrorString(e);

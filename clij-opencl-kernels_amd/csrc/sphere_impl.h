@@ -158,4 +158,186 @@ __global__ __launch_bounds__(256) void sphere_tile_kernel(const SphereTileParams
   }
 }
 
+// Sliding form for neighbourhoods up to 9 wide in x and 7 in y: a block
+// filters 256 consecutive x of TY rows (y0 .. y0+TY-1) for a run of zs
+// planes.  LDS holds a ring of 2ez+2 source planes, each the rows
+// y0-ey .. y0+TY-1+ey by columns x0-ex .. x0+255+ex (clamped into the
+// volume) at a fixed row pitch of kSlideRL words.  Plane zz sits in slot
+// (zz - z0 + ez) mod (2ez+2).  A z step reads planes z-ez .. z+ez and loads
+// only plane z+ez+1: its loads are issued before the step's arithmetic and
+// written to LDS after it, into the one slot the step does not read (the
+// plane z-ez-1 it replaces); a block barrier then closes the step.  The
+// offset list (host-built, the reference's x, y, z loop order) gives each
+// offset one wave-uniform LDS base, which the TY outputs of a thread read at
+// immediate offsets.  Values are staged as floats (min/max/mean) or as
+// order-preserving keys of the dst-typed value (median, by bisection).
+// Launches the sliding form when the neighbourhood fits it (and the
+// sphere_direct knob is 0); false otherwise.  Host code, sphere_slide.hip.
+struct SphereParams;
+bool launch_sphere_slide(int op, hipStream_t st, const SphereParams& p, int src_t, int dst_t);
+
+constexpr int kSlideRL = 264;       // row pitch in words: 256 + 2*ex for ex <= 4
+constexpr int kSlideMaxEY = 3;
+constexpr int kSlideMaxLDS = 64 * 1024;
+
+struct SphereSlideParams {
+  const void* src;
+  void* dst;
+  int64_t W, H, D;
+  int32_t ex, ey, ez, noff;
+  int32_t zs;  // output planes per block
+  int32_t reserved;
+  uint32_t off[kSphereTileMaxOff];  // (dx + 128) | (dy + 128) << 8 | (dz + 128) << 16
+};
+
+template <class TS, class TD, int OP, int TY>
+__global__ __launch_bounds__(256) void sphere_slide_kernel(const SphereSlideParams p) {
+  constexpr int NMAX = TY + 2 * kSlideMaxEY;
+  constexpr int RL = kSlideRL;
+  extern __shared__ uint32_t ring[];
+  const int t = threadIdx.x;
+  const int64_t x0 = (int64_t)blockIdx.x * 256;
+  const int64_t y0 = (int64_t)blockIdx.y * TY;
+  const int64_t z0 = (int64_t)blockIdx.z * p.zs;
+  const int64_t z1 = (z0 + p.zs < p.D) ? z0 + p.zs : p.D;
+  const int ex = p.ex, ey = p.ey, ez = p.ez;
+  const int nrow = TY + 2 * ey;
+  const int S = 2 * ez + 2;
+  const int slot_words = nrow * RL;
+  const TS* __restrict__ src = static_cast<const TS*>(p.src);
+  // staging columns of this thread: c = t, and c = 256 + t for t < 2ex
+  const int64_t xa = clamp64(x0 - ex + t, 0, p.W - 1);
+  const bool two = t < 2 * ex;
+  const int64_t xb = clamp64(x0 - ex + 256 + t, 0, p.W - 1);
+  auto word = [&](TS v) -> uint32_t {
+    if constexpr (OP == SPH_MEDIAN) return value_key<TD>(gather_value<TS, TD>(v, 0));
+    else return __float_as_uint(static_cast<float>(v));
+  };
+  // raw values of one plane's rows, converted when written to LDS so that
+  // the loads stay in flight across a step's arithmetic
+  TS va[NMAX], vb[NMAX];
+  auto fetch = [&](int64_t zz) {
+    const TS* pl = src + clamp64(zz, 0, p.D - 1) * p.H * p.W;
+#pragma unroll
+    for (int r = 0; r < NMAX; ++r) {
+      if (r < nrow) {
+        const TS* row = pl + clamp64(y0 - ey + r, 0, p.H - 1) * p.W;
+        va[r] = row[xa];
+        vb[r] = two ? row[xb] : TS(0);
+      }
+    }
+  };
+  auto put = [&](int slot) {
+    uint32_t* s = ring + slot * slot_words + t;
+#pragma unroll
+    for (int r = 0; r < NMAX; ++r) {
+      if (r < nrow) {
+        s[r * RL] = word(va[r]);
+        if (two) s[r * RL + 256] = word(vb[r]);
+      }
+    }
+  };
+  for (int i = 0; i <= 2 * ez; ++i) {
+    fetch(z0 - ez + i);
+    put(i);
+  }
+  __syncthreads();
+  const int64_t x = x0 + t;
+  const uint32_t* const lane = ring + t + ex;  // this thread's output column
+  TD* const dst = static_cast<TD*>(p.dst);
+  int cs = 0;          // slot of plane z - ez
+  int ws = 2 * ez + 1; // slot the next plane goes to
+  for (int64_t z = z0; z < z1; ++z) {
+    const bool more = z + 1 < z1;
+    if (more) fetch(z + ez + 1);
+    // LDS word of window offset (dx, dy, dz) for output row 0
+    auto base = [&](uint32_t o) -> const uint32_t* {
+      const int dx = (int)(o & 255u) - 128, dy = (int)((o >> 8) & 255u) - 128, dz = (int)(o >> 16) - 128;
+      int sl = cs + dz + ez;
+      if (sl >= S) sl -= S;
+      return lane + sl * slot_words + (dy + ey) * RL + dx;
+    };
+    auto store = [&](int j, TD v) {
+      if (x < p.W && y0 + j < p.H) dst[(z * p.H + y0 + j) * p.W + x] = v;
+    };
+    // f(LDS word of row 0) for every offset in list order, the next offset
+    // fetched before the current one is used
+    auto walk = [&](auto&& f) {
+      uint32_t o = p.off[0];
+      for (int i = 0; i < p.noff; ++i) {
+        const uint32_t cur = o;
+        if (i + 1 < p.noff) o = p.off[i + 1];
+        f(base(cur));
+      }
+    };
+    if constexpr (OP == SPH_MEDIAN) {
+      // rank noff/2 of the keys (median_sphere_3d_x.cl:54-68) by bisection:
+      // each round halves [lo, hi], so KeyBits rounds leave lo == hi
+      const int k = p.noff / 2;
+      uint32_t lo[TY], hi[TY];
+#pragma unroll
+      for (int j = 0; j < TY; ++j) {
+        lo[j] = 0;
+        hi[j] = (KeyBits<TD>::v == 32) ? 0xffffffffu : ((1u << KeyBits<TD>::v) - 1u);
+      }
+      for (int round = 0; round < KeyBits<TD>::v; ++round) {
+        uint32_t mid[TY];
+        int cnt[TY];
+#pragma unroll
+        for (int j = 0; j < TY; ++j) {
+          mid[j] = lo[j] + ((hi[j] - lo[j]) >> 1);
+          cnt[j] = 0;
+        }
+        walk([&](const uint32_t* b) {
+#pragma unroll
+          for (int j = 0; j < TY; ++j) cnt[j] += b[j * RL] <= mid[j];
+        });
+#pragma unroll
+        for (int j = 0; j < TY; ++j) {
+          if (cnt[j] >= k + 1) hi[j] = mid[j];
+          else lo[j] = mid[j] + 1;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < TY; ++j) store(j, static_cast<TD>(key_value<TD>(lo[j])));
+    } else if constexpr (OP == SPH_MEAN) {
+      float sum[TY];
+#pragma unroll
+      for (int j = 0; j < TY; ++j) sum[j] = 0.f;  // mean_sphere_3d_x.cl:17
+      walk([&](const uint32_t* b) {
+#pragma unroll
+        for (int j = 0; j < TY; ++j) sum[j] = sum[j] + __uint_as_float(b[j * RL]);  // :44, loop order
+      });
+#pragma unroll
+      for (int j = 0; j < TY; ++j) store(j, convert_sat<TD>(sum[j] / static_cast<float>(p.noff)));
+    } else {
+      // the centre first (minimum_sphere_3d_x.cl:15), then every offset
+      float m[TY];
+      const uint32_t* c = base(128u | (128u << 8) | (128u << 16));
+#pragma unroll
+      for (int j = 0; j < TY; ++j) m[j] = __uint_as_float(c[j * RL]);
+      walk([&](const uint32_t* b) {
+#pragma unroll
+        for (int j = 0; j < TY; ++j) {
+          const float v = __uint_as_float(b[j * RL]);
+          if constexpr (Pix<TS>::kInt) {
+            // integer values: no NaN or -0, so the IEEE min/max is the same
+            m[j] = OP == SPH_MIN ? __builtin_fminf(v, m[j]) : __builtin_fmaxf(v, m[j]);
+          } else if (OP == SPH_MIN ? (v < m[j]) : (v > m[j])) {
+            m[j] = v;  // :44-46
+          }
+        }
+      });
+#pragma unroll
+      for (int j = 0; j < TY; ++j) store(j, convert_sat<TD>(m[j]));
+    }
+    if (more) {
+      put(ws);
+      ws = ws + 1 == S ? 0 : ws + 1;
+    }
+    cs = cs + 1 == S ? 0 : cs + 1;
+    __syncthreads();
+  }
+}
+
 }  // namespace clk

@@ -1,0 +1,96 @@
+// Host side of the sliding-window ellipsoid filters (sphere_slide_kernel in
+// sphere_impl.h): planning and the typed launches.
+#include <string.h>
+
+#include "clk_internal.h"
+#include "clk_tuning.h"
+#include "sphere_impl.h"
+
+namespace clk {
+
+namespace {
+
+template <class TS, class TD, int TY>
+void slide_op(int op, hipStream_t st, const SphereSlideParams& sp, dim3 grid, size_t shmem) {
+  switch (op) {
+    case SPH_MIN: hipLaunchKernelGGL((sphere_slide_kernel<TS, TD, SPH_MIN, TY>), grid, dim3(256), shmem, st, sp); break;
+    case SPH_MAX: hipLaunchKernelGGL((sphere_slide_kernel<TS, TD, SPH_MAX, TY>), grid, dim3(256), shmem, st, sp); break;
+    case SPH_MEAN: hipLaunchKernelGGL((sphere_slide_kernel<TS, TD, SPH_MEAN, TY>), grid, dim3(256), shmem, st, sp); break;
+    default: hipLaunchKernelGGL((sphere_slide_kernel<TS, TD, SPH_MEDIAN, TY>), grid, dim3(256), shmem, st, sp); break;
+  }
+}
+
+template <class TS, class TD>
+void slide_ty(int op, hipStream_t st, const SphereSlideParams& sp, int ty, dim3 grid, size_t shmem) {
+  if (ty == 8) slide_op<TS, TD, 8>(op, st, sp, grid, shmem);
+  else slide_op<TS, TD, 4>(op, st, sp, grid, shmem);
+}
+
+template <class TS>
+void slide_dst(int op, hipStream_t st, const SphereSlideParams& sp, int dst_t, int ty, dim3 grid, size_t shmem) {
+  switch (dst_t) {
+    case CLK_CHAR: slide_ty<TS, int8_t>(op, st, sp, ty, grid, shmem); break;
+    case CLK_UCHAR: slide_ty<TS, uint8_t>(op, st, sp, ty, grid, shmem); break;
+    case CLK_SHORT: slide_ty<TS, int16_t>(op, st, sp, ty, grid, shmem); break;
+    case CLK_USHORT: slide_ty<TS, uint16_t>(op, st, sp, ty, grid, shmem); break;
+    default: slide_ty<TS, float>(op, st, sp, ty, grid, shmem); break;
+  }
+}
+
+}  // namespace
+
+bool launch_sphere_slide(int op, hipStream_t st, const SphereParams& p, int src_t, int dst_t) {
+  if (tuning(TUNE_SPHERE_DIRECT) != 0) return false;
+  if (p.ex < 0 || p.ey < 0 || p.ez < 0 || 2 * p.ex > kSlideRL - 256 || p.ey > kSlideMaxEY || p.n == 0 ||
+      p.n > kSphereTileMaxOff)
+    return false;
+  int ty = 0;
+  size_t shmem = 0;
+  for (int c : {8, 4}) {
+    const size_t b = (size_t)(2 * p.ez + 2) * (size_t)(c + 2 * p.ey) * kSlideRL * 4;
+    if (b <= (size_t)kSlideMaxLDS) {
+      ty = c;
+      shmem = b;
+      break;
+    }
+  }
+  if (ty == 0) return false;
+  SphereSlideParams* spp = new (std::nothrow) SphereSlideParams;
+  if (!spp) return false;
+  SphereSlideParams& sp = *spp;
+  memset(&sp, 0, sizeof(sp));
+  sp.src = p.src;
+  sp.dst = p.dst;
+  sp.W = p.W;
+  sp.H = p.H;
+  sp.D = p.D;
+  sp.ex = p.ex;
+  sp.ey = p.ey;
+  sp.ez = p.ez;
+  for (int x = -p.ex; x <= p.ex; ++x)  // the reference's loop order
+    for (int y = -p.ey; y <= p.ey; ++y)
+      for (int z = -p.ez; z <= p.ez; ++z)
+        if (p.tx[x + p.ex] + p.ty[y + p.ey] + p.tz[z + p.ez] <= 1.0f)
+          sp.off[sp.noff++] = (uint32_t)(x + 128) | ((uint32_t)(y + 128) << 8) | ((uint32_t)(z + 128) << 16);
+  // about 2048 blocks, each sliding over at least 8 planes (or the depth)
+  const int64_t gx = (p.W + 255) / 256, gy = (p.H + ty - 1) / ty;
+  int64_t gz = (2048 + gx * gy - 1) / (gx * gy);
+  if (gz > p.D) gz = p.D;
+  int64_t zs = (p.D + gz - 1) / gz;
+  const int64_t zmin = p.D < 8 ? p.D : 8;
+  if (zs < zmin) zs = zmin;
+  gz = (p.D + zs - 1) / zs;
+  sp.zs = (int32_t)zs;
+  const dim3 grid((uint32_t)gx, (uint32_t)gy, (uint32_t)gz);
+  switch (src_t) {
+    case CLK_CHAR: slide_dst<int8_t>(op, st, sp, dst_t, ty, grid, shmem); break;
+    case CLK_UCHAR: slide_dst<uint8_t>(op, st, sp, dst_t, ty, grid, shmem); break;
+    case CLK_SHORT: slide_dst<int16_t>(op, st, sp, dst_t, ty, grid, shmem); break;
+    case CLK_USHORT: slide_dst<uint16_t>(op, st, sp, dst_t, ty, grid, shmem); break;
+    default: slide_dst<float>(op, st, sp, dst_t, ty, grid, shmem); break;
+  }
+  delete spp;
+  return true;
+}
+
+}  // namespace clk

@@ -275,7 +275,9 @@ int32_t clk_batch_filter_3d(void* stream, const clk_filter_job* jobs, int32_t nj
 /* Launch-shape knobs; results never depend on them.  Keys: "strided_rounds"
  * (y/z blocks per resident slot), "x_sub" (LDS x pass segments per wave),
  * "batch_seg", "no_xreg", "strided_variant", "no_hist8", "xreg_maxc",
- * "ystatic_cols", "strided_order", "xreg_units", "x_dynamic", and
+ * "ystatic_cols", "strided_order", "xreg_units", "x_dynamic", "x_tasks",
+ * "no_xpack", "sphere_direct" (ellipsoid filters: 0 = auto, 1 = per-voxel
+ * gathers, 2 = LDS row tiles without z sliding), and
  * "autotune": 1 = the first single-axis launch of each signature (pixel
  * types, axis, N, volume size; volumes of 2^22 voxels and more) times a set
  * of launch shapes on the caller's stream, synchronising it once, and later

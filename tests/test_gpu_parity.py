@@ -519,10 +519,12 @@ def test_copy_3d_unaligned_and_ragged(gpu):
     assert dst.cpu().numpy()[0] == 0
 
 
-@pytest.mark.parametrize("direct", [0, 1])
+@pytest.mark.parametrize("direct", [0, 1, 2])
 def test_sphere_tiled_and_direct_paths(gpu, direct):
-    """Both ellipsoid kernels (LDS row tiles; per-voxel gathers) against the
-    oracle, including a neighbourhood too large for the tiles (r = 7)."""
+    """The ellipsoid kernels (0: z-sliding LDS ring where the neighbourhood
+    fits it, else LDS row tiles; 1: per-voxel gathers; 2: LDS row tiles)
+    against the oracle, including a neighbourhood too large for the tiles
+    (r = 7)."""
     import clkmi
     a = random_volume((9, 12, 300), np.int16, seed=12 + direct)
     ts, si = to_device(a)
@@ -539,6 +541,29 @@ def test_sphere_tiled_and_direct_paths(gpu, direct):
                               "%s %s direct=%d" % (name, Ns, direct))
     finally:
         clkmi.set_tuning("sphere_direct", old)
+
+
+@pytest.mark.parametrize("src_t,dst_t", [(np.uint8, np.uint8), (np.float32, np.float32), (np.int16, np.uint16),
+                                         (np.uint16, np.float32)], ids=lambda t: DTYPE_NAMES[t])
+def test_sphere_sliding_ring(gpu, src_t, dst_t):
+    """The z-sliding form over volumes whose blocks each slide across several
+    planes (depth 37: ring wrap-around, a partial last run), with several x
+    tiles (width 300) and y tiles (height 21, not a multiple of the rows per
+    block), for balls, ellipsoids and flat shapes up to 9 x 7 x 7."""
+    import clkmi
+    a = random_volume((37, 21, 300), src_t, seed=31)
+    if src_t == np.float32:
+        a.reshape(-1)[::97] = -0.0
+        a.reshape(-1)[5::211] = np.nan
+    ts, si = to_device(a)
+    for name in SPHERE_OPS:
+        for Ns in ((3, 3, 3), (5, 5, 5), (9, 7, 3), (3, 5, 7), (1, 3, 1), (7, 1, 5)):
+            if name == "median_sphere_3d" and src_t == np.float32:
+                continue  # NaN ranks are order-dependent in the reference: not compared
+            td, di = empty_device(a.shape, dst_t)
+            getattr(clkmi, name)(di, si, *Ns)
+            assert_parity(from_device(td, a.shape, dst_t), O.sphere_3d(name, a, dst_t, *Ns),
+                          "%s %s %s->%s" % (name, Ns, DTYPE_NAMES[src_t], DTYPE_NAMES[dst_t]))
 
 
 @pytest.mark.parametrize("pack", [0, 1])
