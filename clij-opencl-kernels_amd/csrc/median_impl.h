@@ -258,6 +258,13 @@ struct Hist8Lane {
     __hip_atomic_fetch_add(word(b), inc ? one : 0u - one, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
   __device__ __forceinline__ int count(int b) const { return (int)((*word(b) >> ((b & 3) * 8)) & 255u); }
+  // the same update for a key b << 6: its counter dword sits key & 0x3f00
+  // bytes from h, the byte shift inside it is (key >> 3) & 31
+  __device__ __forceinline__ void add_key(int key, bool inc) const {
+    const uint32_t one = 1u << ((key >> 3) & 31);
+    __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(h) + (key & 0x3f00)),
+                           inc ? one : 0u - one, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
   // median = the bin m with lt(m) <= k < lt(m) + count(m); lt = number of
   // values below med on entry and on return
   __device__ __forceinline__ void walk(int& med, int& lt, int k) const {
@@ -362,12 +369,7 @@ __device__ __forceinline__ void median_hist8_fixed_body(const MedParams& p, uint
   const bool valid = x < p.W;
   const Hist8Lane hl{hist + lane};
   hl.clear();
-  char* const hbase = reinterpret_cast<char*>(hist + lane);
-  auto add_key = [&](int key, bool inc) {
-    const uint32_t one = 1u << ((key >> 3) & 31);
-    __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(hbase + (key & 0x3f00)), inc ? one : 0u - one,
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-  };
+  auto add_key = [&](int key, bool inc) { hl.add_key(key, inc); };
   auto key_of = [&](Raw r) -> int {
     if constexpr (std::is_same<TS, TD>::value && std::is_integral<TS>::value)
       return (r - static_cast<int>(Pix<TD>::lo)) << 6;

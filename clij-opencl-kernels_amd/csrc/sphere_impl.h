@@ -171,8 +171,10 @@ __global__ __launch_bounds__(256) void sphere_tile_kernel(const SphereTileParams
 // offset one wave-uniform LDS base, which the TY outputs of a thread read at
 // immediate offsets.  Values are staged as floats (min/max/mean) or as
 // order-preserving keys of the dst-typed value (median, by bisection).
-// Launches the sliding form when the neighbourhood fits it (and the
-// sphere_direct knob is 0); false otherwise.  Host code, sphere_slide.hip.
+// Launches a sliding form when the neighbourhood fits one (and the
+// sphere_direct knob is 0): the z-sliding byte histogram for medians with an
+// 8-bit dst, else the z-sliding LDS ring; false otherwise.  Host code,
+// sphere_slide.hip.
 struct SphereParams;
 bool launch_sphere_slide(int op, hipStream_t st, const SphereParams& p, int src_t, int dst_t);
 
@@ -337,6 +339,106 @@ __global__ __launch_bounds__(256) void sphere_slide_kernel(const SphereSlidePara
     }
     cs = cs + 1 == S ? 0 : cs + 1;
     __syncthreads();
+  }
+}
+
+// median_sphere_3d with an 8-bit dst and at most 255 values: the per-lane
+// byte histogram of the box median slid along z.  The ellipsoid is a set of
+// columns (dx, dy), each holding the offsets dz in [-h, h] (x^2/a^2 + y^2/b^2
+// + z^2/c^2 grows with |z| and float addition is monotone, so the set is
+// contiguous and symmetric); stepping from z to z+1 removes plane z-h and
+// adds plane z+1+h of every column.  A lane keeps the clamped in-plane
+// offsets of its columns in registers and loads each step's values one step
+// ahead.  The median is the rank n/2 of the dst-typed values, as the
+// reference's sort (median_sphere_3d_x.cl:54-68).
+constexpr int kSphereHistMaxCols = 32;
+
+struct SphereHistParams {
+  const void* src;
+  void* dst;
+  int64_t W, H, D;
+  int32_t ncol, n;
+  uint32_t col[kSphereHistMaxCols];  // (dx + 128) | (dy + 128) << 8 | h << 16
+};
+
+template <class TS, class TD, int NC>
+__global__ __launch_bounds__(64) void sphere_median_hist8_kernel(const SphereHistParams p) {
+  using Raw = typename std::conditional<std::is_integral<TS>::value, int32_t, TS>::type;
+  __shared__ uint32_t hist[64 * 64];
+  const int lane = threadIdx.x;
+  const int64_t x = (int64_t)blockIdx.x * 64 + lane;
+  const int64_t y = blockIdx.y;
+  const int64_t z0 = (int64_t)blockIdx.z * kHistZSeg;
+  const int64_t z1 = (z0 + kHistZSeg < p.D) ? z0 + kHistZSeg : p.D;
+  if (x >= p.W) return;  // lanes never share data: no barrier follows
+  const Hist8Lane hl{hist + lane};
+  hl.clear();
+  const TS* __restrict__ src = static_cast<const TS*>(p.src);
+  const int64_t PS = p.H * p.W;
+  int32_t off[NC];  // in-plane offset of column c at this lane (H*W < 2^31)
+  int hh[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    off[c] = 0;
+    hh[c] = 0;
+    if (c < p.ncol) {
+      const uint32_t o = p.col[c];
+      const int64_t xx = clamp64(x + (int)(o & 255u) - 128, 0, p.W - 1);
+      const int64_t yy = clamp64(y + (int)((o >> 8) & 255u) - 128, 0, p.H - 1);
+      off[c] = (int32_t)(yy * p.W + xx);
+      hh[c] = (int)(o >> 16);
+    }
+  }
+  auto at = [&](int c, int64_t zz) -> Raw { return static_cast<Raw>(src[clamp64(zz, 0, p.D - 1) * PS + off[c]]); };
+  auto key_of = [&](Raw r) -> int {
+    if constexpr (std::is_same<TS, TD>::value && std::is_integral<TS>::value)
+      return (r - static_cast<int>(Pix<TD>::lo)) << 6;
+    else return hist8_bin<TS, TD>(static_cast<TS>(r), 0) << 6;
+  };
+  const int k = p.n / 2;
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+    if (c < p.ncol)
+      for (int dz = -hh[c]; dz <= hh[c]; ++dz) hl.add_key(key_of(at(c, z0 + dz)), true);
+  int med = 0, lt = 0;
+  hl.walk(med, lt, k);
+  TD* __restrict__ dst = static_cast<TD*>(p.dst);
+  dst[z0 * PS + y * p.W + x] = static_cast<TD>(med + static_cast<int>(Pix<TD>::lo));
+  struct Step {
+    Raw out[NC], in[NC];
+  };
+  // step z removes plane z-1-h and adds plane z+h of each column; loads
+  // past the volume clamp, so the prefetch after the last step stays in
+  // bounds
+  auto load = [&](int64_t z, Step& s) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      if (c < p.ncol) {
+        s.out[c] = at(c, z - 1 - hh[c]);
+        s.in[c] = at(c, z + hh[c]);
+      }
+    }
+  };
+  auto step = [&](int64_t z, const Step& cur, Step& nxt) {
+    load(z + 1, nxt);
+    const int mkey = med << 6;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      if (c < p.ncol) {
+        const int a = key_of(cur.out[c]), b = key_of(cur.in[c]);
+        hl.add_key(a, false);
+        hl.add_key(b, true);
+        lt += (b < mkey) - (a < mkey);
+      }
+    }
+    hl.walk(med, lt, k);
+    dst[z * PS + y * p.W + x] = static_cast<TD>(med + static_cast<int>(Pix<TD>::lo));
+  };
+  Step sa, sb;
+  load(z0 + 1, sa);
+  for (int64_t z = z0 + 1; z < z1; z += 2) {
+    step(z, sa, sb);
+    if (z + 1 < z1) step(z + 1, sb, sa);
   }
 }
 
