@@ -189,8 +189,11 @@ struct SphereSlideParams {
   int32_t ex, ey, ez, noff;
   int32_t zs;  // output planes per block
   int32_t reserved;
-  uint32_t off[kSphereTileMaxOff];  // (dx + 128) | (dy + 128) << 8 | (dz + 128) << 16
+  // per offset: (dy + ey) * kSlideRL + dx + ex (its word within a ring slot,
+  // output row 0) | (dz + ez) << 16; padded with the centre to a multiple of 8
+  uint32_t off[kSphereTileMaxOff];
 };
+static_assert(kSphereTileMaxOff % 8 == 0, "offset chunks of 8");
 
 template <class TS, class TD, int OP, int TY>
 __global__ __launch_bounds__(256) void sphere_slide_kernel(const SphereSlideParams p) {
@@ -245,31 +248,46 @@ __global__ __launch_bounds__(256) void sphere_slide_kernel(const SphereSlidePara
   }
   __syncthreads();
   const int64_t x = x0 + t;
-  const uint32_t* const lane = ring + t + ex;  // this thread's output column
+  const uint32_t* const lane = ring + t;  // this thread's staging column
   TD* const dst = static_cast<TD*>(p.dst);
   int cs = 0;          // slot of plane z - ez
   int ws = 2 * ez + 1; // slot the next plane goes to
   for (int64_t z = z0; z < z1; ++z) {
     const bool more = z + 1 < z1;
     if (more) fetch(z + ez + 1);
-    // LDS word of window offset (dx, dy, dz) for output row 0
+    // LDS word of a window offset for output row 0: its ring slot follows
+    // from dz and the slot of plane z - ez
     auto base = [&](uint32_t o) -> const uint32_t* {
-      const int dx = (int)(o & 255u) - 128, dy = (int)((o >> 8) & 255u) - 128, dz = (int)(o >> 16) - 128;
-      int sl = cs + dz + ez;
+      int sl = cs + (int)(o >> 16);
       if (sl >= S) sl -= S;
-      return lane + sl * slot_words + (dy + ey) * RL + dx;
+      return lane + sl * slot_words + (int)(o & 0xffffu);
     };
     auto store = [&](int j, TD v) {
       if (x < p.W && y0 + j < p.H) dst[(z * p.H + y0 + j) * p.W + x] = v;
     };
-    // f(LDS word of row 0) for every offset in list order, the next offset
-    // fetched before the current one is used
+    // f(LDS word of row 0) for every offset in list order, eight offsets
+    // per scalar load so that a chunk's LDS reads are issued together.  The
+    // host pads the list to a multiple of 8 with the centre offset: min/max
+    // run the padded chunks whole (a second look at the centre changes
+    // nothing), mean/median skip the padding.
     auto walk = [&](auto&& f) {
-      uint32_t o = p.off[0];
-      for (int i = 0; i < p.noff; ++i) {
-        const uint32_t cur = o;
-        if (i + 1 < p.noff) o = p.off[i + 1];
-        f(base(cur));
+      constexpr bool kPadOk = OP == SPH_MIN || OP == SPH_MAX;
+      const int n8 = kPadOk ? (p.noff + 7) & ~7 : p.noff & ~7;
+      for (int i = 0; i < n8; i += 8) {
+        uint32_t o[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) o[u] = p.off[i + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) f(base(o[u]));
+      }
+      if (!kPadOk && n8 < p.noff) {
+        uint32_t o[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) o[u] = p.off[n8 + u];
+        const int rem = p.noff - n8;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (u < rem) f(base(o[u]));
       }
     };
     if constexpr (OP == SPH_MEDIAN) {
@@ -315,7 +333,7 @@ __global__ __launch_bounds__(256) void sphere_slide_kernel(const SphereSlidePara
     } else {
       // the centre first (minimum_sphere_3d_x.cl:15), then every offset
       float m[TY];
-      const uint32_t* c = base(128u | (128u << 8) | (128u << 16));
+      const uint32_t* c = base((uint32_t)(ey * RL + ex) | ((uint32_t)ez << 16));
 #pragma unroll
       for (int j = 0; j < TY; ++j) m[j] = __uint_as_float(c[j * RL]);
       walk([&](const uint32_t* b) {

@@ -131,7 +131,9 @@ bool launch_sphere_slide(int op, hipStream_t st, const SphereParams& p, int src_
     for (int y = -p.ey; y <= p.ey; ++y)
       for (int z = -p.ez; z <= p.ez; ++z)
         if (p.tx[x + p.ex] + p.ty[y + p.ey] + p.tz[z + p.ez] <= 1.0f)
-          sp.off[sp.noff++] = (uint32_t)(x + 128) | ((uint32_t)(y + 128) << 8) | ((uint32_t)(z + 128) << 16);
+          sp.off[sp.noff++] = (uint32_t)((y + p.ey) * kSlideRL + x + p.ex) | ((uint32_t)(z + p.ez) << 16);
+  for (int i = sp.noff; i % 8 != 0; ++i)  // the centre
+    sp.off[i] = (uint32_t)(p.ey * kSlideRL + p.ex) | ((uint32_t)p.ez << 16);
   // about 2048 blocks, each sliding over at least 8 planes (or the depth)
   const int64_t gx = (p.W + 255) / 256, gy = (p.H + ty - 1) / ty;
   int64_t gz = (2048 + gx * gy - 1) / (gx * gy);
