@@ -3,6 +3,8 @@
 // (kernels/maximum_z_projection_x.cl:2-20).
 #include <string.h>
 
+#include <type_traits>
+
 #include "clk_device.h"
 #include "clk_internal.h"
 
@@ -48,6 +50,88 @@ __global__ __launch_bounds__(256) void detect_maxima_kernel(const TS* __restrict
   dst[(k * H + j) * W + i] = static_cast<TD>(best == 13 ? 1 : 0);  // :45-50
 }
 
+// Integer sources: the values and centre - 1 are exact floats and there is
+// no NaN, so "the first strict maximum in loop order is the centre"
+// (offset 13) holds exactly when every offset before it in loop order holds
+// less than the centre and none after it holds more (an earlier value >= c
+// would update first and block the centre's update; a later one > c would
+// update after it).  Before the centre come the plane x-1, then (x, y-1, *),
+// then (x, y, z-1).  A wave takes 64 consecutive x of one row y and slides
+// along a run of z: each lane loads one value per row y-1, y, y+1 of a plane
+// and takes its x neighbours from lanes l±1 by DPP (lanes 0 and 63 load the
+// clamped halo); the next plane is loaded one step ahead.  Lanes past the
+// row end act as its last pixel (the clamp) and store nothing.
+constexpr int kDetectZSeg = 32;
+
+template <class TS, class TD>
+__global__ __launch_bounds__(64) void detect_maxima_int_kernel(const TS* __restrict__ src, TD* __restrict__ dst,
+                                                               int64_t W, int64_t H, int64_t D) {
+  const int lane = threadIdx.x;
+  const int64_t x = (int64_t)blockIdx.x * 64 + lane;
+  const int64_t y = blockIdx.y;
+  const int64_t z0 = (int64_t)blockIdx.z * kDetectZSeg;
+  const int64_t z1 = (z0 + kDetectZSeg < D) ? z0 + kDetectZSeg : D;
+  const bool valid = x < W;
+  const int64_t xi = valid ? x : W - 1;
+  const bool edge = lane == 0 || lane == 63;
+  const int64_t xh = clamp64(xi + (lane == 0 ? -1 : 1), 0, W - 1);  // halo column of lanes 0 and 63
+  const int64_t PS = H * W;
+  int64_t rows[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) rows[r] = clamp64(y + r - 1, 0, H - 1) * W;
+  struct Raw {
+    int own[3], halo[3];
+  };
+  struct Plane {
+    int v[3][3];  // [dy + 1][dx + 1]
+  };
+  auto load = [&](int64_t zz, Raw& R) {
+    const TS* pl = src + clamp64(zz, 0, D - 1) * PS;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      R.own[r] = (int)pl[rows[r] + xi];
+      R.halo[r] = 0;
+    }
+    if (edge) {
+#pragma unroll
+      for (int r = 0; r < 3; ++r) R.halo[r] = (int)pl[rows[r] + xh];
+    }
+  };
+  auto expand = [&](const Raw& R, Plane& P) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      P.v[r][1] = R.own[r];
+      P.v[r][0] = (int)dpp_from_left((uint32_t)R.halo[r], (uint32_t)R.own[r]);
+      P.v[r][2] = (int)dpp_from_right((uint32_t)R.halo[r], (uint32_t)R.own[r]);
+    }
+  };
+  auto mx = [](int a, int b) { return a > b ? a : b; };
+  Raw ra, rb, rc;
+  Plane A, B, C;
+  load(z0 - 1, ra);
+  load(z0, rb);
+  load(z0 + 1, rc);
+  expand(ra, A);
+  expand(rb, B);
+  for (int64_t z = z0; z < z1; ++z) {
+    Raw rn;
+    load(z + 2, rn);  // clamped: in bounds after the last step too
+    expand(rc, C);
+    const int c = B.v[1][1];
+    int a = mx(A.v[1][1], mx(A.v[0][1], mx(B.v[0][1], C.v[0][1])));
+    int b = mx(C.v[1][1], mx(A.v[2][1], mx(B.v[2][1], C.v[2][1])));
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      a = mx(a, mx(A.v[r][0], mx(B.v[r][0], C.v[r][0])));
+      b = mx(b, mx(A.v[r][2], mx(B.v[r][2], C.v[r][2])));
+    }
+    if (valid) dst[(z * H + y) * W + x] = static_cast<TD>((a < c && b <= c) ? 1 : 0);
+    A = B;
+    B = C;
+    rc = rn;
+  }
+}
+
 template <class TS, class TD>
 __global__ __launch_bounds__(256) void max_z_projection_kernel(const TS* __restrict__ src, TD* __restrict__ dst,
                                                                int64_t W, int64_t H, int64_t D) {
@@ -64,6 +148,13 @@ __global__ __launch_bounds__(256) void max_z_projection_kernel(const TS* __restr
 
 template <class TS, class TD>
 void launch_detect(hipStream_t st, const clk_image& dst, const clk_image& src) {
+  if constexpr (std::is_integral<TS>::value) {
+    const dim3 grid((uint32_t)((src.width + 63) / 64), (uint32_t)src.height,
+                    (uint32_t)((src.depth + kDetectZSeg - 1) / kDetectZSeg));
+    hipLaunchKernelGGL((detect_maxima_int_kernel<TS, TD>), grid, dim3(64), 0, st, static_cast<const TS*>(src.ptr),
+                       static_cast<TD*>(dst.ptr), src.width, src.height, src.depth);
+    return;
+  }
   const dim3 grid((uint32_t)((src.width + 255) / 256), (uint32_t)src.height, (uint32_t)src.depth);
   hipLaunchKernelGGL((detect_maxima_kernel<TS, TD>), grid, dim3(256), 0, st, static_cast<const TS*>(src.ptr),
                      static_cast<TD*>(dst.ptr), src.width, src.height, src.depth);

@@ -610,6 +610,25 @@ def test_detect_maxima_and_projection_all_type_pairs(gpu, src_t, dst_t):
                   "maximum_z_projection")
 
 
+@pytest.mark.parametrize("src_t", [np.int8, np.uint8, np.int16, np.uint16], ids=lambda t: DTYPE_NAMES[t])
+def test_detect_maxima_integer_sliding(gpu, src_t):
+    """Integer sources take the z-sliding kernel: depth 70 spans three z runs
+    (the last partial), width 130 three 64-lane tiles with halo lanes, height
+    9; plateau data (ties everywhere) and full-range data (the type's
+    extremes, where centre - 1 leaves the type's range)."""
+    import clkmi
+    for data in ("plateau", "full"):
+        a = random_volume((70, 9, 130), src_t, seed=41)
+        if data == "plateau":
+            a = (a % 3).astype(src_t)
+        ts, si = to_device(a)
+        for dst_t in (np.uint8, np.float32):
+            td, di = empty_device(a.shape, dst_t)
+            clkmi.detect_maxima_3d(si, di)
+            assert_parity(from_device(td, a.shape, dst_t), O.detect_maxima_3d(a, dst_t),
+                          "detect_maxima_3d %s %s" % (DTYPE_NAMES[src_t], data))
+
+
 def test_consumers_by_name_and_errors(gpu):
     import clkmi
     a = random_volume((4, 9, 33), np.uint16, seed=2)
