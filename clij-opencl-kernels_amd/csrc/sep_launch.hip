@@ -33,20 +33,36 @@ __global__ __launch_bounds__(256) void convert_copy_kernel(const TS* __restrict_
     dst[i] = convert_sat<TD>(static_cast<float>(src[i]));
 }
 
-// 8 consecutive pixels per thread with one vector load and store each
-// (pointers aligned to 8 pixels); the last partial group element by element.
+// 4 consecutive pixels per lane per step and consecutive lanes on
+// consecutive groups of 4, so every load and store instruction of a wave
+// covers one contiguous span (a 32-byte-per-lane store as two 16-byte halves
+// would leave every 64-byte line half written per instruction).  A thread
+// takes kCopySteps steps 1024 pixels apart, their loads issued before any
+// store; pointers are aligned to 4 pixels, a partial last group goes pixel
+// by pixel.
+constexpr int kCopySteps = 4;
+
 template <class TS, class TD>
-__global__ __launch_bounds__(256) void convert_copy8_kernel(const TS* __restrict__ src,
+__global__ __launch_bounds__(256) void convert_copy4_kernel(const TS* __restrict__ src,
                                                             TD* __restrict__ dst, int64_t n) {
-  const int64_t i = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 8;
-  if (i + 8 <= n) {
-    const PixVec<TS, 8> v = load_nt<TS, 8>(src + i);
-    TD o[8];
+  const int64_t base = (int64_t)blockIdx.x * (1024 * kCopySteps) + (int64_t)threadIdx.x * 4;
+  PixVec<TS, 4> v[kCopySteps];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) o[j] = convert_sat<TD>(static_cast<float>(v.v[j]));
-    store_nt<TD, 8>(dst + i, o);
-  } else {
-    for (int64_t j = i; j < n; ++j) dst[j] = convert_sat<TD>(static_cast<float>(src[j]));
+  for (int u = 0; u < kCopySteps; ++u) {
+    const int64_t i = base + (int64_t)u * 1024;
+    if (i + 4 <= n) v[u] = load_nt<TS, 4>(src + i);
+  }
+#pragma unroll
+  for (int u = 0; u < kCopySteps; ++u) {
+    const int64_t i = base + (int64_t)u * 1024;
+    if (i + 4 <= n) {
+      TD o[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) o[j] = convert_sat<TD>(static_cast<float>(v[u].v[j]));
+      store_nt<TD, 4>(dst + i, o);
+    } else {
+      for (int64_t j = i; j < n; ++j) dst[j] = convert_sat<TD>(static_cast<float>(src[j]));
+    }
   }
 }
 
@@ -65,11 +81,11 @@ static uint32_t grid_for(int64_t n) {
 
 template <class TS, class TD>
 static int32_t copy_typed(hipStream_t st, const clk_image& dst, const clk_image& src, int64_t n) {
-  const int64_t groups = (n + 7) / 8;
-  const bool vec = (uintptr_t)src.ptr % (8 * sizeof(TS)) == 0 && (uintptr_t)dst.ptr % (8 * sizeof(TD)) == 0 &&
-                   (groups + 255) / 256 <= 0x7fffffffLL;
+  const int64_t blocks = (n + 1024 * kCopySteps - 1) / (1024 * kCopySteps);
+  const bool vec = (uintptr_t)src.ptr % (4 * sizeof(TS)) == 0 && (uintptr_t)dst.ptr % (4 * sizeof(TD)) == 0 &&
+                   blocks <= 0x7fffffffLL;
   if (vec)
-    hipLaunchKernelGGL((convert_copy8_kernel<TS, TD>), dim3((uint32_t)((groups + 255) / 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL((convert_copy4_kernel<TS, TD>), dim3((uint32_t)blocks), dim3(256), 0, st,
                        static_cast<const TS*>(src.ptr), static_cast<TD*>(dst.ptr), n);
   else
     hipLaunchKernelGGL((convert_copy_kernel<TS, TD>), dim3(grid_for(n)), dim3(256), 0, st,

@@ -503,10 +503,17 @@ def test_sphere_by_name_and_errors(gpu):
 
 
 def test_copy_3d_unaligned_and_ragged(gpu):
-    """The vector copy needs 8-pixel alignment; offset pointers take the
-    element path, and a ragged tail is finished element by element."""
+    """The vector copy needs 4-pixel alignment; offset pointers take the
+    element path, and a ragged tail is finished element by element (also
+    after several blocks of the vector path)."""
     import clkmi
     import torch
+    for st, dt in ((np.uint8, np.float32), (np.float32, np.int16), (np.uint16, np.uint8)):
+        b = random_volume((3, 67, 129), st, seed=9)  # 25929 pixels: 7 blocks, 1 left over
+        tb, bi = to_device(b)
+        td, di = empty_device(b.shape, dt)
+        clkmi.copy_3d(di, bi)
+        assert_parity(from_device(td, b.shape, dt), O.copy_3d(b, dt), "copy %s->%s" % (DTYPE_NAMES[st], DTYPE_NAMES[dt]))
     a = random_volume((3, 7, 13), np.int16, seed=4)
     n = a.size
     src = torch.from_numpy(np.concatenate([np.zeros(1, np.int16), a.reshape(-1)])).cuda()
