@@ -132,6 +132,50 @@ __global__ __launch_bounds__(64) void detect_maxima_int_kernel(const TS* __restr
   }
 }
 
+// maximum_z_projection of integer sources: the reference's running value
+// (the first plane's, then each strictly greater one) is the plain maximum,
+// which does not depend on order, so the depth is split: the four waves of a
+// block take a quarter of the planes each for the same 256 x (4 pixels per
+// lane, a vector load when the rows are 4-pixel aligned), and the partial
+// maxima meet in LDS.
+template <class TS, class TD>
+__global__ __launch_bounds__(256) void max_z_projection_int_kernel(const TS* __restrict__ src, TD* __restrict__ dst,
+                                                                   int64_t W, int64_t H, int64_t D, int32_t vec) {
+  __shared__ int part[4][256];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t x0 = (int64_t)blockIdx.x * 256 + lane * 4;
+  const int64_t y = blockIdx.y;
+  const int64_t zc = (D + 3) / 4;
+  const int64_t za = wv * zc, zb = (za + zc < D) ? za + zc : D;
+  int m[4] = {INT32_MIN, INT32_MIN, INT32_MIN, INT32_MIN};
+  const bool full = vec && x0 + 4 <= W;
+  const int64_t plane = H * W;
+  const TS* col = src + y * W + x0;
+#pragma unroll 4
+  for (int64_t z = za; z < zb; ++z) {
+    const TS* p = col + z * plane;
+    if (full) {
+      const PixVec<TS, 4> v = load_nt<TS, 4>(p);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) m[j] = m[j] > (int)v.v[j] ? m[j] : (int)v.v[j];
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (x0 + j < W) m[j] = m[j] > (int)p[j] ? m[j] : (int)p[j];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) part[wv][lane * 4 + j] = m[j];
+  __syncthreads();
+  const int t = threadIdx.x;
+  const int64_t x = (int64_t)blockIdx.x * 256 + t;
+  if (x >= W) return;
+  int r = part[0][t];  // the first quarter is never empty
+#pragma unroll
+  for (int w = 1; w < 4; ++w) r = r > part[w][t] ? r : part[w][t];
+  dst[y * W + x] = convert_sat<TD>(static_cast<float>(r));  // :18, CONVERT_dst_max
+}
+
 template <class TS, class TD>
 __global__ __launch_bounds__(256) void max_z_projection_kernel(const TS* __restrict__ src, TD* __restrict__ dst,
                                                                int64_t W, int64_t H, int64_t D) {
@@ -163,6 +207,12 @@ void launch_detect(hipStream_t st, const clk_image& dst, const clk_image& src) {
 template <class TS, class TD>
 void launch_proj(hipStream_t st, const clk_image& dst, const clk_image& src) {
   const dim3 grid((uint32_t)((src.width + 255) / 256), (uint32_t)src.height, 1);
+  if constexpr (std::is_integral<TS>::value) {
+    const int32_t vec = src.width % 4 == 0 && (uintptr_t)src.ptr % (4 * sizeof(TS)) == 0;
+    hipLaunchKernelGGL((max_z_projection_int_kernel<TS, TD>), grid, dim3(256), 0, st, static_cast<const TS*>(src.ptr),
+                       static_cast<TD*>(dst.ptr), src.width, src.height, src.depth, vec);
+    return;
+  }
   hipLaunchKernelGGL((max_z_projection_kernel<TS, TD>), grid, dim3(256), 0, st, static_cast<const TS*>(src.ptr),
                      static_cast<TD*>(dst.ptr), src.width, src.height, src.depth);
 }

@@ -636,6 +636,22 @@ def test_detect_maxima_integer_sliding(gpu, src_t):
                           "detect_maxima_3d %s %s" % (DTYPE_NAMES[src_t], data))
 
 
+@pytest.mark.parametrize("shape", [(37, 5, 131), (1, 3, 260), (3, 4, 64), (130, 2, 300)])
+def test_maximum_z_projection_integer_depth_split(gpu, shape):
+    """Integer sources split the depth over four waves: depths below and
+    above 4, widths that are not a multiple of 4 (element loads) and a
+    partial last 256-pixel block."""
+    import clkmi
+    for src_t in (np.int8, np.uint8, np.int16, np.uint16):
+        a = random_volume(shape, src_t, seed=shape[0] + 3)
+        ts, si = to_device(a)
+        for dst_t in (np.uint8, np.int16, np.float32):
+            tp, dp = empty_device((1,) + a.shape[1:], dst_t)
+            clkmi.maximum_z_projection(dp, si)
+            assert_parity(from_device(tp, (1,) + a.shape[1:], dst_t), O.maximum_z_projection(a, dst_t),
+                          "projection %s %s->%s" % (shape, DTYPE_NAMES[src_t], DTYPE_NAMES[dst_t]))
+
+
 def test_consumers_by_name_and_errors(gpu):
     import clkmi
     a = random_volume((4, 9, 33), np.uint16, seed=2)
