@@ -169,6 +169,57 @@ def test_host_chain_matches_oracle_chain(gpu):
         assert_parity(from_device(td, src.shape, np.uint16), want, fn)
 
 
+def test_full_size_c1_gaussian_chain(gpu):
+    """The reference's CPU-sized case at its full size: float32 256x256x64,
+    U[0, 1000) (seed 1), the 3D Gaussian sigma = (2, 2, 1) -> N = (17, 17, 9)
+    through the host chain, every voxel against the oracle's chain."""
+    import clkmi
+    src = np.random.default_rng(1).uniform(0, 1000, size=(64, 256, 256)).astype(np.float32)
+    ts, si = to_device(src)
+    td, di = empty_device(src.shape, np.float32)
+    clkmi.gaussian_blur_3d(si, di, 2.0, 2.0, 1.0)
+    want = O.gaussian_blur_3d(src, np.float32, 2.0, 2.0, 1.0)
+    assert_parity(from_device(td, src.shape, np.float32), want, "C1 gaussian chain")
+
+
+def test_full_size_c3_median_crops(gpu):
+    """median_box_3d r = 1 and r = 2 on the full uint8 512x512x256
+    salt-and-pepper volume, checked on 24^3 crops: the four corners of the
+    first and last planes (the volume's borders, where the clamp happens)
+    and random interior crops, each compared inside a margin of r against
+    the oracle run on the crop (an interior voxel's window lies in the crop;
+    at a true border both clamp to the same edge)."""
+    import torch
+    import clkmi
+    rng = np.random.default_rng(3)
+    D, H, W = 256, 512, 512
+    vol = rng.integers(40, 200, size=(D, H, W), endpoint=True).astype(np.uint8)
+    m = rng.random(vol.shape) < 0.05
+    vol[m] = np.where(rng.random(int(m.sum())) < 0.5, 0, 255).astype(np.uint8)
+    ts, si = to_device(vol)
+    C = 24
+    starts = [(z, y, x) for z in (0, D - C) for y in (0, H - C) for x in (0, W - C)]
+    starts += [tuple(int(rng.integers(0, n - C)) for n in (D, H, W)) for _ in range(8)]
+    for r in (1, 2):
+        td, di = empty_device(vol.shape, np.uint8)
+        clkmi.median_box_3d(di, si, 2 * r + 1, 2 * r + 1, 2 * r + 1)
+        out = from_device(td, vol.shape, np.uint8)
+        for (z, y, x) in starts:
+            crop = vol[z:z + C, y:y + C, x:x + C]
+            want = O.median_box_3d(crop, np.uint8, 2 * r + 1, 2 * r + 1, 2 * r + 1)
+            # keep the part whose windows do not reach a crop face that is
+            # not also a volume face
+            sl = []
+            for a, n in zip((z, y, x), (D, H, W)):
+                lo = 0 if a == 0 else r
+                hi = C if a + C == n else C - r
+                sl.append(slice(lo, hi))
+            sl = tuple(sl)
+            assert_parity(out[z:z + C, y:y + C, x:x + C][sl], want[sl], "C3 median r=%d crop %s" % (r, (z, y, x)))
+    del ts
+    torch.cuda.empty_cache()
+
+
 def test_errors_are_opencl_codes(gpu):
     import clkmi
     src = random_volume((4, 5, 6), np.uint8, seed=0)
