@@ -10,14 +10,15 @@
 //
 //  * 3x3x3 boxes: the 27 values sit in registers and a forgetful-selection
 //    network (repeatedly discard the minimum and maximum of a k+2 element
-//    buffer, then feed the next value) leaves the median; about 400 min/max
-//    per voxel, no memory traffic beyond the gather.
+//    buffer, then feed the next value) leaves the median; for 8- and 16-bit
+//    dst two voxels share each instruction (16-bit pairs), and single
+//    launches slide along z so that a step stages one new plane of rows.
 //  * 8-bit dst and up to 255 values: each lane walks a z-column of outputs
 //    with a 256-bin histogram of its box in LDS (one byte per bin); a step
 //    in z removes the leaving plane, adds the entering one and moves the
-//    tracked median a few bins (Huang's running median, along z).  The
-//    bins of each lane are rotated by 4*lane so that lanes holding equal
-//    values hit different LDS banks.
+//    tracked median a few bins (Huang's running median, along z).  Counter
+//    dword d of lane l sits at d*64 + l, so every lane stays in its own
+//    LDS bank.
 //  * any other box: a bisection over the dst_t value range counts, per probe,
 //    how many window values are <= the probe (8, 16 or 32 probes).  No array,
 //    so no MAX_ARRAY_SIZE limit.
@@ -201,6 +202,93 @@ __device__ __forceinline__ void median_27_pk8_body(const MedParams& p, uint32_t 
   TD* __restrict__ dst = static_cast<TD*>(p.dst) + (z * p.H + y) * p.W;
   if (x < W) dst[x] = static_cast<TD>((int)m.x - kBias);
   if (x + 1 < W) dst[x + 1] = static_cast<TD>((int)m.y - kBias);
+}
+
+// 3x3x3 with an 8- or 16-bit dst for single launches: the packed network
+// of median_27_pk8_body over a run of kPk8ZRun planes per block.  LDS holds
+// a ring of four planes (rows y-1, y, y+1 as the even- and odd-aligned
+// 16-bit pairs); a z step reads the slots of planes z-1, z, z+1, while the
+// loads of plane z+2 are in flight in registers, then writes them to the
+// fourth slot (that of plane z-2, not read in this step) before one block
+// barrier.
+constexpr int kPk8ZRun = 16;
+
+template <class TS, class TD>
+__device__ __forceinline__ void median_27_pk8_slide_body(const MedParams& p, uint32_t bx, uint32_t by, uint32_t bz) {
+  __shared__ uint32_t pe[4][3][257];
+  __shared__ uint32_t po[4][3][256];
+  const int t = threadIdx.x;
+  const int64_t xt = (int64_t)bx * 512;
+  const int64_t y = by;
+  const int64_t z0 = (int64_t)bz * kPk8ZRun;
+  const int64_t z1 = (z0 + kPk8ZRun < p.D) ? z0 + kPk8ZRun : p.D;
+  const int W = (int)p.W;
+  const TS* __restrict__ src = static_cast<const TS*>(p.src);
+  constexpr int kBias = -static_cast<int>(Pix<TD>::lo);  // signed values shift to 0..2^bits-1
+  // this thread's columns x = xt - 1 + 2t + i, i = 0..3 (i = 3 for t = 255 only), clamped
+  int64_t xc[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int64_t xx = xt - 1 + 2 * t + i;
+    xc[i] = xx < 0 ? 0 : (xx >= W ? W - 1 : xx);
+  }
+  int64_t rows[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) rows[r] = clamp64(y + r - 1, 0, p.H - 1) * p.W;
+  const int64_t PS = p.H * p.W;
+  TS raw[3][4];
+  auto fetch = [&](int64_t zz) {
+    const TS* pl = src + clamp64(zz, 0, p.D - 1) * PS;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) raw[r][i] = pl[rows[r] + xc[i]];
+      raw[r][3] = t == 255 ? pl[rows[r] + xc[3]] : TS(0);
+    }
+  };
+  auto put = [&](int slot) {
+    auto v = [&](TS x) -> uint32_t { return (uint32_t)((int)gather_value<TS, TD>(x, p.wrap) + kBias); };
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const uint32_t v0 = v(raw[r][0]), v1 = v(raw[r][1]), v2 = v(raw[r][2]);
+      pe[slot][r][t] = v0 | (v1 << 16);
+      po[slot][r][t] = v1 | (v2 << 16);
+      if (t == 255) pe[slot][r][256] = v2 | (v(raw[r][3]) << 16);
+    }
+  };
+  // plane zz sits in slot (zz - z0 + 1) & 3
+  fetch(z0 - 1);
+  put(0);
+  fetch(z0);
+  put(1);
+  fetch(z0 + 1);
+  put(2);
+  __syncthreads();
+  const int64_t x = xt + 2 * t;
+  TD* __restrict__ dst = static_cast<TD*>(p.dst);
+  for (int64_t z = z0; z < z1; ++z) {
+    const int zi = (int)(z - z0);
+    const bool more = z + 1 < z1;
+    if (more) fetch(z + 2);
+    us2 a[27];
+#pragma unroll
+    for (int dz = 0; dz < 3; ++dz) {
+      const int sl = (zi + dz) & 3;  // plane z - 1 + dz
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const uint32_t w0 = pe[sl][r][t], w1 = po[sl][r][t], w2 = pe[sl][r][t + 1];
+        __builtin_memcpy(&a[9 * dz + 3 * r + 0], &w0, 4);
+        __builtin_memcpy(&a[9 * dz + 3 * r + 1], &w1, 4);
+        __builtin_memcpy(&a[9 * dz + 3 * r + 2], &w2, 4);
+      }
+    }
+    const us2 m = forgetful_median<27>(a);
+    TD* const row = dst + (z * p.H + y) * p.W;
+    if (x < W) row[x] = static_cast<TD>((int)m.x - kBias);
+    if (x + 1 < W) row[x + 1] = static_cast<TD>((int)m.y - kBias);
+    if (more) put((zi + 3) & 3);
+    __syncthreads();
+  }
 }
 
 template <class TS, class TD>
@@ -472,7 +560,6 @@ __device__ __forceinline__ void median_hist8_fixed_body(const MedParams& p, uint
     NAME##_body<TS, TD>(p, blockIdx.x, blockIdx.y, blockIdx.z);                      \
   }
 CLK_MEDIAN_KERNEL(median_generic, 256)
-CLK_MEDIAN_KERNEL(median_27_pk8, 256)
 CLK_MEDIAN_KERNEL(median_27, 256)
 #undef CLK_MEDIAN_KERNEL
 
@@ -484,6 +571,11 @@ __global__ __launch_bounds__(64) void median_hist8_kernel(const MedParams p) {
 template <class TS, class TD, int EX, int EY>
 __global__ __launch_bounds__(64) void median_hist8_fixed_kernel(const MedParams p) {
   median_hist8_fixed_body<TS, TD, EX, EY>(p, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+template <class TS, class TD>
+__global__ __launch_bounds__(256) void median_27_pk8_slide_kernel(const MedParams p) {
+  median_27_pk8_slide_body<TS, TD>(p, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 template <class TS, class TD>

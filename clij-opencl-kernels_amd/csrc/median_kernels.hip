@@ -15,7 +15,12 @@ static int32_t median_typed(hipStream_t st, MedParams& p, std::string* err) {
   dim3 grid;
   int threads;
   switch (plan_median(p, (int)sizeof(TD), &grid, &threads)) {
-    case MED_PK8: hipLaunchKernelGGL((median_27_pk8_kernel<TS, TD>), grid, dim3(threads), 0, st, p); break;
+    case MED_PK8: {
+      // single launches slide along z (runs of kPk8ZRun planes)
+      const dim3 g(grid.x, grid.y, (uint32_t)((p.D + kPk8ZRun - 1) / kPk8ZRun));
+      hipLaunchKernelGGL((median_27_pk8_slide_kernel<TS, TD>), g, dim3(threads), 0, st, p);
+      break;
+    }
     case MED_PK9: hipLaunchKernelGGL((median_9_pk8_kernel<TS, TD>), grid, dim3(threads), 0, st, p); break;
     case MED_HIST8:
       // planned for 8-bit dst only

@@ -135,6 +135,17 @@ def test_median_hist8_wide_rows(gpu, src_t):
             assert_parity(got, want, "median %s -> %s %s" % (DTYPE_NAMES[src_t], DTYPE_NAMES[dst_t], Ns))
 
 
+@pytest.mark.parametrize("src_t,dst_t", [(np.uint8, np.uint8), (np.int16, np.int16), (np.float32, np.uint16),
+                                         (np.uint16, np.int8)], ids=lambda t: DTYPE_NAMES[t])
+def test_median_3x3x3_packed_z_runs(gpu, src_t, dst_t):
+    """The packed 3x3x3 median of single launches slides along z in runs of
+    16 planes: depth 37 (three runs, the last partial), width 1030 (three
+    512-wide tiles, the last partial), height 5."""
+    a = random_volume((37, 5, 1030), src_t, seed=71)
+    got = gpu_median(a, dst_t, 3, 3, 3)
+    assert_parity(got, O.median_box_3d(a, dst_t, 3, 3, 3), "median 3x3x3 %s->%s" % (DTYPE_NAMES[src_t], DTYPE_NAMES[dst_t]))
+
+
 def test_execute_by_name_matches_typed_call(gpu):
     import clkmi
     src = random_volume((6, 7, 8), np.uint16, seed=1)
