@@ -291,6 +291,77 @@ __device__ __forceinline__ void median_27_pk8_slide_body(const MedParams& p, uin
   }
 }
 
+// 3x3 in one plane (median_box_2d, median_slice_by_slice_box_3d) with an 8-
+// or 16-bit dst for single launches: the same network over a run of
+// kPk9YRun rows per block, sliding along y over a ring of four staged rows
+// (rows y-1, y, y+1 read while row y+2 is loaded into registers, then
+// written to the fourth slot before one block barrier).
+constexpr int kPk9YRun = 16;
+
+template <class TS, class TD>
+__device__ __forceinline__ void median_9_pk8_slide_body(const MedParams& p, uint32_t bx, uint32_t by, uint32_t bz) {
+  __shared__ uint32_t pe[4][257];
+  __shared__ uint32_t po[4][256];
+  const int t = threadIdx.x;
+  const int64_t xt = (int64_t)bx * 512;
+  const int64_t y0 = (int64_t)by * kPk9YRun;
+  const int64_t y1 = (y0 + kPk9YRun < p.H) ? y0 + kPk9YRun : p.H;
+  const int64_t z = bz;
+  const int W = (int)p.W;
+  const TS* __restrict__ plane = static_cast<const TS*>(p.src) + z * p.H * p.W;
+  constexpr int kBias = -static_cast<int>(Pix<TD>::lo);
+  int64_t xc[4];  // x = xt - 1 + 2t + i, clamped (i = 3 for t = 255 only)
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int64_t xx = xt - 1 + 2 * t + i;
+    xc[i] = xx < 0 ? 0 : (xx >= W ? W - 1 : xx);
+  }
+  TS raw[4];
+  auto fetch = [&](int64_t yy) {
+    const TS* row = plane + clamp64(yy, 0, p.H - 1) * p.W;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) raw[i] = row[xc[i]];
+    raw[3] = t == 255 ? row[xc[3]] : TS(0);
+  };
+  auto put = [&](int slot) {
+    auto v = [&](TS x) -> uint32_t { return (uint32_t)((int)gather_value<TS, TD>(x, p.wrap) + kBias); };
+    const uint32_t v0 = v(raw[0]), v1 = v(raw[1]), v2 = v(raw[2]);
+    pe[slot][t] = v0 | (v1 << 16);
+    po[slot][t] = v1 | (v2 << 16);
+    if (t == 255) pe[slot][256] = v2 | (v(raw[3]) << 16);
+  };
+  // row yy sits in slot (yy - y0 + 1) & 3
+  fetch(y0 - 1);
+  put(0);
+  fetch(y0);
+  put(1);
+  fetch(y0 + 1);
+  put(2);
+  __syncthreads();
+  const int64_t x = xt + 2 * t;
+  TD* __restrict__ dst = static_cast<TD*>(p.dst) + z * p.H * p.W;
+  for (int64_t y = y0; y < y1; ++y) {
+    const int yi = (int)(y - y0);
+    const bool more = y + 1 < y1;
+    if (more) fetch(y + 2);
+    us2 a[9];
+#pragma unroll
+    for (int dr = 0; dr < 3; ++dr) {
+      const int sl = (yi + dr) & 3;  // row y - 1 + dr
+      const uint32_t w0 = pe[sl][t], w1 = po[sl][t], w2 = pe[sl][t + 1];
+      __builtin_memcpy(&a[3 * dr + 0], &w0, 4);
+      __builtin_memcpy(&a[3 * dr + 1], &w1, 4);
+      __builtin_memcpy(&a[3 * dr + 2], &w2, 4);
+    }
+    const us2 m = forgetful_median<9>(a);
+    TD* const row = dst + y * p.W;
+    if (x < W) row[x] = static_cast<TD>((int)m.x - kBias);
+    if (x + 1 < W) row[x + 1] = static_cast<TD>((int)m.y - kBias);
+    if (more) put((yi + 3) & 3);
+    __syncthreads();
+  }
+}
+
 template <class TS, class TD>
 __device__ __forceinline__ void median_27_body(const MedParams& p, uint32_t bx, uint32_t by, uint32_t bz) {
   const int64_t x = (int64_t)bx * 256 + threadIdx.x;
@@ -579,9 +650,10 @@ __global__ __launch_bounds__(256) void median_27_pk8_slide_kernel(const MedParam
 }
 
 template <class TS, class TD>
-__global__ __launch_bounds__(256) void median_9_pk8_kernel(const MedParams p) {
-  median_27_pk8_body<TS, TD, 1>(p, blockIdx.x, blockIdx.y, blockIdx.z);
+__global__ __launch_bounds__(256) void median_9_pk8_slide_kernel(const MedParams p) {
+  median_9_pk8_slide_body<TS, TD>(p, blockIdx.x, blockIdx.y, blockIdx.z);
 }
+
 
 enum MedKind : int { MED_PK8 = 0, MED_HIST8 = 1, MED_27 = 2, MED_GENERIC = 3, MED_PK9 = 4 };
 

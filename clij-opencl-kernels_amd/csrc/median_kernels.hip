@@ -21,7 +21,12 @@ static int32_t median_typed(hipStream_t st, MedParams& p, std::string* err) {
       hipLaunchKernelGGL((median_27_pk8_slide_kernel<TS, TD>), g, dim3(threads), 0, st, p);
       break;
     }
-    case MED_PK9: hipLaunchKernelGGL((median_9_pk8_kernel<TS, TD>), grid, dim3(threads), 0, st, p); break;
+    case MED_PK9: {
+      // single launches slide along y (runs of kPk9YRun rows)
+      const dim3 g(grid.x, (uint32_t)((p.H + kPk9YRun - 1) / kPk9YRun), grid.z);
+      hipLaunchKernelGGL((median_9_pk8_slide_kernel<TS, TD>), g, dim3(threads), 0, st, p);
+      break;
+    }
     case MED_HIST8:
       // planned for 8-bit dst only
       if constexpr (sizeof(TD) == 1) {

@@ -464,6 +464,27 @@ def test_2d_median_all_type_pairs(gpu, src_t, dst_t):
         assert_parity(got, want, "median_box_2d %dx%d" % (Nx, Ny))
 
 
+@pytest.mark.parametrize("src_t,dst_t", [(np.uint8, np.uint8), (np.uint16, np.uint16), (np.int16, np.int8),
+                                         (np.float32, np.uint16)], ids=lambda t: DTYPE_NAMES[t])
+def test_median_3x3_packed_y_runs(gpu, src_t, dst_t):
+    """The packed 3x3 median of single launches slides along y in runs of
+    16 rows: height 37 (three runs, the last partial), width 1030 (three
+    512-wide tiles, the last partial), for median_box_2d and for each plane
+    of median_slice_by_slice_box_3d."""
+    import clkmi
+    a = random_volume((37, 1030), src_t, seed=73)
+    ts, si = _image_2d(a)
+    td, di = _image_2d(a.shape, dst_t)
+    clkmi.median_box_2d(di, si, 3, 3)
+    assert_parity(from_device(td, a.shape, dst_t), O.median_box_2d(a, dst_t, 3, 3), "median_box_2d 3x3")
+    v = random_volume((3, 37, 1030), src_t, seed=74)
+    tv, vi = to_device(v)
+    tw, wi = empty_device(v.shape, dst_t)
+    clkmi.median_slice_by_slice_box_3d(wi, vi, 3, 3)
+    assert_parity(from_device(tw, v.shape, dst_t), O.median_slice_by_slice_box_3d(v, dst_t, 3, 3),
+                  "slice median 3x3")
+
+
 def test_2d_by_name_chains_and_errors(gpu):
     """execute() with the _2d names and program file names, the 2D host
     chains, and the depth-1 rule."""
