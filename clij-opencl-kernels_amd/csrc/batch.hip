@@ -65,8 +65,8 @@ __global__ __launch_bounds__(256) void batch_median_kernel(const MedParams* __re
   const uint32_t bx = t.vblk % p.gx;
   const uint32_t r = t.vblk / p.gx;
   const uint32_t by = r % p.gy, bz = r / p.gy;
-  if constexpr (KIND == MED_PK8) median_27_pk8_body<TS, TD>(p, bx, by, bz);
-  else if constexpr (KIND == MED_PK9) median_27_pk8_body<TS, TD, 1>(p, bx, by, bz);
+  if constexpr (KIND == MED_PK8) median_27_pk8_slide_body<TS, TD>(p, bx, by, bz);
+  else if constexpr (KIND == MED_PK9) median_9_pk8_slide_body<TS, TD>(p, bx, by, bz);
   else if constexpr (KIND == MED_27) median_27_body<TS, TD>(p, bx, by, bz);
   else median_generic_body<TS, TD>(p, bx, by, bz);
 }
@@ -383,6 +383,13 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
     dim3 grid;
     int threads;
     const MedKind mk = plan_median(p, clk_dtype_size(jb.dst.dtype), &grid, &threads);
+    // the packed networks slide along z (3x3x3) or y (3x3) as in single launches
+    if (mk == MED_PK8) {
+      grid.z = (uint32_t)((p.D + kPk8ZRun - 1) / kPk8ZRun);
+    } else if (mk == MED_PK9) {
+      grid.y = (uint32_t)((p.H + kPk9YRun - 1) / kPk9YRun);
+      p.gy = grid.y;
+    }
     ClassKey key{K_MEDIAN, (int)mk, jb.src.dtype, jb.dst.dtype};
     ClassWork& w = classes[0][key];
     const uint32_t jid = (uint32_t)w.med.size();

@@ -151,66 +151,18 @@ __device__ __forceinline__ T forgetful_median(T (&a)[N]) {
   return vmax(vmin(x0, x1), vmin(vmax(x0, x1), x2));
 }
 
-// 3x3x3 (ZN = 3 planes) or 3x3x1 (ZN = 1: median_box_2d, slice medians)
-// median, 8- or 16-bit integer dst: a block filters 512 consecutive x of one
-// (y, z).  The 3*ZN source rows (x-1 .. x+512, clamped), converted to
-// the dst type and biased to unsigned 16-bit keys (signed types shift by
-// -lo, which keeps their order), are staged in LDS as 16-bit value pairs
-// twice, pairs starting at even and at odd positions, so the three pairs a
-// thread needs per row (its two outputs shifted by -1, 0, +1) are three
-// aligned 32-bit reads; the selection network then runs on two voxels per
-// instruction (v_pk_min_u16 / v_pk_max_u16).
-template <class TS, class TD, int ZN = 3>
-__device__ __forceinline__ void median_27_pk8_body(const MedParams& p, uint32_t bx, uint32_t by, uint32_t bz) {
-  static_assert(ZN == 1 || ZN == 3, "one or three planes");
-  constexpr int R = 3 * ZN;  // staged rows
-  __shared__ uint32_t pe[R][257];
-  __shared__ uint32_t po[R][256];
-  const int t = threadIdx.x;
-  const int64_t xt = (int64_t)bx * 512;
-  const int64_t y = by, z = bz;
-  const int W = (int)p.W;
-  const TS* __restrict__ src = static_cast<const TS*>(p.src);
-  constexpr int kBias = -static_cast<int>(Pix<TD>::lo);  // signed values shift to 0..2^bits-1
-  auto val = [&](const TS* row, int64_t x) -> uint32_t {
-    x = x < 0 ? 0 : (x >= W ? W - 1 : x);
-    return (uint32_t)((int)gather_value<TS, TD>(row[x], p.wrap) + kBias);
-  };
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int64_t zz = clamp64(z + r / 3 - (ZN == 3 ? 1 : 0), 0, p.D - 1);
-    const int64_t yy = clamp64(y + r % 3 - 1, 0, p.H - 1);
-    const TS* row = src + (zz * p.H + yy) * p.W;
-    // v[i] = value at x = xt - 1 + i
-    const int64_t xb = xt - 1 + 2 * t;
-    const uint32_t v0 = val(row, xb), v1 = val(row, xb + 1), v2 = val(row, xb + 2);
-    pe[r][t] = v0 | (v1 << 16);
-    po[r][t] = v1 | (v2 << 16);
-    if (t == 255) pe[r][256] = v2 | (val(row, xb + 3) << 16);
-  }
-  __syncthreads();
-  us2 a[3 * R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const uint32_t w0 = pe[r][t], w1 = po[r][t], w2 = pe[r][t + 1];
-    __builtin_memcpy(&a[3 * r + 0], &w0, 4);
-    __builtin_memcpy(&a[3 * r + 1], &w1, 4);
-    __builtin_memcpy(&a[3 * r + 2], &w2, 4);
-  }
-  const us2 m = forgetful_median<3 * R>(a);
-  const int64_t x = xt + 2 * t;
-  TD* __restrict__ dst = static_cast<TD*>(p.dst) + (z * p.H + y) * p.W;
-  if (x < W) dst[x] = static_cast<TD>((int)m.x - kBias);
-  if (x + 1 < W) dst[x + 1] = static_cast<TD>((int)m.y - kBias);
-}
-
-// 3x3x3 with an 8- or 16-bit dst for single launches: the packed network
-// of median_27_pk8_body over a run of kPk8ZRun planes per block.  LDS holds
-// a ring of four planes (rows y-1, y, y+1 as the even- and odd-aligned
-// 16-bit pairs); a z step reads the slots of planes z-1, z, z+1, while the
-// loads of plane z+2 are in flight in registers, then writes them to the
-// fourth slot (that of plane z-2, not read in this step) before one block
-// barrier.
+// 3x3x3 median with an 8- or 16-bit dst: a block filters 512 consecutive x
+// of one row y for a run of kPk8ZRun planes.  Source values, converted to the
+// dst type and biased to unsigned 16-bit keys (signed types shift by -lo,
+// which keeps their order), are staged in LDS as 16-bit value pairs twice,
+// pairs starting at even and at odd positions, so the three pairs a thread
+// needs per row (its two outputs shifted by -1, 0, +1) are three aligned
+// 32-bit reads; the selection network then runs on two voxels per
+// instruction (v_pk_min_u16 / v_pk_max_u16).  LDS holds a ring of four
+// planes (rows y-1, y, y+1); a z step reads the slots of planes z-1, z, z+1,
+// while the loads of plane z+2 are in flight in registers, then writes them
+// to the fourth slot (that of plane z-2, not read in this step) before one
+// block barrier.
 constexpr int kPk8ZRun = 16;
 
 template <class TS, class TD>
@@ -292,10 +244,10 @@ __device__ __forceinline__ void median_27_pk8_slide_body(const MedParams& p, uin
 }
 
 // 3x3 in one plane (median_box_2d, median_slice_by_slice_box_3d) with an 8-
-// or 16-bit dst for single launches: the same network over a run of
-// kPk9YRun rows per block, sliding along y over a ring of four staged rows
-// (rows y-1, y, y+1 read while row y+2 is loaded into registers, then
-// written to the fourth slot before one block barrier).
+// or 16-bit dst: the same pairs and network over a run of kPk9YRun rows per
+// block, sliding along y over a ring of four staged rows (rows y-1, y, y+1
+// read while row y+2 is loaded into registers, then written to the fourth
+// slot before one block barrier).
 constexpr int kPk9YRun = 16;
 
 template <class TS, class TD>

@@ -69,6 +69,23 @@ PAIRS = [(np.uint16, np.uint16), (np.uint16, np.float32), (np.float32, np.float3
          (np.int16, np.float32), (np.float32, np.uint16), (np.int8, np.int8), (np.uint16, np.uint8)]
 
 
+def test_batch_packed_medians_slide(gpu):
+    """Batched 3x3x1 and 3x3x3 medians (the packed networks, sliding along y
+    and z in runs of 16) on shapes spanning several runs and 512-wide tiles,
+    equal to single calls and the oracle."""
+    jobs = []
+    for i, (shape, radius, st, dt) in enumerate([
+            ((3, 37, 530), (1, 1, 0), np.uint8, np.uint8), ((2, 16, 20), (1, 1, 0), np.uint16, np.uint16),
+            ((37, 5, 600), (1, 1, 1), np.uint8, np.uint8), ((17, 19, 33), (1, 1, 1), np.int16, np.int16),
+            ((40, 3, 9), (1, 1, 1), np.float32, np.float32), ((5, 33, 40), (1, 1, 0), np.uint16, np.uint8)]):
+        src = random_volume(shape, st, seed=500 + i)
+        jobs.append(("median_box_3d", src, dt, (0.0, 0.0, 0.0), radius))
+    got = run_batch(jobs)
+    for i, (job, g) in enumerate(zip(jobs, got)):
+        assert same_bits_or_value(g, run_single(job)), "job %d single" % i
+        assert same_bits_or_value(g, oracle_job(*job)), "job %d oracle" % i
+
+
 @pytest.mark.parametrize("seed", [1, 2])
 def test_batch_matches_single_calls_and_oracle(gpu, seed):
     rng = np.random.default_rng(seed)
