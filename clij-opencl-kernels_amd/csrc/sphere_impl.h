@@ -195,7 +195,11 @@ struct SphereSlideParams {
 };
 static_assert(kSphereTileMaxOff % 8 == 0, "offset chunks of 8");
 
-template <class TS, class TD, int OP, int TY>
+// NET > 0 (median, dst of 8 or 16 bits, exactly NET offsets): the keys of
+// each pair of output rows are gathered once into 16-bit pairs and the
+// forgetful-selection network of the box median runs on them, instead of
+// bisection rounds over LDS.
+template <class TS, class TD, int OP, int TY, int NET = 0>
 __global__ __launch_bounds__(256) void sphere_slide_kernel(const SphereSlideParams p) {
   constexpr int NMAX = TY + 2 * kSlideMaxEY;
   constexpr int RL = kSlideRL;
@@ -290,7 +294,25 @@ __global__ __launch_bounds__(256) void sphere_slide_kernel(const SphereSlidePara
           if (u < rem) f(base(o[u]));
       }
     };
-    if constexpr (OP == SPH_MEDIAN) {
+    if constexpr (OP == SPH_MEDIAN && NET > 0) {
+      // rank NET/2 of the keys (median_sphere_3d_x.cl:54-68); keys of an
+      // 8- or 16-bit dst fit 16 bits, rows j and j+1 share a register
+      static_assert(TY % 2 == 0 && (NET & 1) && KeyBits<TD>::v <= 16, "row pairs, odd count, 16-bit keys");
+      // one pair of rows at a time keeps a single NET-entry array live
+#pragma unroll
+      for (int jp = 0; jp < TY / 2; ++jp) {
+        us2 a[NET];
+#pragma unroll
+        for (int i = 0; i < NET; ++i) {
+          const uint32_t* b = base(p.off[i]);
+          const uint32_t w = b[(2 * jp) * RL] | (b[(2 * jp + 1) * RL] << 16);
+          __builtin_memcpy(&a[i], &w, 4);
+        }
+        const us2 m = forgetful_median<NET>(a);
+        store(2 * jp, static_cast<TD>(key_value<TD>(m.x)));
+        store(2 * jp + 1, static_cast<TD>(key_value<TD>(m.y)));
+      }
+    } else if constexpr (OP == SPH_MEDIAN) {
       // rank noff/2 of the keys (median_sphere_3d_x.cl:54-68) by bisection:
       // each round halves [lo, hi], so KeyBits rounds leave lo == hi
       const int k = p.noff / 2;

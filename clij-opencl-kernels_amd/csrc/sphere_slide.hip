@@ -10,8 +10,24 @@ namespace clk {
 
 namespace {
 
+// the register network for medians of 7 or 33 values (the balls of N = 3
+// and 5) with an 8- or 16-bit dst
+inline bool slide_net(int op, int n, int dst_t) {
+  return op == SPH_MEDIAN && (n == 7 || n == 33) && dst_t != CLK_FLOAT;
+}
+
 template <class TS, class TD, int TY>
 void slide_op(int op, hipStream_t st, const SphereSlideParams& sp, dim3 grid, size_t shmem) {
+  if constexpr (sizeof(TD) <= 2) {
+    if (op == SPH_MEDIAN && sp.noff == 7) {
+      hipLaunchKernelGGL((sphere_slide_kernel<TS, TD, SPH_MEDIAN, TY, 7>), grid, dim3(256), shmem, st, sp);
+      return;
+    }
+    if (op == SPH_MEDIAN && sp.noff == 33) {
+      hipLaunchKernelGGL((sphere_slide_kernel<TS, TD, SPH_MEDIAN, TY, 33>), grid, dim3(256), shmem, st, sp);
+      return;
+    }
+  }
   switch (op) {
     case SPH_MIN: hipLaunchKernelGGL((sphere_slide_kernel<TS, TD, SPH_MIN, TY>), grid, dim3(256), shmem, st, sp); break;
     case SPH_MAX: hipLaunchKernelGGL((sphere_slide_kernel<TS, TD, SPH_MAX, TY>), grid, dim3(256), shmem, st, sp); break;
@@ -100,7 +116,7 @@ bool launch_sphere_hist8(hipStream_t st, const SphereParams& p, int src_t, int d
 
 bool launch_sphere_slide(int op, hipStream_t st, const SphereParams& p, int src_t, int dst_t) {
   if (tuning(TUNE_SPHERE_DIRECT) != 0) return false;
-  if (op == SPH_MEDIAN && launch_sphere_hist8(st, p, src_t, dst_t)) return true;
+  if (op == SPH_MEDIAN && !slide_net(op, p.n, dst_t) && launch_sphere_hist8(st, p, src_t, dst_t)) return true;
   if (p.ex < 0 || p.ey < 0 || p.ez < 0 || 2 * p.ex > kSlideRL - 256 || p.ey > kSlideMaxEY || p.n == 0 ||
       p.n > kSphereTileMaxOff)
     return false;
