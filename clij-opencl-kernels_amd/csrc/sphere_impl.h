@@ -195,12 +195,20 @@ struct SphereSlideParams {
 };
 static_assert(kSphereTileMaxOff % 8 == 0, "offset chunks of 8");
 
-// NET > 0 (median, dst of 8 or 16 bits, exactly NET offsets): the keys of
-// each pair of output rows are gathered once into 16-bit pairs and the
+// PK = 1: values travel as 16-bit keys and the ring word of row r holds
+// rows r and r+1 (low and high half), so one LDS read serves a pair of
+// output rows and v_pk_{min,max,add}_u16 handle both: min/max of 8/16-bit
+// integer sources (keys v - lo keep the order), the mean of byte sources
+// over at most 257 values (the key sum fits 16 bits; the reference's float
+// sum of integers is exact, so the integer sum gives its result), and the
+// median network.  NET > 0 (median, dst of 8 or 16 bits, exactly NET
+// offsets): the keys of each pair of output rows are gathered once and the
 // forgetful-selection network of the box median runs on them, instead of
 // bisection rounds over LDS.
-template <class TS, class TD, int OP, int TY, int NET = 0>
+template <class TS, class TD, int OP, int TY, int NET = 0, int PK = 0>
 __global__ __launch_bounds__(256) void sphere_slide_kernel(const SphereSlideParams p) {
+  static_assert(NET == 0 || PK == 1, "the median network reads packed row pairs");
+  static_assert(PK == 0 || TY % 2 == 0, "rows in pairs");
   constexpr int NMAX = TY + 2 * kSlideMaxEY;
   constexpr int RL = kSlideRL;
   extern __shared__ uint32_t ring[];
@@ -220,6 +228,7 @@ __global__ __launch_bounds__(256) void sphere_slide_kernel(const SphereSlidePara
   const int64_t xb = clamp64(x0 - ex + 256 + t, 0, p.W - 1);
   auto word = [&](TS v) -> uint32_t {
     if constexpr (OP == SPH_MEDIAN) return value_key<TD>(gather_value<TS, TD>(v, 0));
+    else if constexpr (PK) return (uint32_t)((int)v - (int)Pix<TS>::lo);
     else return __float_as_uint(static_cast<float>(v));
   };
   // raw values of one plane's rows, converted when written to LDS so that
@@ -241,10 +250,23 @@ __global__ __launch_bounds__(256) void sphere_slide_kernel(const SphereSlidePara
 #pragma unroll
     for (int r = 0; r < NMAX; ++r) {
       if (r < nrow) {
-        s[r * RL] = word(va[r]);
-        if (two) s[r * RL + 256] = word(vb[r]);
+        if constexpr (PK) {
+          const int r1 = r + 1 < NMAX ? r + 1 : r;  // row r + 1 (the last row's high half is never read)
+          const bool has1 = r + 1 < nrow;
+          s[r * RL] = word(va[r]) | (has1 ? word(va[r1]) << 16 : 0u);
+          if (two) s[r * RL + 256] = word(vb[r]) | (has1 ? word(vb[r1]) << 16 : 0u);
+        } else {
+          s[r * RL] = word(va[r]);
+          if (two) s[r * RL + 256] = word(vb[r]);
+        }
       }
     }
+  };
+  auto pair_at = [](const uint32_t* b, int jp) -> us2 {
+    const uint32_t w = b[(2 * jp) * RL];
+    us2 u;
+    __builtin_memcpy(&u, &w, 4);
+    return u;
   };
   for (int i = 0; i <= 2 * ez; ++i) {
     fetch(z0 - ez + i);
@@ -303,11 +325,7 @@ __global__ __launch_bounds__(256) void sphere_slide_kernel(const SphereSlidePara
       for (int jp = 0; jp < TY / 2; ++jp) {
         us2 a[NET];
 #pragma unroll
-        for (int i = 0; i < NET; ++i) {
-          const uint32_t* b = base(p.off[i]);
-          const uint32_t w = b[(2 * jp) * RL] | (b[(2 * jp + 1) * RL] << 16);
-          __builtin_memcpy(&a[i], &w, 4);
-        }
+        for (int i = 0; i < NET; ++i) a[i] = pair_at(base(p.off[i]), jp);
         const us2 m = forgetful_median<NET>(a);
         store(2 * jp, static_cast<TD>(key_value<TD>(m.x)));
         store(2 * jp + 1, static_cast<TD>(key_value<TD>(m.y)));
@@ -342,6 +360,36 @@ __global__ __launch_bounds__(256) void sphere_slide_kernel(const SphereSlidePara
       }
 #pragma unroll
       for (int j = 0; j < TY; ++j) store(j, static_cast<TD>(key_value<TD>(lo[j])));
+    } else if constexpr (OP == SPH_MEAN && PK) {
+      us2 sum[TY / 2];
+#pragma unroll
+      for (int jp = 0; jp < TY / 2; ++jp) sum[jp] = us2{0, 0};
+      walk([&](const uint32_t* b) {
+#pragma unroll
+        for (int jp = 0; jp < TY / 2; ++jp) sum[jp] += pair_at(b, jp);
+      });
+      const int bias = p.noff * (int)Pix<TS>::lo;
+#pragma unroll
+      for (int jp = 0; jp < TY / 2; ++jp) {
+        store(2 * jp, convert_sat<TD>(static_cast<float>((int)sum[jp].x + bias) / static_cast<float>(p.noff)));
+        store(2 * jp + 1, convert_sat<TD>(static_cast<float>((int)sum[jp].y + bias) / static_cast<float>(p.noff)));
+      }
+    } else if constexpr (PK) {
+      // min / max of the keys, the centre first
+      us2 m[TY / 2];
+      const uint32_t* c = base((uint32_t)(ey * RL + ex) | ((uint32_t)ez << 16));
+#pragma unroll
+      for (int jp = 0; jp < TY / 2; ++jp) m[jp] = pair_at(c, jp);
+      walk([&](const uint32_t* b) {
+#pragma unroll
+        for (int jp = 0; jp < TY / 2; ++jp) m[jp] = OP == SPH_MIN ? vmin(m[jp], pair_at(b, jp)) : vmax(m[jp], pair_at(b, jp));
+      });
+      constexpr int lo = (int)Pix<TS>::lo;
+#pragma unroll
+      for (int jp = 0; jp < TY / 2; ++jp) {
+        store(2 * jp, convert_sat<TD>(static_cast<float>((int)m[jp].x + lo)));
+        store(2 * jp + 1, convert_sat<TD>(static_cast<float>((int)m[jp].y + lo)));
+      }
     } else if constexpr (OP == SPH_MEAN) {
       float sum[TY];
 #pragma unroll

@@ -2,6 +2,8 @@
 // sphere_impl.h): planning and the typed launches.
 #include <string.h>
 
+#include <type_traits>
+
 #include "clk_internal.h"
 #include "clk_tuning.h"
 #include "sphere_impl.h"
@@ -18,22 +20,51 @@ inline bool slide_net(int op, int n, int dst_t) {
 
 template <class TS, class TD, int TY>
 void slide_op(int op, hipStream_t st, const SphereSlideParams& sp, dim3 grid, size_t shmem) {
+#define CLK_SLIDE(OP, NET, PK) \
+  hipLaunchKernelGGL((sphere_slide_kernel<TS, TD, OP, TY, NET, PK>), grid, dim3(256), shmem, st, sp)
   if constexpr (sizeof(TD) <= 2) {
     if (op == SPH_MEDIAN && sp.noff == 7) {
-      hipLaunchKernelGGL((sphere_slide_kernel<TS, TD, SPH_MEDIAN, TY, 7>), grid, dim3(256), shmem, st, sp);
+      CLK_SLIDE(SPH_MEDIAN, 7, 1);
       return;
     }
     if (op == SPH_MEDIAN && sp.noff == 33) {
-      hipLaunchKernelGGL((sphere_slide_kernel<TS, TD, SPH_MEDIAN, TY, 33>), grid, dim3(256), shmem, st, sp);
+      CLK_SLIDE(SPH_MEDIAN, 33, 1);
       return;
     }
   }
-  switch (op) {
-    case SPH_MIN: hipLaunchKernelGGL((sphere_slide_kernel<TS, TD, SPH_MIN, TY>), grid, dim3(256), shmem, st, sp); break;
-    case SPH_MAX: hipLaunchKernelGGL((sphere_slide_kernel<TS, TD, SPH_MAX, TY>), grid, dim3(256), shmem, st, sp); break;
-    case SPH_MEAN: hipLaunchKernelGGL((sphere_slide_kernel<TS, TD, SPH_MEAN, TY>), grid, dim3(256), shmem, st, sp); break;
-    default: hipLaunchKernelGGL((sphere_slide_kernel<TS, TD, SPH_MEDIAN, TY>), grid, dim3(256), shmem, st, sp); break;
+  if constexpr (std::is_integral<TS>::value) {
+    // 8/16-bit integer sources: packed row pairs for min/max, and for the
+    // mean of bytes while the key sum fits 16 bits
+    if (op == SPH_MIN) {
+      CLK_SLIDE(SPH_MIN, 0, 1);
+      return;
+    }
+    if (op == SPH_MAX) {
+      CLK_SLIDE(SPH_MAX, 0, 1);
+      return;
+    }
+    if constexpr (sizeof(TS) == 1) {
+      if (op == SPH_MEAN && sp.noff <= 257) {
+        CLK_SLIDE(SPH_MEAN, 0, 1);
+        return;
+      }
+    }
+  } else {
+    if (op == SPH_MIN) {
+      CLK_SLIDE(SPH_MIN, 0, 0);
+      return;
+    }
+    if (op == SPH_MAX) {
+      CLK_SLIDE(SPH_MAX, 0, 0);
+      return;
+    }
   }
+  if (op == SPH_MEAN) {
+    CLK_SLIDE(SPH_MEAN, 0, 0);
+    return;
+  }
+  CLK_SLIDE(SPH_MEDIAN, 0, 0);
+#undef CLK_SLIDE
 }
 
 template <class TS, class TD>
