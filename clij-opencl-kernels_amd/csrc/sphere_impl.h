@@ -180,7 +180,12 @@ bool launch_sphere_slide(int op, hipStream_t st, const SphereParams& p, int src_
 
 constexpr int kSlideRL = 264;       // row pitch in words: 256 + 2*ex for ex <= 4
 constexpr int kSlideMaxEY = 3;
-constexpr int kSlideMaxLDS = 64 * 1024;
+// LDS per block: up to 80 KB (two blocks per CU) buys 8 output rows per
+// block for 5x5x5-sized neighbourhoods, which beats 4 rows at three blocks
+// per CU while a step waits on its plane's loads; the median network, whose
+// registers already limit it to three waves per SIMD, keeps 64 KB
+constexpr int kSlideMaxLDS = 80 * 1024;
+constexpr int kSlideMaxLDSNet = 64 * 1024;
 
 struct SphereSlideParams {
   const void* src;

@@ -153,9 +153,10 @@ bool launch_sphere_slide(int op, hipStream_t st, const SphereParams& p, int src_
     return false;
   int ty = 0;
   size_t shmem = 0;
+  const size_t cap = slide_net(op, p.n, dst_t) ? (size_t)kSlideMaxLDSNet : (size_t)kSlideMaxLDS;
   for (int c : {8, 4}) {
     const size_t b = (size_t)(2 * p.ez + 2) * (size_t)(c + 2 * p.ey) * kSlideRL * 4;
-    if (b <= (size_t)kSlideMaxLDS) {
+    if (b <= cap) {
       ty = c;
       shmem = b;
       break;
