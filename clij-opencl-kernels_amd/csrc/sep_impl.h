@@ -72,6 +72,23 @@ __device__ __forceinline__ void load_vt(const TS* __restrict__ p, VT (&out)[COLS
   }
 }
 
+// Loads of the static-width y/z pass: float sources stream with
+// non-temporal loads (measured faster for float32 passes over volumes of
+// gigabytes); 8/16-bit sources keep cached loads, whose L2 hits on the halo
+// rows of neighbouring segments they need.  The generic strided body, which
+// the batch engine runs over many small volumes, keeps cached loads for all
+// types (non-temporal ones made a batch of 256 small volumes 12% slower).
+template <class TS, class VT, int COLS>
+__device__ __forceinline__ void load_vt_stream(const TS* __restrict__ p, VT (&out)[COLS]) {
+  if constexpr (!Pix<TS>::kInt) {
+    const PixVec<TS, COLS> v = load_nt<TS, COLS>(p);
+#pragma unroll
+    for (int i = 0; i < COLS; ++i) out[i] = static_cast<VT>(v.v[i]);
+  } else {
+    load_vt<TS, VT, COLS>(p, out);
+  }
+}
+
 template <int OP, bool kFloatSrc, class VT>
 __device__ __forceinline__ VT tap_apply(VT acc, VT x, float h) {
   if constexpr (OP == OP_GAUSS) {

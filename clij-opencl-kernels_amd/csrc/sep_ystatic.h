@@ -54,17 +54,7 @@ __global__ __launch_bounds__(256) void sep_ystatic_kernel(const SepParams p) {
 
   auto ldp = [&](int pos, VT (&dst)[COLS]) {
     pos = pos < 0 ? 0 : (pos > L - 1 ? L - 1 : pos);
-    // float sources stream with non-temporal loads (measured faster for the
-    // float32 Gaussian z pass); 8/16-bit sources keep cached loads, whose L2
-    // hits on the halo rows of neighbouring segments they need
-    const TS* q = reinterpret_cast<const TS*>(ls0 + (int64_t)pos * ss + lso);
-    if constexpr (!Pix<TS>::kInt) {
-      const PixVec<TS, COLS> v = load_nt<TS, COLS>(q);
-#pragma unroll
-      for (int i = 0; i < COLS; ++i) dst[i] = static_cast<VT>(v.v[i]);
-    } else {
-      load_vt<TS, VT, COLS>(q, dst);
-    }
+    load_vt_stream<TS, VT, COLS>(reinterpret_cast<const TS*>(ls0 + (int64_t)pos * ss + lso), dst);
   };
 
   VT win[WN][COLS];
