@@ -735,6 +735,35 @@ def test_maximum_z_projection_integer_depth_split(gpu, shape):
                           "projection %s %s->%s" % (shape, DTYPE_NAMES[src_t], DTYPE_NAMES[dst_t]))
 
 
+@pytest.mark.parametrize("shape", [(1, 1, 1), (1, 1, 5), (3, 1, 1), (1, 4, 1), (2, 2, 2), (1, 2, 600)])
+def test_neighbourhood_kernels_degenerate_shapes(gpu, shape):
+    """Single pixels, rows, columns and planes through the sliding kernels
+    (packed medians, ellipsoid filters, detect_maxima, projection), where
+    every clamp collapses onto the volume's edges."""
+    import clkmi
+    a = random_volume(shape, np.uint8, seed=sum(shape))
+    ts, si = to_device(a)
+    got = gpu_median(a, np.uint8, 3, 3, 3)
+    assert_parity(got, O.median_box_3d(a, np.uint8, 3, 3, 3), "median 3x3x3 %s" % (shape,))
+    td, di = empty_device(a.shape, np.uint8)
+    clkmi.median_slice_by_slice_box_3d(di, si, 3, 3)
+    assert_parity(from_device(td, a.shape, np.uint8), O.median_slice_by_slice_box_3d(a, np.uint8, 3, 3),
+                  "slice median %s" % (shape,))
+    for name in SPHERE_OPS:
+        for Ns in ((3, 3, 3), (5, 5, 5)):
+            td, di = empty_device(a.shape, np.uint8)
+            getattr(clkmi, name)(di, si, *Ns)
+            assert_parity(from_device(td, a.shape, np.uint8), O.sphere_3d(name, a, np.uint8, *Ns),
+                          "%s %s %s" % (name, Ns, shape))
+    td, di = empty_device(a.shape, np.uint8)
+    clkmi.detect_maxima_3d(si, di)
+    assert_parity(from_device(td, a.shape, np.uint8), O.detect_maxima_3d(a, np.uint8), "detect_maxima %s" % (shape,))
+    tp, dp = empty_device((1,) + a.shape[1:], np.uint8)
+    clkmi.maximum_z_projection(dp, si)
+    assert_parity(from_device(tp, (1,) + a.shape[1:], np.uint8), O.maximum_z_projection(a, np.uint8),
+                  "projection %s" % (shape,))
+
+
 def test_consumers_by_name_and_errors(gpu):
     import clkmi
     a = random_volume((4, 9, 33), np.uint16, seed=2)
