@@ -350,7 +350,7 @@ __device__ __forceinline__ int bin8(float v) {
   return static_cast<int>(v) - static_cast<int>(Pix<TD>::lo);
 }
 
-constexpr int kHistZSeg = 32;  // outputs per lane along z
+constexpr int kHistZSeg = 64;  // outputs per lane along z
 
 // Shared by the histogram median bodies: 256 byte counters per lane packed
 // four to a dword; dword d of lane l is stored at d*64 + l, so every lane
@@ -375,6 +375,22 @@ struct Hist8Lane {
     const uint32_t one = 1u << ((key >> 3) & 31);
     __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(h) + (key & 0x3f00)),
                            inc ? one : 0u - one, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  // The first median of a segment: whole counter dwords (four bins, summed
+  // by v_sad_u8 against 0) are skipped while they stay at or below rank k,
+  // then the bins of the dword that crosses it are resolved in registers.
+  __device__ __forceinline__ void walk_from_zero(int& med, int& lt, int k) const {
+    int d = 0;
+    lt = 0;
+    uint32_t w = h[0];
+    for (int s = (int)__builtin_amdgcn_sad_u8(w, 0u, 0u); lt + s <= k;
+         w = h[++d * 64], s = (int)__builtin_amdgcn_sad_u8(w, 0u, 0u))
+      lt += s;
+    med = d * 4;
+    for (int c = (int)(w & 255u); lt + c <= k; c = (int)((w >> (8 * (med & 3))) & 255u)) {
+      lt += c;
+      ++med;
+    }
   }
   // median = the bin m with lt(m) <= k < lt(m) + count(m); lt = number of
   // values below med on entry and on return
@@ -434,7 +450,7 @@ __device__ __forceinline__ void median_hist8_body(const MedParams& p, uint32_t b
       for (int dx = -ex; dx <= ex; ++dx) hl.add(bin_at(row, dx), true);
     }
   int med = 0, lt = 0;
-  hl.walk(med, lt, k);
+  hl.walk_from_zero(med, lt, k);
   TD* __restrict__ dst = static_cast<TD*>(p.dst);
   const int64_t plane = p.H * p.W;
   dst[z0 * plane + y * p.W + x] = static_cast<TD>(med + static_cast<int>(Pix<TD>::lo));
@@ -542,7 +558,7 @@ __device__ __forceinline__ void median_hist8_fixed_body(const MedParams& p, uint
     }
   }
   int med = 0, lt = 0;
-  hl.walk(med, lt, k);
+  hl.walk_from_zero(med, lt, k);
   TD* __restrict__ dst = static_cast<TD*>(p.dst);
   const int64_t plane = p.H * p.W;
   if (valid) dst[z0 * plane + y * p.W + x] = static_cast<TD>(med + static_cast<int>(Pix<TD>::lo));

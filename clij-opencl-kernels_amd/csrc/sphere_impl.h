@@ -494,7 +494,7 @@ __global__ __launch_bounds__(64) void sphere_median_hist8_kernel(const SphereHis
     if (c < p.ncol)
       for (int dz = -hh[c]; dz <= hh[c]; ++dz) hl.add_key(key_of(at(c, z0 + dz)), true);
   int med = 0, lt = 0;
-  hl.walk(med, lt, k);
+  hl.walk_from_zero(med, lt, k);
   TD* __restrict__ dst = static_cast<TD*>(p.dst);
   dst[z0 * PS + y * p.W + x] = static_cast<TD>(med + static_cast<int>(Pix<TD>::lo));
   struct Step {
