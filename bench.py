@@ -458,8 +458,8 @@ def main():
         rocprof_summary(args.rocprof_summary)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: the filters shard by z-slab or by volume, and this "
-              "build has no multi-GPU path (see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the filters would shard by z-slab (with halo planes) or "
+              "by volume, but the library has no multi-GPU path. Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import torch
