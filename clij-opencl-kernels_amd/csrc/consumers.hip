@@ -176,18 +176,58 @@ __global__ __launch_bounds__(256) void max_z_projection_int_kernel(const TS* __r
   dst[y * W + x] = convert_sat<TD>(static_cast<float>(r));  // :18, CONVERT_dst_max
 }
 
-template <class TS, class TD>
-__global__ __launch_bounds__(256) void max_z_projection_kernel(const TS* __restrict__ src, TD* __restrict__ dst,
-                                                               int64_t W, int64_t H, int64_t D) {
-  const int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x;
+// maximum_z_projection of float sources, depth split exactly.  The
+// reference's chain (the first plane's value, then every strictly greater
+// one) ends at the first occurrence of the largest non-NaN value, unless the
+// first value is NaN, which nothing replaces.  Wave 0 runs that chain on the
+// first quarter of the planes; waves 1..3 reduce their quarters to the first
+// occurrence of the largest non-NaN value (NaN when there is none).  Combined
+// in z order with the same strict comparison — a NaN running value stays, a
+// NaN part never wins, an equal part leaves the earlier value (-0 / +0) —
+// the parts give the chain's result bit for bit.
+template <class TD>
+__global__ __launch_bounds__(256) void max_z_projection_f32_kernel(const float* __restrict__ src, TD* __restrict__ dst,
+                                                                   int64_t W, int64_t H, int64_t D, int32_t vec) {
+  __shared__ float part[4][256];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t x0 = (int64_t)blockIdx.x * 256 + lane * 4;
   const int64_t y = blockIdx.y;
-  if (x >= W) return;
-  float m = 0;  // :9
-  for (int64_t z = 0; z < D; ++z) {
-    const float v = static_cast<float>(src[(z * H + y) * W + x]);
-    if (v > m || z == 0) m = v;  // :14
+  const int64_t zc = (D + 3) / 4;
+  const int64_t za = wv * zc, zb = (za + zc < D) ? za + zc : D;
+  const float none = __builtin_nanf("");
+  float m[4] = {none, none, none, none};
+  const bool full = vec && x0 + 4 <= W;
+  const int64_t plane = H * W;
+  const float* col = src + y * W + x0;
+  auto take = [&](int j, float v, bool first) {
+    if (first) m[j] = v;  // :14, the first plane
+    else if (v > m[j] || (m[j] != m[j] && wv != 0)) m[j] = v;  // wave 0: strict chain; others: skip "none"
+  };
+#pragma unroll 4
+  for (int64_t z = za; z < zb; ++z) {
+    const float* p = col + z * plane;
+    const bool first = z == 0;
+    if (full) {
+      const PixVec<float, 4> v = load_nt<float, 4>(p);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) take(j, v.v[j], first);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (x0 + j < W) take(j, p[j], first);
+    }
   }
-  dst[y * W + x] = convert_sat<TD>(m);  // :18, CONVERT_dst_max
+#pragma unroll
+  for (int j = 0; j < 4; ++j) part[wv][lane * 4 + j] = m[j];
+  __syncthreads();
+  const int t = threadIdx.x;
+  const int64_t x = (int64_t)blockIdx.x * 256 + t;
+  if (x >= W) return;
+  float r = part[0][t];
+#pragma unroll
+  for (int w = 1; w < 4; ++w)
+    if (part[w][t] > r) r = part[w][t];
+  dst[y * W + x] = convert_sat<TD>(r);  // :18, CONVERT_dst_max
 }
 
 template <class TS, class TD>
@@ -213,8 +253,9 @@ void launch_proj(hipStream_t st, const clk_image& dst, const clk_image& src) {
                        static_cast<TD*>(dst.ptr), src.width, src.height, src.depth, vec);
     return;
   }
-  hipLaunchKernelGGL((max_z_projection_kernel<TS, TD>), grid, dim3(256), 0, st, static_cast<const TS*>(src.ptr),
-                     static_cast<TD*>(dst.ptr), src.width, src.height, src.depth);
+  const int32_t vec = src.width % 4 == 0 && (uintptr_t)src.ptr % 16 == 0;
+  hipLaunchKernelGGL((max_z_projection_f32_kernel<TD>), grid, dim3(256), 0, st, static_cast<const float*>(src.ptr),
+                     static_cast<TD*>(dst.ptr), src.width, src.height, src.depth, vec);
 }
 
 template <class TS>

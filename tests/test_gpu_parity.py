@@ -720,19 +720,32 @@ def test_detect_maxima_integer_sliding(gpu, src_t):
 
 
 @pytest.mark.parametrize("shape", [(37, 5, 131), (1, 3, 260), (3, 4, 64), (130, 2, 300)])
-def test_maximum_z_projection_integer_depth_split(gpu, shape):
-    """Integer sources split the depth over four waves: depths below and
-    above 4, widths that are not a multiple of 4 (element loads) and a
-    partial last 256-pixel block."""
+def test_maximum_z_projection_depth_split(gpu, shape):
+    """The depth is split over four waves: depths below and above 4, widths
+    that are not a multiple of 4 (element loads) and a partial last
+    256-pixel block.  Float data carries NaN in the first plane and at the
+    start of a quarter, and columns of zeros that begin with -0 (the
+    reference keeps the first of equal values)."""
     import clkmi
-    for src_t in (np.int8, np.uint8, np.int16, np.uint16):
+    for src_t in (np.int8, np.uint8, np.int16, np.uint16, np.float32):
         a = random_volume(shape, src_t, seed=shape[0] + 3)
+        if src_t == np.float32:
+            D = shape[0]
+            a[0, :, ::7] = np.nan
+            a[min((D + 3) // 4, D - 1), :, 3::11] = np.nan
+            a[:, :, 5::13] = 0.0
+            a[0, :, 5::13] = -0.0
+            a[:, 1:, 6::17] = -5.0
+            a[-1, 1:, 6::17] = np.nan
         ts, si = to_device(a)
         for dst_t in (np.uint8, np.int16, np.float32):
             tp, dp = empty_device((1,) + a.shape[1:], dst_t)
             clkmi.maximum_z_projection(dp, si)
-            assert_parity(from_device(tp, (1,) + a.shape[1:], dst_t), O.maximum_z_projection(a, dst_t),
-                          "projection %s %s->%s" % (shape, DTYPE_NAMES[src_t], DTYPE_NAMES[dst_t]))
+            got, want = from_device(tp, (1,) + a.shape[1:], dst_t), O.maximum_z_projection(a, dst_t)
+            assert_parity(got, want, "projection %s %s->%s" % (shape, DTYPE_NAMES[src_t], DTYPE_NAMES[dst_t]))
+            if dst_t == np.float32:
+                # the chain keeps the first of equal values: -0 and +0 differ in bits
+                assert np.array_equal(np.signbit(got), np.signbit(want)), "projection %s signs" % (shape,)
 
 
 @pytest.mark.parametrize("shape", [(1, 1, 1), (1, 1, 5), (3, 1, 1), (1, 4, 1), (2, 2, 2), (1, 2, 600)])
