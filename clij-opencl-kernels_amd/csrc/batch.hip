@@ -68,6 +68,7 @@ __global__ __launch_bounds__(256) void batch_median_kernel(const MedParams* __re
   if constexpr (KIND == MED_PK8) median_27_pk8_slide_body<TS, TD>(p, bx, by, bz);
   else if constexpr (KIND == MED_PK9) median_9_pk8_slide_body<TS, TD>(p, bx, by, bz);
   else if constexpr (KIND == MED_27) median_27_body<TS, TD>(p, bx, by, bz);
+  else if constexpr (KIND >= MED_REGS) median_regs_body<TS, TD, KIND - MED_REGS>(p, bx, by, bz);
   else median_generic_body<TS, TD>(p, bx, by, bz);
 }
 
@@ -148,6 +149,13 @@ void launch_median_class(hipStream_t st, const ClassKey& k, const ClassWork& w, 
     case MED_PK9: hipLaunchKernelGGL((batch_median_kernel<TS, TD, MED_PK9>), dim3(nt), dim3(256), 0, st, ms, tiles); break;
     case MED_HIST8: hipLaunchKernelGGL((batch_hist8_kernel<TS, TD>), dim3(nt), dim3(64), 0, st, ms, tiles); break;
     case MED_27: hipLaunchKernelGGL((batch_median_kernel<TS, TD, MED_27>), dim3(nt), dim3(256), 0, st, ms, tiles); break;
+#define CLK_MED_TIER(I)                                                                                     \
+  case MED_REGS + I:                                                                                        \
+    if constexpr (sizeof(TD) >= 2)                                                                          \
+      hipLaunchKernelGGL((batch_median_kernel<TS, TD, MED_REGS + I>), dim3(nt), dim3(256), 0, st, ms, tiles); \
+    break;
+    CLK_MED_TIER(0) CLK_MED_TIER(1) CLK_MED_TIER(2) CLK_MED_TIER(3) CLK_MED_TIER(4) CLK_MED_TIER(5)
+#undef CLK_MED_TIER
     default: hipLaunchKernelGGL((batch_median_kernel<TS, TD, MED_GENERIC>), dim3(nt), dim3(256), 0, st, ms, tiles); break;
   }
 }

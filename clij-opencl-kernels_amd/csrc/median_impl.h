@@ -19,9 +19,12 @@
 //    tracked median a few bins (Huang's running median, along z).  Counter
 //    dword d of lane l sits at d*64 + l, so every lane stays in its own
 //    LDS bank.
-//  * any other box: a bisection over the dst_t value range counts, per probe,
-//    how many window values are <= the probe (8, 16 or 32 probes).  No array,
-//    so no MAX_ARRAY_SIZE limit.
+//  * any other box: a bisection between the window's smallest and largest
+//    key counts, per probe, how many window values are <= the probe (at most
+//    8, 16 or 32 probes; a count of exactly n/2 + 1 ends it, the result being
+//    the largest value <= that probe).  No array, so no MAX_ARRAY_SIZE limit.  Boxes within 7x7x1 or 5x5x5 (16- and 32-bit
+//    dst) keep the window's keys in registers, read once, and search only
+//    between the window's smallest and largest key.
 //
 // Integer pixel types are exact.  For float dst the value of the rank-n/2
 // element is reproduced; a zero median is returned as +0 and NaN inputs have
@@ -89,26 +92,108 @@ __device__ __forceinline__ void median_generic_body(const MedParams& p, uint32_t
   const int64_t z = bz;
   if (x >= p.W) return;
   const TS* __restrict__ src = static_cast<const TS*>(p.src);
-  const int64_t k = p.n / 2;  // rank of the result
-  uint32_t lo = 0;
-  uint32_t hi = (KeyBits<TD>::v == 32) ? 0xffffffffu : ((1u << KeyBits<TD>::v) - 1u);
-  while (lo < hi) {
-    const uint32_t mid = lo + ((hi - lo) >> 1);
-    int64_t cnt = 0;
+  // f(key) for every value of the window
+  auto each = [&](auto&& f) {
     for (int dz = -p.ez; dz <= p.ez; ++dz) {
       const int64_t zz = clamp64(z + dz, 0, p.D - 1);
       for (int dy = -p.ey; dy <= p.ey; ++dy) {
         const int64_t yy = clamp64(y + dy, 0, p.H - 1);
         const TS* rowp = src + (zz * p.H + yy) * p.W;
-        for (int dx = -p.ex; dx <= p.ex; ++dx) {
-          const int64_t xx = clamp64(x + dx, 0, p.W - 1);
-          cnt += value_key<TD>(gather_value<TS, TD>(rowp[xx], p.wrap)) <= mid;
-        }
+        for (int dx = -p.ex; dx <= p.ex; ++dx) f(value_key<TD>(gather_value<TS, TD>(rowp[clamp64(x + dx, 0, p.W - 1)], p.wrap)));
       }
     }
-    if (cnt >= k + 1) hi = mid; else lo = mid + 1;
+  };
+  // The result r (rank k) lies between the window's smallest and largest
+  // key.  A probe whose count of keys <= it is exactly k + 1 ends the search:
+  // r is the largest of those keys; otherwise the search ends at lo == hi == r.
+  const int64_t k = p.n / 2;
+  uint32_t lo = 0xffffffffu, hi = 0u;
+  each([&](uint32_t v) {
+    lo = v < lo ? v : lo;
+    hi = v > hi ? v : hi;
+  });
+  while (lo < hi) {
+    const uint32_t mid = lo + ((hi - lo) >> 1);
+    int64_t cnt = 0;
+    each([&](uint32_t v) { cnt += v <= mid; });
+    if (cnt > k) {
+      hi = mid;
+      if (cnt == k + 1) break;
+    } else {
+      lo = mid + 1;
+    }
   }
-  static_cast<TD*>(p.dst)[(z * p.H + y) * p.W + x] = static_cast<TD>(key_value<TD>(lo));
+  uint32_t r = 0;
+  each([&](uint32_t v) { r = (v <= hi && v > r) ? v : r; });
+  static_cast<TD*>(p.dst)[(z * p.H + y) * p.W + x] = static_cast<TD>(key_value<TD>(r));
+}
+
+// Register tiers of the bisection median, by size: half extents (MX, MY, MZ)
+// of a box that holds the requested one.
+constexpr int kMedTiers = 6;
+constexpr int8_t kMedTier[kMedTiers][3] = {{1, 1, 0}, {2, 2, 0}, {1, 1, 1}, {3, 3, 0}, {2, 2, 1}, {2, 2, 2}};
+
+// The bisection of median_generic_body with the window read once: each
+// offset of tier I's box has its own register holding the value's key, and
+// every step counts over registers.  Offsets outside the requested box hold
+// 0xffffffff, which no step counts (a midpoint is always below the upper
+// bound, itself at most 0xffffffff).  The search starts from the window's
+// smallest and largest key, between which the result lies, and may end early
+// (below), so the result is the generic body's key for key.
+template <class TS, class TD, int I>
+__device__ __forceinline__ void median_regs_body(const MedParams& p, uint32_t bx, uint32_t by, uint32_t bz) {
+  constexpr int MX = kMedTier[I][0], MY = kMedTier[I][1], MZ = kMedTier[I][2];
+  constexpr int NX = 2 * MX + 1, NY = 2 * MY + 1, NS = NX * NY * (2 * MZ + 1);
+  const int64_t x = (int64_t)bx * 256 + threadIdx.x;
+  const int64_t y = by;
+  const int64_t z = bz;
+  if (x >= p.W) return;
+  const TS* __restrict__ src = static_cast<const TS*>(p.src);
+  int64_t xs[NX];
+#pragma unroll
+  for (int dx = -MX; dx <= MX; ++dx) xs[dx + MX] = clamp64(x + dx, 0, p.W - 1);
+  uint32_t key[NS];
+  uint32_t lo = 0xffffffffu, hi = 0u;
+#pragma unroll
+  for (int dz = -MZ; dz <= MZ; ++dz) {
+    const int64_t zz = clamp64(z + dz, 0, p.D - 1);
+#pragma unroll
+    for (int dy = -MY; dy <= MY; ++dy) {
+      const int64_t yy = clamp64(y + dy, 0, p.H - 1);
+      const TS* rowp = src + (zz * p.H + yy) * p.W;
+      const bool row_in = dz >= -p.ez && dz <= p.ez && dy >= -p.ey && dy <= p.ey;  // wave-uniform
+#pragma unroll
+      for (int dx = -MX; dx <= MX; ++dx) {
+        uint32_t k = 0xffffffffu;
+        if (row_in && dx >= -p.ex && dx <= p.ex) {
+          k = value_key<TD>(gather_value<TS, TD>(rowp[xs[dx + MX]], p.wrap));
+          lo = k < lo ? k : lo;
+          hi = k > hi ? k : hi;
+        }
+        key[((dz + MZ) * NY + dy + MY) * NX + dx + MX] = k;
+      }
+    }
+  }
+  // The result r (rank kth) stays in [lo, hi].  A step whose count of keys
+  // <= hi is exactly kth + 1 ends the search: r is then the largest of those
+  // keys.  Otherwise the search ends at lo == hi == r, the largest key <= r.
+  const int32_t kth = p.n / 2;
+  while (lo < hi) {
+    const uint32_t mid = lo + ((hi - lo) >> 1);
+    int32_t cnt = 0;
+#pragma unroll
+    for (int i = 0; i < NS; ++i) cnt += key[i] <= mid;
+    if (cnt > kth) {
+      hi = mid;
+      if (cnt == kth + 1) break;
+    } else {
+      lo = mid + 1;
+    }
+  }
+  uint32_t r = 0;
+#pragma unroll
+  for (int i = 0; i < NS; ++i) r = (key[i] <= hi && key[i] > r) ? key[i] : r;
+  static_cast<TD*>(p.dst)[(z * p.H + y) * p.W + x] = static_cast<TD>(key_value<TD>(r));
 }
 
 // Median (rank N/2) of N = 2k+1 values held in registers by forgetful
@@ -612,6 +697,11 @@ __global__ __launch_bounds__(64) void median_hist8_fixed_kernel(const MedParams 
   median_hist8_fixed_body<TS, TD, EX, EY>(p, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
+template <class TS, class TD, int I>
+__global__ __launch_bounds__(256) void median_regs_kernel(const MedParams p) {
+  median_regs_body<TS, TD, I>(p, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
 template <class TS, class TD>
 __global__ __launch_bounds__(256) void median_27_pk8_slide_kernel(const MedParams p) {
   median_27_pk8_slide_body<TS, TD>(p, blockIdx.x, blockIdx.y, blockIdx.z);
@@ -623,7 +713,20 @@ __global__ __launch_bounds__(256) void median_9_pk8_slide_kernel(const MedParams
 }
 
 
-enum MedKind : int { MED_PK8 = 0, MED_HIST8 = 1, MED_27 = 2, MED_GENERIC = 3, MED_PK9 = 4 };
+// MED_REGS + I: register tier I of the bisection median
+enum MedKind : int { MED_PK8 = 0, MED_HIST8 = 1, MED_27 = 2, MED_GENERIC = 3, MED_PK9 = 4, MED_REGS = 5 };
+
+// The smallest register tier whose box holds the requested one, when the
+// requested box fills at least a quarter of it (else the generic body, whose
+// steps re-read only the requested window, does less work); -1 if none.
+inline int median_tier(const MedParams& p) {
+  for (int i = 0; i < kMedTiers; ++i) {
+    if (p.ex > kMedTier[i][0] || p.ey > kMedTier[i][1] || p.ez > kMedTier[i][2]) continue;
+    const int ns = (2 * kMedTier[i][0] + 1) * (2 * kMedTier[i][1] + 1) * (2 * kMedTier[i][2] + 1);
+    return 4 * p.n >= ns ? i : -1;
+  }
+  return -1;
+}
 
 // Kernel and grid for one median launch; sets p.gx, p.gy.
 inline MedKind plan_median(MedParams& p, int td_size, dim3* grid, int* threads) {
@@ -638,7 +741,9 @@ inline MedKind plan_median(MedParams& p, int td_size, dim3* grid, int* threads) 
     *grid = dim3((uint32_t)((p.W + 63) / 64), (uint32_t)p.H, (uint32_t)((p.D + kHistZSeg - 1) / kHistZSeg));
     *threads = 64;
   } else {
-    k = (p.ex == 1 && p.ey == 1 && p.ez == 1) ? MED_27 : MED_GENERIC;
+    const int tier = td_size >= 2 ? median_tier(p) : -1;  // 8-bit dst: generic when the histogram is off
+    if (p.ex == 1 && p.ey == 1 && p.ez == 1) k = MED_27;
+    else k = tier >= 0 ? (MedKind)(MED_REGS + tier) : MED_GENERIC;
     *grid = dim3((uint32_t)((p.W + 255) / 256), (uint32_t)p.H, (uint32_t)p.D);
     *threads = 256;
   }

@@ -14,7 +14,7 @@ static int32_t median_typed(hipStream_t st, MedParams& p, std::string* err) {
   }
   dim3 grid;
   int threads;
-  switch (plan_median(p, (int)sizeof(TD), &grid, &threads)) {
+  switch ((int)plan_median(p, (int)sizeof(TD), &grid, &threads)) {
     case MED_PK8: {
       // single launches slide along z (runs of kPk8ZRun planes)
       const dim3 g(grid.x, grid.y, (uint32_t)((p.D + kPk8ZRun - 1) / kPk8ZRun));
@@ -43,6 +43,13 @@ static int32_t median_typed(hipStream_t st, MedParams& p, std::string* err) {
       }
       break;
     case MED_27: hipLaunchKernelGGL((median_27_kernel<TS, TD>), grid, dim3(threads), 0, st, p); break;
+#define CLK_MED_TIER(I)                                                                     \
+  case MED_REGS + I:                                                                        \
+    if constexpr (sizeof(TD) >= 2)                                                          \
+      hipLaunchKernelGGL((median_regs_kernel<TS, TD, I>), grid, dim3(threads), 0, st, p); \
+    break;
+      CLK_MED_TIER(0) CLK_MED_TIER(1) CLK_MED_TIER(2) CLK_MED_TIER(3) CLK_MED_TIER(4) CLK_MED_TIER(5)
+#undef CLK_MED_TIER
     default: hipLaunchKernelGGL((median_generic_kernel<TS, TD>), grid, dim3(threads), 0, st, p); break;
   }
   const hipError_t e = hipGetLastError();

@@ -86,6 +86,21 @@ def test_batch_packed_medians_slide(gpu):
         assert same_bits_or_value(g, oracle_job(*job)), "job %d oracle" % i
 
 
+def test_batch_median_register_tiers(gpu):
+    """Batched medians on every register tier (3x3x1 .. 5x5x5 boxes, 16- and
+    32-bit dst) and on a sparse box that keeps the generic search."""
+    jobs = []
+    radii = [(1, 1, 0), (2, 2, 0), (0, 1, 1), (3, 3, 0), (2, 2, 1), (2, 2, 2), (2, 1, 2), (0, 0, 2)]
+    for i, radius in enumerate(radii):
+        st, dt = [(np.uint16, np.uint16), (np.float32, np.float32), (np.int16, np.float32)][i % 3]
+        src = random_volume((9 + i, 11, 37 + 5 * i), st, seed=600 + i)
+        jobs.append(("median_box_3d", src, dt, (0.0, 0.0, 0.0), radius))
+    got = run_batch(jobs)
+    for i, (job, g) in enumerate(zip(jobs, got)):
+        assert same_bits_or_value(g, run_single(job)), "job %d single" % i
+        assert same_bits_or_value(g, oracle_job(*job)), "job %d oracle" % i
+
+
 @pytest.mark.parametrize("seed", [1, 2])
 def test_batch_matches_single_calls_and_oracle(gpu, seed):
     rng = np.random.default_rng(seed)

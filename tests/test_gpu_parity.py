@@ -122,6 +122,26 @@ def test_median_salt_and_pepper(gpu):
         assert_parity(got, want, "median r=%d" % r)
 
 
+@pytest.mark.parametrize("src_t", (np.uint8, np.int16, np.uint16, np.float32), ids=lambda t: DTYPE_NAMES[t])
+def test_median_register_tiers(gpu, src_t):
+    """Boxes within 7x7x1 or 5x5x5 and 16/32-bit dst keep the window in
+    registers (one tier per box size), boxes just outside every tier and
+    sparse ones (a quarter of the tier's box or less) use the generic search.
+    Float data carries -0 and +0 side by side, and values beyond the dst
+    range (saturating conversion)."""
+    src = random_volume((7, 13, 70), src_t, seed=91)
+    if src_t == np.float32:
+        src[:, ::3, ::5] = 0.0
+        src[:, 1::3, ::5] = -0.0
+        src[2, 4:9, 10:60] = 1e6
+    for Ns in ((3, 3, 1), (5, 5, 1), (7, 7, 1), (1, 3, 3), (3, 5, 3), (5, 5, 3), (5, 3, 5), (5, 5, 5),
+               (7, 5, 1), (1, 1, 5), (7, 7, 3), (3, 1, 1), (6, 4, 2)):
+        for dst_t in (np.int16, np.uint16, np.float32):
+            got = gpu_median(src, dst_t, *Ns)
+            assert_parity(got, O.median_box_3d(src, dst_t, *Ns),
+                          "median %s %s->%s" % (Ns, DTYPE_NAMES[src_t], DTYPE_NAMES[dst_t]))
+
+
 @pytest.mark.parametrize("src_t", (np.uint8, np.int8, np.float32), ids=lambda t: DTYPE_NAMES[t])
 def test_median_hist8_wide_rows(gpu, src_t):
     # 8-bit dst with a window beyond 3x3x3 takes the histogram kernel: rows of
