@@ -8,11 +8,12 @@
 // that element does not depend on how it is found, so the kernels here use
 // selection methods that fit registers and the VALU:
 //
-//  * 3x3x3 boxes: the 27 values sit in registers and a forgetful-selection
-//    network (repeatedly discard the minimum and maximum of a k+2 element
-//    buffer, then feed the next value) leaves the median; for 8- and 16-bit
-//    dst two voxels share each instruction (16-bit pairs), and single
-//    launches slide along z so that a step stages one new plane of rows.
+//  * 3x3x3 boxes: the 27 values sit in registers.  For 8- and 16-bit dst two
+//    voxels share each instruction (16-bit pairs) and Batcher's merge sort,
+//    pruned to its middle output, leaves the median; single launches slide
+//    along z so that a step stages one new plane of rows.  Float dst uses a
+//    forgetful-selection network (repeatedly discard the minimum and maximum
+//    of a k+2 element buffer, then feed the next value).
 //  * 8-bit dst and up to 255 values: each lane walks a z-column of outputs
 //    with a 256-bin histogram of its box in LDS (one byte per bin); a step
 //    in z removes the leaving plane, adds the entering one and moves the
@@ -22,9 +23,9 @@
 //  * any other box: a bisection between the window's smallest and largest
 //    key counts, per probe, how many window values are <= the probe (at most
 //    8, 16 or 32 probes; a count of exactly n/2 + 1 ends it, the result being
-//    the largest value <= that probe).  No array, so no MAX_ARRAY_SIZE limit.  Boxes within 7x7x1 or 5x5x5 (16- and 32-bit
-//    dst) keep the window's keys in registers, read once, and search only
-//    between the window's smallest and largest key.
+//    the largest value <= that probe).  No array, so no MAX_ARRAY_SIZE limit.
+//    Boxes within 7x7x1 or 5x5x5 (16- and 32-bit dst) keep the window's keys
+//    in registers, read once.
 //
 // Integer pixel types are exact.  For float dst the value of the rank-n/2
 // element is reproduced; a zero median is returned as +0 and NaN inputs have
@@ -236,6 +237,39 @@ __device__ __forceinline__ T forgetful_median(T (&a)[N]) {
   return vmax(vmin(x0, x1), vmin(vmax(x0, x1), x2));
 }
 
+// Rank N/2 of N 16-bit key pairs by Batcher's odd-even merge sort (Knuth,
+// TAOCP vol. 3, 5.3.4, algorithm M) on P = 2^ceil(log2 N) wires, wires N..P-1
+// held at 0xffff.  Only wire N/2 is used: the compiler folds every
+// comparator that meets a padding wire (min(x, 0xffff) = x) and drops those
+// whose results cannot reach wire N/2, leaving 226 min/max instructions for
+// N = 27 where forgetful selection needs 388.
+template <int N>
+__device__ __forceinline__ us2 batcher_median(const us2 (&a)[N]) {
+  constexpr int P = N <= 8 ? 8 : N <= 16 ? 16 : N <= 32 ? 32 : 64;
+  static_assert(N <= P && P <= 64, "at most 64 wires");
+  us2 w[P];
+#pragma unroll
+  for (int i = 0; i < P; ++i) w[i] = i < N ? a[i] : us2{0xffff, 0xffff};
+#pragma unroll
+  for (int p = 1; p < P; p *= 2) {
+#pragma unroll
+    for (int k = p; k >= 1; k /= 2) {
+#pragma unroll
+      for (int j = k % p; j < P - k; j += 2 * k) {
+#pragma unroll
+        for (int i = 0; i < k; ++i) {
+          if (i < P - j - k && (i + j) / (2 * p) == (i + j + k) / (2 * p)) {
+            const us2 u = w[i + j], v = w[i + j + k];
+            w[i + j] = vmin(u, v);
+            w[i + j + k] = vmax(u, v);
+          }
+        }
+      }
+    }
+  }
+  return w[N / 2];
+}
+
 // 3x3x3 median with an 8- or 16-bit dst: a block filters 512 consecutive x
 // of one row y for a run of kPk8ZRun planes.  Source values, converted to the
 // dst type and biased to unsigned 16-bit keys (signed types shift by -lo,
@@ -319,7 +353,7 @@ __device__ __forceinline__ void median_27_pk8_slide_body(const MedParams& p, uin
         __builtin_memcpy(&a[9 * dz + 3 * r + 2], &w2, 4);
       }
     }
-    const us2 m = forgetful_median<27>(a);
+    const us2 m = batcher_median<27>(a);
     TD* const row = dst + (z * p.H + y) * p.W;
     if (x < W) row[x] = static_cast<TD>((int)m.x - kBias);
     if (x + 1 < W) row[x + 1] = static_cast<TD>((int)m.y - kBias);

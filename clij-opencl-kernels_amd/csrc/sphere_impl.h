@@ -208,7 +208,7 @@ static_assert(kSphereTileMaxOff % 8 == 0, "offset chunks of 8");
 // sum of integers is exact, so the integer sum gives its result), and the
 // median network.  NET > 0 (median, dst of 8 or 16 bits, exactly NET
 // offsets): the keys of each pair of output rows are gathered once and the
-// forgetful-selection network of the box median runs on them, instead of
+// pruned Batcher network of the box median runs on them, instead of
 // bisection rounds over LDS.
 template <class TS, class TD, int OP, int TY, int NET = 0, int PK = 0>
 __global__ __launch_bounds__(256) void sphere_slide_kernel(const SphereSlideParams p) {
@@ -331,7 +331,7 @@ __global__ __launch_bounds__(256) void sphere_slide_kernel(const SphereSlidePara
         us2 a[NET];
 #pragma unroll
         for (int i = 0; i < NET; ++i) a[i] = pair_at(base(p.off[i]), jp);
-        const us2 m = forgetful_median<NET>(a);
+        const us2 m = batcher_median<NET>(a);
         store(2 * jp, static_cast<TD>(key_value<TD>(m.x)));
         store(2 * jp + 1, static_cast<TD>(key_value<TD>(m.y)));
       }
