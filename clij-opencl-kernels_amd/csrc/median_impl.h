@@ -8,12 +8,11 @@
 // that element does not depend on how it is found, so the kernels here use
 // selection methods that fit registers and the VALU:
 //
-//  * 3x3x3 boxes: the 27 values sit in registers.  For 8- and 16-bit dst two
-//    voxels share each instruction (16-bit pairs) and Batcher's merge sort,
-//    pruned to its middle output, leaves the median; single launches slide
-//    along z so that a step stages one new plane of rows.  Float dst uses a
-//    forgetful-selection network (repeatedly discard the minimum and maximum
-//    of a k+2 element buffer, then feed the next value).
+//  * 3x3x3 boxes: the 27 values sit in registers as order-preserving keys
+//    and Batcher's merge sort, pruned to its middle output, leaves the
+//    median.  For 8- and 16-bit dst two voxels share each instruction
+//    (16-bit pairs) and single launches slide along z so that a step stages
+//    one new plane of rows.
 //  * 8-bit dst and up to 255 values: each lane walks a z-column of outputs
 //    with a 256-bin histogram of its box in LDS (one byte per bin); a step
 //    in z removes the leaving plane, adds the entering one and moves the
@@ -25,7 +24,8 @@
 //    8, 16 or 32 probes; a count of exactly n/2 + 1 ends it, the result being
 //    the largest value <= that probe).  No array, so no MAX_ARRAY_SIZE limit.
 //    Boxes within 7x7x1 or 5x5x5 (16- and 32-bit dst) keep the window's keys
-//    in registers, read once.
+//    in registers, read once; a box that fills its register tier exactly
+//    takes the pruned network instead when its key range is wide.
 //
 // Integer pixel types are exact.  For float dst the value of the rank-n/2
 // element is reproduced; a zero median is returned as +0 and NaN inputs have
@@ -129,10 +129,94 @@ __device__ __forceinline__ void median_generic_body(const MedParams& p, uint32_t
   static_cast<TD*>(p.dst)[(z * p.H + y) * p.W + x] = static_cast<TD>(key_value<TD>(r));
 }
 
+// Median (rank N/2) of N = 2k+1 values held in registers by forgetful
+// selection: keep a buffer of k+2 values; its minimum and maximum cannot be
+// the median of the values seen so far plus the rest, so move them to the
+// buffer ends with compare-exchanges, drop both, and put the next unseen
+// value into the freed slot.  After k-1 rounds the median of the last 3 is
+// the answer.
+typedef unsigned short us2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ float vmin(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ float vmax(float a, float b) { return fmaxf(a, b); }
+// two 16-bit lanes per instruction (v_pk_min_u16 / v_pk_max_u16)
+__device__ __forceinline__ us2 vmin(us2 a, us2 b) { return __builtin_elementwise_min(a, b); }
+__device__ __forceinline__ us2 vmax(us2 a, us2 b) { return __builtin_elementwise_max(a, b); }
+
+template <int N, class T>
+__device__ __forceinline__ T forgetful_median(T (&a)[N]) {
+  constexpr int k = N / 2;
+  constexpr int m = k + 2;
+  constexpr int hi = m - 1;
+#pragma unroll
+  for (int step = 0; step < N - m; ++step) {
+    const int lo = step;
+#pragma unroll
+    for (int i = lo + 1; i <= hi; ++i) {
+      const T u = a[lo], w = a[i];
+      a[lo] = vmin(u, w);
+      a[i] = vmax(u, w);
+    }
+#pragma unroll
+    for (int i = lo + 1; i < hi; ++i) {
+      const T u = a[i], w = a[hi];
+      a[i] = vmin(u, w);
+      a[hi] = vmax(u, w);
+    }
+    a[hi] = a[m + step];
+  }
+  const T x0 = a[N - m], x1 = a[N - m + 1], x2 = a[hi];  // the last 3 (N - m + 2 == hi)
+  return vmax(vmin(x0, x1), vmin(vmax(x0, x1), x2));
+}
+
+__device__ __forceinline__ uint32_t vmin(uint32_t a, uint32_t b) { return a < b ? a : b; }
+__device__ __forceinline__ uint32_t vmax(uint32_t a, uint32_t b) { return a > b ? a : b; }
+
+// Rank N/2 of N keys (16-bit key pairs, or 32-bit keys) by Batcher's
+// odd-even merge sort (Knuth, TAOCP vol. 3, 5.3.4, algorithm M) on
+// P = 2^ceil(log2 N) wires, wires N..P-1 held at the largest key.  Only wire
+// N/2 is used: the compiler folds every comparator that meets a padding wire
+// (min(x, top) = x) and drops those whose results cannot reach wire N/2,
+// leaving 224 min/max instructions for N = 27 where forgetful selection
+// needs 364, and 2244 for N = 125.
+template <int N, class T>
+__device__ __forceinline__ T batcher_median(const T (&a)[N], const T top) {
+  constexpr int P = N <= 8 ? 8 : N <= 16 ? 16 : N <= 32 ? 32 : N <= 64 ? 64 : 128;
+  static_assert(N <= P, "at most 128 wires");
+  T w[P];
+#pragma unroll
+  for (int i = 0; i < P; ++i) w[i] = i < N ? a[i] : top;
+#pragma unroll
+  for (int p = 1; p < P; p *= 2) {
+#pragma unroll
+    for (int k = p; k >= 1; k /= 2) {
+#pragma unroll
+      for (int j = k % p; j < P - k; j += 2 * k) {
+#pragma unroll
+        for (int i = 0; i < k; ++i) {
+          if (i < P - j - k && (i + j) / (2 * p) == (i + j + k) / (2 * p)) {
+            const T u = w[i + j], v = w[i + j + k];
+            w[i + j] = vmin(u, v);
+            w[i + j + k] = vmax(u, v);
+          }
+        }
+      }
+    }
+  }
+  return w[N / 2];
+}
+
+template <int N>
+__device__ __forceinline__ us2 batcher_median(const us2 (&a)[N]) {
+  return batcher_median<N, us2>(a, us2{0xffff, 0xffff});
+}
+
 // Register tiers of the bisection median, by size: half extents (MX, MY, MZ)
 // of a box that holds the requested one.
 constexpr int kMedTiers = 6;
 constexpr int8_t kMedTier[kMedTiers][3] = {{1, 1, 0}, {2, 2, 0}, {1, 1, 1}, {3, 3, 0}, {2, 2, 1}, {2, 2, 2}};
+// min/max instructions of the pruned Batcher network on each tier's slots
+constexpr int16_t kMedTierNetOps[kMedTiers] = {40, 202, 226, 590, 1198, 2244};
 
 // The bisection of median_generic_body with the window read once: each
 // offset of tier I's box has its own register holding the value's key, and
@@ -175,6 +259,16 @@ __device__ __forceinline__ void median_regs_body(const MedParams& p, uint32_t bx
       }
     }
   }
+  TD* const out = static_cast<TD*>(p.dst) + (z * p.H + y) * p.W + x;
+  // A box that fills the tier has its rank at a compile-time position, so
+  // the pruned network applies; it is taken when the bisection would need
+  // more instructions (about two per slot for each bit of the key range:
+  // 8-bit data stays on the bisection, float data takes the network).
+  const int range_bits = 32 - __builtin_clz((hi - lo) | 1u);
+  if (p.ex == MX && p.ey == MY && p.ez == MZ && 2 * NS * range_bits > kMedTierNetOps[I]) {
+    *out = static_cast<TD>(key_value<TD>(batcher_median<NS, uint32_t>(key, 0xffffffffu)));
+    return;
+  }
   // The result r (rank kth) stays in [lo, hi].  A step whose count of keys
   // <= hi is exactly kth + 1 ends the search: r is then the largest of those
   // keys.  Otherwise the search ends at lo == hi == r, the largest key <= r.
@@ -194,80 +288,7 @@ __device__ __forceinline__ void median_regs_body(const MedParams& p, uint32_t bx
   uint32_t r = 0;
 #pragma unroll
   for (int i = 0; i < NS; ++i) r = (key[i] <= hi && key[i] > r) ? key[i] : r;
-  static_cast<TD*>(p.dst)[(z * p.H + y) * p.W + x] = static_cast<TD>(key_value<TD>(r));
-}
-
-// Median (rank N/2) of N = 2k+1 values held in registers by forgetful
-// selection: keep a buffer of k+2 values; its minimum and maximum cannot be
-// the median of the values seen so far plus the rest, so move them to the
-// buffer ends with compare-exchanges, drop both, and put the next unseen
-// value into the freed slot.  After k-1 rounds the median of the last 3 is
-// the answer.
-typedef unsigned short us2 __attribute__((ext_vector_type(2)));
-
-__device__ __forceinline__ float vmin(float a, float b) { return fminf(a, b); }
-__device__ __forceinline__ float vmax(float a, float b) { return fmaxf(a, b); }
-// two 16-bit lanes per instruction (v_pk_min_u16 / v_pk_max_u16)
-__device__ __forceinline__ us2 vmin(us2 a, us2 b) { return __builtin_elementwise_min(a, b); }
-__device__ __forceinline__ us2 vmax(us2 a, us2 b) { return __builtin_elementwise_max(a, b); }
-
-template <int N, class T>
-__device__ __forceinline__ T forgetful_median(T (&a)[N]) {
-  constexpr int k = N / 2;
-  constexpr int m = k + 2;
-  constexpr int hi = m - 1;
-#pragma unroll
-  for (int step = 0; step < N - m; ++step) {
-    const int lo = step;
-#pragma unroll
-    for (int i = lo + 1; i <= hi; ++i) {
-      const T u = a[lo], w = a[i];
-      a[lo] = vmin(u, w);
-      a[i] = vmax(u, w);
-    }
-#pragma unroll
-    for (int i = lo + 1; i < hi; ++i) {
-      const T u = a[i], w = a[hi];
-      a[i] = vmin(u, w);
-      a[hi] = vmax(u, w);
-    }
-    a[hi] = a[m + step];
-  }
-  const T x0 = a[N - m], x1 = a[N - m + 1], x2 = a[hi];  // the last 3 (N - m + 2 == hi)
-  return vmax(vmin(x0, x1), vmin(vmax(x0, x1), x2));
-}
-
-// Rank N/2 of N 16-bit key pairs by Batcher's odd-even merge sort (Knuth,
-// TAOCP vol. 3, 5.3.4, algorithm M) on P = 2^ceil(log2 N) wires, wires N..P-1
-// held at 0xffff.  Only wire N/2 is used: the compiler folds every
-// comparator that meets a padding wire (min(x, 0xffff) = x) and drops those
-// whose results cannot reach wire N/2, leaving 226 min/max instructions for
-// N = 27 where forgetful selection needs 388.
-template <int N>
-__device__ __forceinline__ us2 batcher_median(const us2 (&a)[N]) {
-  constexpr int P = N <= 8 ? 8 : N <= 16 ? 16 : N <= 32 ? 32 : 64;
-  static_assert(N <= P && P <= 64, "at most 64 wires");
-  us2 w[P];
-#pragma unroll
-  for (int i = 0; i < P; ++i) w[i] = i < N ? a[i] : us2{0xffff, 0xffff};
-#pragma unroll
-  for (int p = 1; p < P; p *= 2) {
-#pragma unroll
-    for (int k = p; k >= 1; k /= 2) {
-#pragma unroll
-      for (int j = k % p; j < P - k; j += 2 * k) {
-#pragma unroll
-        for (int i = 0; i < k; ++i) {
-          if (i < P - j - k && (i + j) / (2 * p) == (i + j + k) / (2 * p)) {
-            const us2 u = w[i + j], v = w[i + j + k];
-            w[i + j] = vmin(u, v);
-            w[i + j + k] = vmax(u, v);
-          }
-        }
-      }
-    }
-  }
-  return w[N / 2];
+  *out = static_cast<TD>(key_value<TD>(r));
 }
 
 // 3x3x3 median with an 8- or 16-bit dst: a block filters 512 consecutive x
@@ -440,7 +461,8 @@ __device__ __forceinline__ void median_27_body(const MedParams& p, uint32_t bx, 
   const int64_t z = bz;
   if (x >= p.W) return;
   const TS* __restrict__ src = static_cast<const TS*>(p.src);
-  float a[27];
+  // order-preserving keys (-0 and +0 are one key), one voxel per instruction
+  uint32_t a[27];
   int i = 0;
 #pragma unroll
   for (int dz = -1; dz <= 1; ++dz) {
@@ -452,15 +474,12 @@ __device__ __forceinline__ void median_27_body(const MedParams& p, uint32_t bx, 
 #pragma unroll
       for (int dx = -1; dx <= 1; ++dx) {
         const int64_t xx = clamp64(x + dx, 0, p.W - 1);
-        a[i++] = gather_value<TS, TD>(rowp[xx], p.wrap);
+        a[i++] = value_key<TD>(gather_value<TS, TD>(rowp[xx], p.wrap));
       }
     }
   }
-  float m = forgetful_median<27, float>(a);
-  if constexpr (!Pix<TD>::kInt) {
-    if (m == 0.f) m = 0.f;
-  }
-  static_cast<TD*>(p.dst)[(z * p.H + y) * p.W + x] = static_cast<TD>(m);
+  const uint32_t m = batcher_median<27, uint32_t>(a, 0xffffffffu);
+  static_cast<TD*>(p.dst)[(z * p.H + y) * p.W + x] = static_cast<TD>(key_value<TD>(m));
 }
 
 // 8-bit dst: histogram bin of a gathered value.
