@@ -300,7 +300,7 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
   std::vector<std::map<ClassKey, ClassWork>> classes(3);
   std::vector<Single> singles;
   void* temp_base = nullptr;
-  if (temp_bytes > 0 && hipMallocAsync(&temp_base, (size_t)temp_bytes, st) != hipSuccess) {
+  if (temp_bytes > 0 && temp_alloc(&temp_base, (size_t)temp_bytes, st) != hipSuccess) {
     *err = "batch temporaries: allocation failed";
     return CLK_MEM_OBJECT_ALLOCATION_FAILURE;
   }
@@ -418,7 +418,7 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
     }
   char* dev_tables = nullptr;
   if (tbytes > 0) {
-    if (hipMallocAsync((void**)&dev_tables, tbytes, st) != hipSuccess) {
+    if (temp_alloc((void**)&dev_tables, tbytes, st) != hipSuccess) {
       if (temp_base) (void)hipFreeAsync(temp_base, st);
       *err = "batch tables: allocation failed";
       return CLK_MEM_OBJECT_ALLOCATION_FAILURE;

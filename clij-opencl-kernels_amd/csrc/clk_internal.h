@@ -81,6 +81,13 @@ int32_t prepare_separable(int op, const clk_image& dst, const clk_image& src, in
                           float s, SepParams& p, int* axis_is_x, bool* fill_nan, std::string* err);
 int32_t launch_fill_nan(hipStream_t st, const clk_image& dst, std::string* err);
 
+// Stream-ordered temporaries (free them with hipFreeAsync) from a memory pool
+// of the library's own per device, which keeps freed memory for the next
+// call instead of handing it back to the driver at every synchronisation as
+// the default pool does (release threshold 0).  The pool holds at most the
+// largest set of temporaries in use at once until the process ends.
+hipError_t temp_alloc(void** p, size_t bytes, hipStream_t st);
+
 // detect_maxima_3d (detect = true; dst and src of one size) or
 // maximum_z_projection (dst W x H x 1); arguments validated by the caller.
 int32_t launch_consumer(bool detect, hipStream_t st, const clk_image& dst, const clk_image& src, std::string* err);

@@ -6,11 +6,48 @@
 #include <string.h>
 
 #include <cmath>
+#include <mutex>
 #include <string>
 
 #include "../../include/clkmi.h"
 #include "clk_device.h"
 #include "clk_internal.h"
+
+namespace clk {
+
+hipError_t temp_alloc(void** p, size_t bytes, hipStream_t st) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static hipMemPool_t pools[kMaxDevices] = {};
+  static bool no_pool[kMaxDevices] = {};
+  int dev = 0;
+  const hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  hipMemPool_t pool = nullptr;
+  if (dev >= 0 && dev < kMaxDevices) {
+    std::lock_guard<std::mutex> g(mu);
+    if (!pools[dev] && !no_pool[dev]) {
+      hipMemPoolProps props;
+      memset(&props, 0, sizeof(props));
+      props.allocType = hipMemAllocationTypePinned;
+      props.location.type = hipMemLocationTypeDevice;
+      props.location.id = dev;
+      hipMemPool_t np = nullptr;
+      if (hipMemPoolCreate(&np, &props) == hipSuccess) {
+        uint64_t keep = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(np, hipMemPoolAttrReleaseThreshold, &keep);
+        pools[dev] = np;
+      } else {
+        (void)hipGetLastError();
+        no_pool[dev] = true;  // the default pool serves this device
+      }
+    }
+    pool = pools[dev];
+  }
+  return pool ? hipMallocFromPoolAsync(p, bytes, pool, st) : hipMallocAsync(p, bytes, st);
+}
+
+}  // namespace clk
 
 namespace {
 
@@ -124,7 +161,7 @@ int32_t chain3(int op, const char* name, void* stream, const clk_image* dst, con
   void* tmp[2] = {nullptr, nullptr};
   const int ntmp = nact - 1 < 2 ? nact - 1 : 2;
   for (int i = 0; i < ntmp; ++i) {
-    if (V > 0 && hipMallocAsync(&tmp[i], (size_t)V * clk_dtype_size((op == clk::OP_MIN || op == clk::OP_MAX) ? src->dtype : CLK_FLOAT), st) != hipSuccess) {
+    if (V > 0 && clk::temp_alloc(&tmp[i], (size_t)V * clk_dtype_size((op == clk::OP_MIN || op == clk::OP_MAX) ? src->dtype : CLK_FLOAT), st) != hipSuccess) {
       for (int j = 0; j < i; ++j) (void)hipFreeAsync(tmp[j], st);
       return fail(CLK_MEM_OBJECT_ALLOCATION_FAILURE, std::string(name) + ": temporary allocation failed");
     }

@@ -211,7 +211,8 @@ int32_t clk_execute(void* stream, const char* kernel_name, const clk_arg* args,
 
 /* The reference leaves the 3-pass chain to its host.  These run the passes
  * x, y, z with float32 temporaries (allocated and freed by the library,
- * stream-ordered); an axis with sigma <= 0 (Gaussian) or radius 0 is a
+ * stream-ordered, from a per-device pool that keeps freed memory for the
+ * next call); an axis with sigma <= 0 (Gaussian) or radius 0 is a
  * converting copy.  N for a Gaussian axis is int(8*sigma) rounded up to
  * odd; for the box filters N = 2*radius+1. */
 int32_t clk_gaussian_blur_3d(void* stream, const clk_image* dst,
