@@ -300,7 +300,8 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
   std::vector<std::map<ClassKey, ClassWork>> classes(3);
   std::vector<Single> singles;
   void* temp_base = nullptr;
-  if (temp_bytes > 0 && temp_alloc(&temp_base, (size_t)temp_bytes, st) != hipSuccess) {
+  auto scratch_lk = stream_scratch_lock();  // until the last launch is issued
+  if (temp_bytes > 0 && stream_scratch(st, 1, (size_t)temp_bytes, &temp_base) != hipSuccess) {
     *err = "batch temporaries: allocation failed";
     return CLK_MEM_OBJECT_ALLOCATION_FAILURE;
   }
@@ -316,7 +317,6 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
       bool nan_fill = false;
       int32_t rc = prepare_separable(ps.op, ps.dst, ps.src, ps.dim, ps.N, ps.s, p, &axis_is_x, &nan_fill, err);
       if (rc) {
-        if (temp_base) (void)hipFreeAsync(temp_base, st);
         return rc;
       }
       const int64_t V = ps.dst.width * ps.dst.height * ps.dst.depth;
@@ -385,7 +385,7 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
     if (!batch_pair(jb.src.dtype, jb.dst.dtype)) {
       std::string e2;
       const int32_t rc = launch_median(st, jb.dst, jb.src, 2 * p.ex + 1, 2 * p.ey + 1, 2 * p.ez + 1, &e2);
-      if (rc) { *err = e2; if (temp_base) (void)hipFreeAsync(temp_base, st); return rc; }
+      if (rc) { *err = e2; return rc; }
       continue;
     }
     dim3 grid;
@@ -418,8 +418,7 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
     }
   char* dev_tables = nullptr;
   if (tbytes > 0) {
-    if (temp_alloc((void**)&dev_tables, tbytes, st) != hipSuccess) {
-      if (temp_base) (void)hipFreeAsync(temp_base, st);
+    if (stream_scratch(st, 2, tbytes, (void**)&dev_tables) != hipSuccess) {
       *err = "batch tables: allocation failed";
       return CLK_MEM_OBJECT_ALLOCATION_FAILURE;
     }
@@ -436,8 +435,6 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
       sg.host = nullptr;
       sg.cap = 0;
       if (hipHostMalloc(&sg.host, tbytes, hipHostMallocDefault) != hipSuccess) {
-        (void)hipFreeAsync(dev_tables, st);
-        if (temp_base) (void)hipFreeAsync(temp_base, st);
         *err = "batch tables: pinned staging allocation failed";
         return CLK_OUT_OF_RESOURCES;
       }
@@ -452,8 +449,6 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
         memcpy(h + w.tiles_off, w.tiles.data(), w.tiles.size() * sizeof(BatchTile));
       }
     if (hipMemcpyAsync(dev_tables, h, tbytes, hipMemcpyHostToDevice, st) != hipSuccess) {
-      (void)hipFreeAsync(dev_tables, st);
-      if (temp_base) (void)hipFreeAsync(temp_base, st);
       *err = "batch tables: upload failed";
       return CLK_OUT_OF_RESOURCES;
     }
@@ -487,8 +482,6 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
       rc = launch_separable(sg.pass.op, st, sg.pass.dst, sg.pass.src, sg.pass.dim, sg.pass.N, sg.pass.s, err);
     }
   }
-  if (dev_tables) (void)hipFreeAsync(dev_tables, st);
-  if (temp_base) (void)hipFreeAsync(temp_base, st);
   return rc;
 }
 

@@ -6,8 +6,10 @@
 #include <string.h>
 
 #include <cmath>
+#include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 
 #include "../../include/clkmi.h"
 #include "clk_device.h"
@@ -45,6 +47,33 @@ hipError_t temp_alloc(void** p, size_t bytes, hipStream_t st) {
     pool = pools[dev];
   }
   return pool ? hipMallocFromPoolAsync(p, bytes, pool, st) : hipMallocAsync(p, bytes, st);
+}
+
+namespace {
+std::recursive_mutex g_scratch_mu;
+std::map<std::tuple<int, hipStream_t, int>, std::pair<void*, size_t>> g_scratch;
+}  // namespace
+
+std::unique_lock<std::recursive_mutex> stream_scratch_lock() {
+  return std::unique_lock<std::recursive_mutex>(g_scratch_mu);
+}
+
+hipError_t stream_scratch(hipStream_t st, int slot, size_t bytes, void** p) {
+  std::lock_guard<std::recursive_mutex> g(g_scratch_mu);
+  int dev = 0;
+  const hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::pair<void*, size_t>& b = g_scratch[std::make_tuple(dev, st, slot)];
+  if (b.second < bytes) {
+    if (b.first) (void)hipFreeAsync(b.first, st);
+    b = {nullptr, 0};
+    void* q = nullptr;
+    const hipError_t ea = temp_alloc(&q, bytes, st);
+    if (ea != hipSuccess) return ea;
+    b = {q, bytes};
+  }
+  *p = b.first;
+  return hipSuccess;
 }
 
 }  // namespace clk
@@ -158,17 +187,20 @@ int32_t chain3(int op, const char* name, void* stream, const clk_image* dst, con
     return rc ? fail(rc, std::string(name) + ": " + err) : ok();
   }
   const int64_t V = voxels(*dst);
-  void* tmp[2] = {nullptr, nullptr};
-  const int ntmp = nact - 1 < 2 ? nact - 1 : 2;
-  for (int i = 0; i < ntmp; ++i) {
-    if (V > 0 && clk::temp_alloc(&tmp[i], (size_t)V * clk_dtype_size((op == clk::OP_MIN || op == clk::OP_MAX) ? src->dtype : CLK_FLOAT), st) != hipSuccess) {
-      for (int j = 0; j < i; ++j) (void)hipFreeAsync(tmp[j], st);
-      return fail(CLK_MEM_OBJECT_ALLOCATION_FAILURE, std::string(name) + ": temporary allocation failed");
-    }
-  }
   // temporaries: float for Gaussian and mean; the source type for min/max,
   // whose values are exact in any type that holds the source
   const int tdt = (op == clk::OP_MIN || op == clk::OP_MAX) ? src->dtype : CLK_FLOAT;
+  void* tmp[2] = {nullptr, nullptr};
+  const int ntmp = nact - 1 < 2 ? nact - 1 : 2;
+  const size_t one = ((size_t)V * clk_dtype_size(tdt) + 255) & ~(size_t)255;
+  auto lk = clk::stream_scratch_lock();  // until the last pass is issued
+  if (ntmp > 0 && V > 0) {
+    void* blk = nullptr;
+    if (clk::stream_scratch(st, 0, ntmp * one, &blk) != hipSuccess)
+      return fail(CLK_MEM_OBJECT_ALLOCATION_FAILURE, std::string(name) + ": temporary allocation failed");
+    tmp[0] = blk;
+    if (ntmp > 1) tmp[1] = static_cast<char*>(blk) + one;
+  }
   clk_image cur = *src;
   int done = 0, ti = 0;
   rc = 0;
@@ -187,8 +219,6 @@ int32_t chain3(int op, const char* name, void* stream, const clk_image* dst, con
     rc = clk::launch_separable(op, st, out, cur, a, ps[a].N, ps[a].s, &err);
     cur = out;
   }
-  for (int i = 0; i < ntmp; ++i)
-    if (tmp[i]) (void)hipFreeAsync(tmp[i], st);
   return rc ? fail(rc, std::string(name) + ": " + err) : ok();
 }
 

@@ -210,9 +210,9 @@ int32_t clk_execute(void* stream, const char* kernel_name, const clk_arg* args,
 /* ---- host chain: full 3D filters (three single-axis passes) ------------- */
 
 /* The reference leaves the 3-pass chain to its host.  These run the passes
- * x, y, z with float32 temporaries (allocated and freed by the library,
- * stream-ordered, from a per-device pool that keeps freed memory for the
- * next call); an axis with sigma <= 0 (Gaussian) or radius 0 is a
+ * x, y, z with float32 temporaries (device blocks of the library's, kept
+ * per device and stream and reused by the next call on that stream); an
+ * axis with sigma <= 0 (Gaussian) or radius 0 is a
  * converting copy.  N for a Gaussian axis is int(8*sigma) rounded up to
  * odd; for the box filters N = 2*radius+1. */
 int32_t clk_gaussian_blur_3d(void* stream, const clk_image* dst,
@@ -268,9 +268,10 @@ typedef struct clk_filter_job {
  * one batch: the single-axis passes of all jobs run stage by stage, with one
  * kernel launch per kernel class and stage instead of one per volume and
  * axis.  Results equal those of the single calls bit for bit.  The batch's
- * temporaries and launch tables are stream-ordered allocations freed when
- * the stream passes the batch.  No reference counterpart: the reference
- * leaves launching to its host. */
+ * temporaries and launch tables live in device blocks kept per (device,
+ * stream) and reused by the next call on that stream (grown when a call
+ * needs more; held until the process ends).  No reference counterpart: the
+ * reference leaves launching to its host. */
 int32_t clk_batch_filter_3d(void* stream, const clk_filter_job* jobs, int32_t njobs);
 
 /* Launch-shape knobs; results never depend on them.  Keys: "strided_rounds"

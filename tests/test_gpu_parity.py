@@ -200,6 +200,36 @@ def test_host_chain_matches_oracle_chain(gpu):
         assert_parity(from_device(td, src.shape, np.uint16), want, fn)
 
 
+def test_host_chain_kept_temporaries_across_calls_and_streams(gpu):
+    """The 3D chain keeps its temporaries per stream and reuses them: calls
+    of shrinking, growing and repeated sizes queued back to back on two
+    streams (no synchronisation between them) each equal the oracle."""
+    import torch
+    import clkmi
+    shapes = [(9, 40, 70), (5, 12, 17), (14, 50, 90), (5, 12, 17), (14, 50, 90), (3, 7, 300)]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    keep, checks = [], []
+    for i, shape in enumerate(shapes):
+        src = random_volume(shape, np.uint16, seed=40 + i, hi=3000)
+        ts, si = to_device(src)
+        st = streams[i % 2]
+        st.wait_stream(torch.cuda.current_stream())  # the upload
+        tg, dg = empty_device(shape, np.float32)
+        tm, dm = empty_device(shape, np.uint16)
+        with torch.cuda.stream(st):
+            clkmi.gaussian_blur_3d(si, dg, 1.5, 1.0, 0.8, stream=st)
+            clkmi.maximum_box_3d(si, dm, 2, 1, 1, stream=st)
+        keep += [ts, tg, tm]
+        checks.append((src, shape, tg, tm))
+    for st in streams:
+        st.synchronize()
+    for i, (src, shape, tg, tm) in enumerate(checks):
+        assert_parity(from_device(tg, shape, np.float32), O.gaussian_blur_3d(src, np.float32, 1.5, 1.0, 0.8),
+                      "gaussian call %d %s" % (i, shape))
+        assert_parity(from_device(tm, shape, np.uint16), O.box_3d(O.OP_MAX, src, np.uint16, 2, 1, 1),
+                      "maximum call %d %s" % (i, shape))
+
+
 def test_full_size_c1_gaussian_chain(gpu):
     """The reference's CPU-sized case at its full size: float32 256x256x64,
     U[0, 1000) (seed 1), the 3D Gaussian sigma = (2, 2, 1) -> N = (17, 17, 9)
