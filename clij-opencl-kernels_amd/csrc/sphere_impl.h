@@ -206,13 +206,13 @@ static_assert(kSphereTileMaxOff % 8 == 0, "offset chunks of 8");
 // integer sources (keys v - lo keep the order), the mean of byte sources
 // over at most 257 values (the key sum fits 16 bits; the reference's float
 // sum of integers is exact, so the integer sum gives its result), and the
-// median network.  NET > 0 (median, dst of 8 or 16 bits, exactly NET
-// offsets): the keys of each pair of output rows are gathered once and the
-// pruned Batcher network of the box median runs on them, instead of
-// bisection rounds over LDS.
+// median network.  NET > 0 (median, exactly NET offsets): the keys of each
+// output row (of each pair of rows for a dst of 8 or 16 bits) are gathered
+// once and the pruned Batcher network of the box median runs on them,
+// instead of bisection rounds over LDS.
 template <class TS, class TD, int OP, int TY, int NET = 0, int PK = 0>
 __global__ __launch_bounds__(256) void sphere_slide_kernel(const SphereSlideParams p) {
-  static_assert(NET == 0 || PK == 1, "the median network reads packed row pairs");
+  static_assert(NET == 0 || OP == SPH_MEDIAN, "the network is the median's");
   static_assert(PK == 0 || TY % 2 == 0, "rows in pairs");
   constexpr int NMAX = TY + 2 * kSlideMaxEY;
   constexpr int RL = kSlideRL;
@@ -321,7 +321,7 @@ __global__ __launch_bounds__(256) void sphere_slide_kernel(const SphereSlidePara
           if (u < rem) f(base(o[u]));
       }
     };
-    if constexpr (OP == SPH_MEDIAN && NET > 0) {
+    if constexpr (OP == SPH_MEDIAN && NET > 0 && PK) {
       // rank NET/2 of the keys (median_sphere_3d_x.cl:54-68); keys of an
       // 8- or 16-bit dst fit 16 bits, rows j and j+1 share a register
       static_assert(TY % 2 == 0 && (NET & 1) && KeyBits<TD>::v <= 16, "row pairs, odd count, 16-bit keys");
@@ -334,6 +334,16 @@ __global__ __launch_bounds__(256) void sphere_slide_kernel(const SphereSlidePara
         const us2 m = batcher_median<NET>(a);
         store(2 * jp, static_cast<TD>(key_value<TD>(m.x)));
         store(2 * jp + 1, static_cast<TD>(key_value<TD>(m.y)));
+      }
+    } else if constexpr (OP == SPH_MEDIAN && NET > 0) {
+      // float dst: 32-bit keys, one output row at a time through the same
+      // pruned network (the keys are the bisection's, so is the result)
+#pragma unroll
+      for (int j = 0; j < TY; ++j) {
+        uint32_t a[NET];
+#pragma unroll
+        for (int i = 0; i < NET; ++i) a[i] = base(p.off[i])[j * RL];
+        store(j, static_cast<TD>(key_value<TD>(batcher_median<NET, uint32_t>(a, 0xffffffffu))));
       }
     } else if constexpr (OP == SPH_MEDIAN) {
       // rank noff/2 of the keys (median_sphere_3d_x.cl:54-68) by bisection:

@@ -13,10 +13,8 @@ namespace clk {
 namespace {
 
 // the register network for medians of 7 or 33 values (the balls of N = 3
-// and 5) with an 8- or 16-bit dst
-inline bool slide_net(int op, int n, int dst_t) {
-  return op == SPH_MEDIAN && (n == 7 || n == 33) && dst_t != CLK_FLOAT;
-}
+// and 5): 16-bit key pairs for an 8- or 16-bit dst, 32-bit keys for float
+inline bool slide_net(int op, int n, int /*dst_t*/) { return op == SPH_MEDIAN && (n == 7 || n == 33); }
 
 template <class TS, class TD, int TY>
 void slide_op(int op, hipStream_t st, const SphereSlideParams& sp, dim3 grid, size_t shmem) {
@@ -29,6 +27,15 @@ void slide_op(int op, hipStream_t st, const SphereSlideParams& sp, dim3 grid, si
     }
     if (op == SPH_MEDIAN && sp.noff == 33) {
       CLK_SLIDE(SPH_MEDIAN, 33, 1);
+      return;
+    }
+  } else {
+    if (op == SPH_MEDIAN && sp.noff == 7) {
+      CLK_SLIDE(SPH_MEDIAN, 7, 0);
+      return;
+    }
+    if (op == SPH_MEDIAN && sp.noff == 33) {
+      CLK_SLIDE(SPH_MEDIAN, 33, 0);
       return;
     }
   }
