@@ -482,6 +482,77 @@ __device__ __forceinline__ void median_27_body(const MedParams& p, uint32_t bx, 
   static_cast<TD*>(p.dst)[(z * p.H + y) * p.W + x] = static_cast<TD>(key_value<TD>(m));
 }
 
+// 3x3x3 median with a float dst in single launches: a block filters 256
+// consecutive x of one row y for a run of kF27ZRun planes.  The 32-bit keys
+// of the source values are staged in LDS, three rows of 258 per plane, in a
+// ring of four planes; a z step reads planes z-1, z, z+1 while the loads of
+// plane z+2 are in flight in registers, then writes them to the fourth slot
+// (that of plane z-2, not read in this step) before one block barrier.
+constexpr int kF27ZRun = 16;
+
+template <class TS, class TD>
+__device__ __forceinline__ void median_27_key_slide_body(const MedParams& p, uint32_t bx, uint32_t by, uint32_t bz) {
+  __shared__ uint32_t ring[4][3][258];
+  const int t = threadIdx.x;
+  const int64_t xt = (int64_t)bx * 256;
+  const int64_t y = by;
+  const int64_t z0 = (int64_t)bz * kF27ZRun;
+  const int64_t z1 = (z0 + kF27ZRun < p.D) ? z0 + kF27ZRun : p.D;
+  const TS* __restrict__ src = static_cast<const TS*>(p.src);
+  // staged columns x = xt - 1 + c: c = t, and c = 256 + t for t < 2; clamped
+  const int64_t xa = clamp64(xt - 1 + t, 0, p.W - 1);
+  const int64_t xb = clamp64(xt + 255 + t, 0, p.W - 1);
+  const bool two = t < 2;
+  int64_t rows[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) rows[r] = clamp64(y + r - 1, 0, p.H - 1) * p.W;
+  const int64_t PS = p.H * p.W;
+  TS va[3], vb[3];
+  auto fetch = [&](int64_t zz) {
+    const TS* pl = src + clamp64(zz, 0, p.D - 1) * PS;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      va[r] = pl[rows[r] + xa];
+      vb[r] = two ? pl[rows[r] + xb] : TS(0);
+    }
+  };
+  auto put = [&](int slot) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      ring[slot][r][t] = value_key<TD>(gather_value<TS, TD>(va[r], p.wrap));
+      if (two) ring[slot][r][256 + t] = value_key<TD>(gather_value<TS, TD>(vb[r], p.wrap));
+    }
+  };
+  // plane zz sits in slot (zz - z0 + 1) & 3
+  fetch(z0 - 1);
+  put(0);
+  fetch(z0);
+  put(1);
+  fetch(z0 + 1);
+  put(2);
+  __syncthreads();
+  const int64_t x = xt + t;
+  TD* __restrict__ dst = static_cast<TD*>(p.dst);
+  for (int64_t z = z0; z < z1; ++z) {
+    const int zi = (int)(z - z0);
+    const bool more = z + 1 < z1;
+    if (more) fetch(z + 2);
+    uint32_t a[27];
+#pragma unroll
+    for (int dz = 0; dz < 3; ++dz) {
+      const int sl = (zi + dz) & 3;  // plane z - 1 + dz
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) a[9 * dz + 3 * r + i] = ring[sl][r][t + i];
+    }
+    const uint32_t m = batcher_median<27, uint32_t>(a, 0xffffffffu);
+    if (x < p.W) dst[(z * p.H + y) * p.W + x] = static_cast<TD>(key_value<TD>(m));
+    if (more) put((zi + 3) & 3);
+    __syncthreads();
+  }
+}
+
 // 8-bit dst: histogram bin of a gathered value.
 template <class TD>
 __device__ __forceinline__ int bin8(float v) {
@@ -753,6 +824,11 @@ __global__ __launch_bounds__(64) void median_hist8_fixed_kernel(const MedParams 
 template <class TS, class TD, int I>
 __global__ __launch_bounds__(256) void median_regs_kernel(const MedParams p) {
   median_regs_body<TS, TD, I>(p, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+template <class TS, class TD>
+__global__ __launch_bounds__(256) void median_27_key_slide_kernel(const MedParams p) {
+  median_27_key_slide_body<TS, TD>(p, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 template <class TS, class TD>

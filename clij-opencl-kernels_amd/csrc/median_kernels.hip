@@ -42,7 +42,16 @@ static int32_t median_typed(hipStream_t st, MedParams& p, std::string* err) {
         hipLaunchKernelGGL((median_hist8_kernel<TS, TD>), grid, dim3(threads), 0, st, p);
       }
       break;
-    case MED_27: hipLaunchKernelGGL((median_27_kernel<TS, TD>), grid, dim3(threads), 0, st, p); break;
+    case MED_27:
+      // planned for float dst only (8/16-bit dst take the packed network);
+      // single launches slide along z (runs of kF27ZRun planes)
+      if constexpr (sizeof(TD) == 4) {
+        const dim3 g(grid.x, grid.y, (uint32_t)((p.D + kF27ZRun - 1) / kF27ZRun));
+        hipLaunchKernelGGL((median_27_key_slide_kernel<TS, TD>), g, dim3(threads), 0, st, p);
+      } else {
+        hipLaunchKernelGGL((median_27_kernel<TS, TD>), grid, dim3(threads), 0, st, p);
+      }
+      break;
 #define CLK_MED_TIER(I)                                                                     \
   case MED_REGS + I:                                                                        \
     if constexpr (sizeof(TD) >= 2)                                                          \

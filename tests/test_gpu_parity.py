@@ -166,6 +166,22 @@ def test_median_3x3x3_packed_z_runs(gpu, src_t, dst_t):
     assert_parity(got, O.median_box_3d(a, dst_t, 3, 3, 3), "median 3x3x3 %s->%s" % (DTYPE_NAMES[src_t], DTYPE_NAMES[dst_t]))
 
 
+@pytest.mark.parametrize("src_t", (np.float32, np.uint8, np.int16), ids=lambda t: DTYPE_NAMES[t])
+def test_median_3x3x3_float_dst_z_runs(gpu, src_t):
+    """The float-dst 3x3x3 median of single launches stages 32-bit keys and
+    slides along z in runs of 16 planes: depth 37 (three runs, the last
+    partial), width 600 (three 256-wide tiles, the last partial), depths 1
+    and 2, and float data with -0 beside +0."""
+    for shape in ((37, 5, 600), (1, 9, 300), (2, 3, 257), (17, 19, 33)):
+        a = random_volume(shape, src_t, seed=73 + shape[0])
+        if src_t == np.float32:
+            a[:, ::2, ::3] = 0.0
+            a[:, 1::2, ::3] = -0.0
+        got = gpu_median(a, np.float32, 3, 3, 3)
+        assert_parity(got, O.median_box_3d(a, np.float32, 3, 3, 3),
+                      "median 3x3x3 %s %s->f32" % (shape, DTYPE_NAMES[src_t]))
+
+
 def test_execute_by_name_matches_typed_call(gpu):
     import clkmi
     src = random_volume((6, 7, 8), np.uint16, seed=1)
