@@ -67,6 +67,7 @@ __global__ __launch_bounds__(256) void batch_median_kernel(const MedParams* __re
   const uint32_t by = r % p.gy, bz = r / p.gy;
   if constexpr (KIND == MED_PK8) median_27_pk8_slide_body<TS, TD>(p, bx, by, bz);
   else if constexpr (KIND == MED_PK9) median_9_pk8_slide_body<TS, TD>(p, bx, by, bz);
+  else if constexpr (KIND == MED_27 && sizeof(TD) == 4) median_27_key_slide_body<TS, TD>(p, bx, by, bz);
   else if constexpr (KIND == MED_27) median_27_body<TS, TD>(p, bx, by, bz);
   else if constexpr (KIND >= MED_REGS) median_regs_body<TS, TD, KIND - MED_REGS>(p, bx, by, bz);
   else median_generic_body<TS, TD>(p, bx, by, bz);
@@ -391,9 +392,12 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
     dim3 grid;
     int threads;
     const MedKind mk = plan_median(p, clk_dtype_size(jb.dst.dtype), &grid, &threads);
-    // the packed networks slide along z (3x3x3) or y (3x3) as in single launches
+    // the 3x3x3 bodies slide along z and the packed 3x3 one along y, as in
+    // single launches
     if (mk == MED_PK8) {
       grid.z = (uint32_t)((p.D + kPk8ZRun - 1) / kPk8ZRun);
+    } else if (mk == MED_27 && clk_dtype_size(jb.dst.dtype) == 4) {
+      grid.z = (uint32_t)((p.D + kF27ZRun - 1) / kF27ZRun);
     } else if (mk == MED_PK9) {
       grid.y = (uint32_t)((p.H + kPk9YRun - 1) / kPk9YRun);
       p.gy = grid.y;
