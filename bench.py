@@ -167,27 +167,30 @@ def suite_c2():
     return L
 
 
-def cpu_baseline(nthreads, sample=(128, 256, 256)):
-    """The oracle on a bounded sub-volume of the same workload, all host
-    cores: the same four 3D filters as three single-axis passes each."""
+def cpu_baseline(src_host, nthreads):
+    """The oracle's -O3 x86-64-v3 build (bit-identical to the checker) on
+    this host's cores, on the benchmark's own uint16 volume at full size:
+    the same four 3D filters as three single-axis passes each."""
     import clk_oracle as O
-    rng = np.random.default_rng(2)
-    D, H, W = sample
-    src = (rng.poisson(100.0, size=sample)).astype(np.uint16)
+    D, H, W = src_host.shape
     t0 = time.perf_counter()
-    g = O.separable(0, src, np.float32, 0, 25, 3.0, nthreads)
-    g = O.separable(0, g, np.float32, 1, 25, 3.0, nthreads)
-    g = O.separable(0, g, np.float32, 2, 13, 1.5, nthreads)
+    g = O.separable(0, src_host, np.float32, 0, 25, 3.0, nthreads, fast=True)
+    g = O.separable(0, g, np.float32, 1, 25, 3.0, nthreads, fast=True)
+    g = O.separable(0, g, np.float32, 2, 13, 1.5, nthreads, fast=True)
+    del g
+    t1 = time.perf_counter()
     for op in (1, 2, 3):
-        a = O.separable(op, src, np.uint16, 0, 7, 0.0, nthreads)
-        a = O.separable(op, a, np.uint16, 1, 7, 0.0, nthreads)
-        a = O.separable(op, a, np.uint16, 2, 7, 0.0, nthreads)
+        a = O.separable(op, src_host, np.uint16, 0, 7, 0.0, nthreads, fast=True)
+        a = O.separable(op, a, np.uint16, 1, 7, 0.0, nthreads, fast=True)
+        a = O.separable(op, a, np.uint16, 2, 7, 0.0, nthreads, fast=True)
+        del a
     dt = time.perf_counter() - t0
     V = D * H * W
     return {"value": round(4 * V / dt / 1e9, 4), "unit": "Gvoxel/s", "cores": int(nthreads),
             "kind": "port",
-            "sample": "uint16 %dx%dx%d Poisson(100), the same 12 single-axis launches, "
-                      "oracle/clk_oracle.c with OpenMP, %.1f s" % (W, H, D, dt)}
+            "sample": "the benchmark's uint16 %dx%dx%d volume (Poisson(100) + 500 blobs, seed 2) at full size: "
+                      "the same 12 single-axis passes (Gaussian %.1f s, mean/min/max %.1f s), "
+                      "oracle/clk_oracle.c built -O3 -march=x86-64-v3 with OpenMP" % (W, H, D, t1 - t0, dt - (t1 - t0))}
 
 
 def make_c3_volume(torch, D, H, W, seed=3):
@@ -229,9 +232,11 @@ def make_c4_jobs(torch, clkmi, n=256, seed=4):
     return jobs, bufs
 
 
-def bench_batched(args, torch, clkmi):
+def measure_batched(torch, clkmi, steps, warmup, autotune_on=True):
+    """The uneven batch: one clk_batch_filter_3d call over the 256 jobs
+    against one 3D call per volume, each timed with HIP events on the
+    launch stream (mean of `steps` runs after `warmup`)."""
     from clkmi import autotune
-    torch.cuda.init()
     jobs, bufs = make_c4_jobs(torch, clkmi)
     V = sum(int(j.src.width * j.src.height * j.src.depth) for j in jobs)
     stream = torch.cuda.current_stream()
@@ -247,34 +252,42 @@ def bench_batched(args, torch, clkmi):
         clkmi.batch_filter_3d(jobs, stream=stream)
 
     tuned = []
-    if not args.no_autotune:
+    if autotune_on:
         tuned.append(autotune.tune(batched, "batch_seg"))
-    ms_plain = autotune.time_ms(plain, repeats=max(3, args.steps), warmup=args.warmup)
-    ms_batch = autotune.time_ms(batched, repeats=max(3, args.steps), warmup=args.warmup)
-    gv_b = V / (ms_batch * 1e-3) / 1e9
-    gv_p = V / (ms_plain * 1e-3) / 1e9
+    ms_plain = autotune.time_ms(plain, repeats=max(3, steps), warmup=warmup)
+    ms_batch = autotune.time_ms(batched, repeats=max(3, steps), warmup=warmup)
+    del jobs, bufs
+    torch.cuda.empty_cache()
+    return {"voxels": V, "batched_ms": round(ms_batch, 4), "plain_ms": round(ms_plain, 4),
+            "batched_Gvoxel_per_s": round(V / (ms_batch * 1e-3) / 1e9, 3),
+            "plain_Gvoxel_per_s": round(V / (ms_plain * 1e-3) / 1e9, 3),
+            "batched_over_plain": round(ms_plain / ms_batch, 3), "autotune": tuned}
+
+
+def bench_batched(args, torch, clkmi):
+    torch.cuda.init()
+    r = measure_batched(torch, clkmi, args.steps, args.warmup, not args.no_autotune)
     out = {
-        "metric": "Gvoxel/s, batched uneven 3D filters", "value": round(gv_b, 3), "unit": "Gvoxel/s",
-        "n_gpus": 1, "steps": max(3, args.steps), "warmup": args.warmup, "ms_per_step": round(ms_batch, 4),
+        "metric": "Gvoxel/s, batched uneven 3D filters", "value": r["batched_Gvoxel_per_s"], "unit": "Gvoxel/s",
+        "n_gpus": 1, "steps": max(3, args.steps), "warmup": args.warmup, "ms_per_step": r["batched_ms"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic: 256 uint16 volumes, axes U{32..256}, Poisson(200), seed 4",
         "config": {"workload": "256 uneven uint16 volumes, one of gaussian(sigma U[0.5,4]) / min / max "
                                "(r 1..5) each, one clk_batch_filter_3d call vs one 3D call per volume",
-                   "voxels": V},
-        "plain_ms": round(ms_plain, 4), "plain_Gvoxel_per_s": round(gv_p, 3),
-        "batched_over_plain": round(ms_plain / ms_batch, 3), "autotune": tuned,
+                   "voxels": r["voxels"]},
     }
+    out.update({k: v for k, v in r.items() if k != "voxels"})
     print(json.dumps(out))
 
 
-def bench_served(args, torch, clkmi):
+def measure_served(torch, clkmi, requests, fractions):
     """Served chains: Gaussian (2,2,1) -> maximum r=2 -> median r=1 on
     device-resident uint16 cubes of edge log-uniform in [64, 768] (Poisson(150)
     + 50 blobs, amplitude 1500, sigma 4), admitted against a 200 GB budget.
     A saturation run (all requests at once) gives the capacity; then Poisson
-    arrivals at fractions of it give p50/p99 latency and throughput."""
+    arrivals at the given fractions of it give p50/p99 latency and
+    throughput.  `requests` requests per rate, drawn from 12 pooled cubes."""
     from clkmi import serve
-    torch.cuda.init()
     rng = np.random.default_rng(5)
     pool = []
     for i in range(12):
@@ -299,36 +312,46 @@ def bench_served(args, torch, clkmi):
         t1 = max(r.t_done for r in reqs)
         vox = sum(r.voxels for r in reqs)
         st = serve.latency_stats(reqs)
-        st.update({"rate_per_s": rate, "requests": n, "Gvoxel_per_s": round(vox / (t1 - t0) / 1e9, 3),
+        st.update({"rate_per_s": round(rate, 2), "requests": n,
+                   "Gvoxel_per_s": round(vox / (t1 - t0) / 1e9, 3),
                    "failed": sum(1 for r in reqs if r.error)})
         return st
 
-    run(0, 6)  # warm-up: kernels, allocator
-    sat = run(0, args.requests)
+    run(0, 12)  # warm-up: kernels, allocator, kept temporaries
+    sat = run(0, requests)
     # requests per second at saturation: throughput over the mean request size
     capacity = sat["Gvoxel_per_s"] * 1e9 / float(np.mean([p[1].width ** 3 for p in pool]))
-    sweep = [run(f * capacity, args.requests) for f in (0.25, 0.5, 0.75, 1.0)]
+    sweep = [run(f * capacity, requests) for f in fractions]
+    stats = {"batches": srv.batches, "size_groups": srv.groups, "max_batches_in_flight": srv.max_in_flight,
+             "temporary_releases": srv.releases}
     srv.close()
-    best = max(sweep, key=lambda x: x["Gvoxel_per_s"])
+    del pool
+    torch.cuda.empty_cache()
+    return {"saturation": sat, "capacity_req_per_s": round(capacity, 2), "sweep": sweep, "server": stats}
+
+
+def bench_served(args, torch, clkmi):
+    torch.cuda.init()
+    r = measure_served(torch, clkmi, args.requests, (0.25, 0.5, 0.75, 1.0))
+    best = max(r["sweep"], key=lambda x: x["Gvoxel_per_s"])
     out = {
         "metric": "served chains: p50/p99 latency and Gvoxel/s", "value": best["Gvoxel_per_s"],
-        "unit": "Gvoxel/s", "n_gpus": 1, "steps": args.requests, "warmup": 6,
+        "unit": "Gvoxel/s", "n_gpus": 1, "steps": args.requests, "warmup": 12,
         "ms_per_step": best.get("p50_ms"), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32",
         "data": "synthetic: uint16 cubes, edge log-uniform [64,768], Poisson(150) + 50 blobs, seed 5",
         "config": {"workload": "served chain gaussian(2,2,1)->maximum r=2->median r=1, Poisson arrivals, "
-                               "200 GB HBM budget, 3 streams, batches up to 64",
-                   "pool_edges": [int(p[1].width) for p in pool]},
-        "saturation": sat, "capacity_req_per_s": round(capacity, 2), "sweep": sweep,
-        "batches": srv.batches, "max_batches_in_flight": srv.max_in_flight,
+                               "200 GB HBM budget, 3 streams, batches up to 64"},
     }
+    out.update(r)
     print(json.dumps(out))
 
 
-def bench_median(args, torch, clkmi):
-    W, H, D = (int(v) for v in (args.size or "512x512x256").split("x"))
+def measure_median(torch, clkmi, steps, warmup, size=(512, 512, 256)):
+    """median_box_3d r=1 and r=2 on the uint8 salt-and-pepper volume; mean
+    launch time by HIP events on the launch stream."""
+    W, H, D = size
     V = W * H * D
-    torch.cuda.init()
     src = make_c3_volume(torch, D, H, W)
     dst = torch.empty_like(src)
     si = clkmi.Image(src.data_ptr(), CLK_UCHAR, W, H, D)
@@ -337,18 +360,28 @@ def bench_median(args, torch, clkmi):
     res = []
     for r in (1, 2):
         n = 2 * r + 1
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             clkmi.median_box_3d(di, si, n, n, n, stream=stream)
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
         a.record(stream)
-        for _ in range(args.steps):
+        for _ in range(steps):
             clkmi.median_box_3d(di, si, n, n, n, stream=stream)
         b.record(stream)
         torch.cuda.synchronize()
-        ms = a.elapsed_time(b) / args.steps
+        ms = a.elapsed_time(b) / steps
         res.append({"radius": r, "ms": round(ms, 4), "Gvoxel_per_s": round(V / (ms * 1e-3) / 1e9, 2),
                     "GBps": round(2 * V / (ms * 1e-3) / 1e9, 1)})
+    del src, dst
+    torch.cuda.empty_cache()
+    return res
+
+
+def bench_median(args, torch, clkmi):
+    W, H, D = (int(v) for v in (args.size or "512x512x256").split("x"))
+    V = W * H * D
+    torch.cuda.init()
+    res = measure_median(torch, clkmi, args.steps, args.warmup, (W, H, D))
     tot = sum(x["ms"] for x in res)
     out = {
         "metric": "Gvoxel/s, median_box_3d", "value": round(2 * V / (tot * 1e-3) / 1e9, 3),
@@ -444,7 +477,9 @@ def main():
                          "batched vs one call per volume; c5: served chains "
                          "(Gaussian -> max -> median) under Poisson arrivals; c6: sphere filters, slice "
                          "median, copy_3d and the 2D siblings")
-    ap.add_argument("--requests", type=int, default=48, help="c5: requests per arrival rate")
+    ap.add_argument("--requests", type=int, default=200, help="c5: requests per arrival rate")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="c2: skip the median, batched and served measurements added to the line")
     ap.add_argument("--no-autotune", action="store_true", help="keep default launch shapes (no first-launch tuning)")
     ap.add_argument("--tune-cache", default=None, help="file of tuned launch shapes to load before and save after tuning")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -556,13 +591,43 @@ def main():
         launches.append({"launch": label, "ms": round(ms, 4), "bytes": nbytes,
                          "GBps": round(gbs, 1), "pct_hbm": round(100 * gbs / HBM_PEAK_GBS, 1)})
     dom = max(launches, key=lambda r: r["ms"])
+    # HBM bytes of the dominant launch are not counted in this run (that
+    # needs rocprofv3 --pmc passes); they come from the committed counter
+    # profile of the same launch on this volume, and say so
+    traffic = committed_traffic(dom["launch"]) if (W, H, D) == (1024, 1024, 512) else None
+    traffic_source = ("rocprofv3 --pmc TCC_EA0_RDREQ/WRREQ counts of this launch in "
+                      + os.path.relpath(PROFILE_COUNTERS, ROOT)) if traffic is not None else None
     nfilters = len({p[0].split(" dim")[0] for p in plan})
     value = nfilters * V / (ms_per_step * 1e-3) / 1e9
 
+    src_host = None if args.no_cpu_baseline else src.cpu().numpy().view(np.uint16)
+    del src, f_a, f_b, u_a, u_b, plan
+    torch.cuda.empty_cache()
+    extra = {}
+    if not args.no_extra:
+        # the rest of the metric, each on its own workload: the median
+        # (uint8 512x512x256, r = 1 and 2), the uneven batch (256 uint16
+        # volumes, batched vs per-volume calls) and the served chains
+        # (p50/p99 at saturation and at half of it, 200 requests each)
+        med = measure_median(torch, clkmi, 10, 2)
+        extra["median_uint8_512x512x256"] = {"r1_ms": med[0]["ms"], "r1_Gvoxel_per_s": med[0]["Gvoxel_per_s"],
+                                             "r2_ms": med[1]["ms"], "r2_Gvoxel_per_s": med[1]["Gvoxel_per_s"]}
+        bat = measure_batched(torch, clkmi, 5, 2, not args.no_autotune)
+        extra["batched_256_uneven_uint16"] = {k: bat[k] for k in (
+            "batched_ms", "plain_ms", "batched_over_plain", "batched_Gvoxel_per_s", "plain_Gvoxel_per_s", "voxels")}
+        srv = measure_served(torch, clkmi, args.requests, (0.5,))
+        keep = ("p50_ms", "p99_ms", "mean_ms", "Gvoxel_per_s", "requests", "rate_per_s", "failed")
+        extra["served_chains"] = {
+            "saturation": {k: srv["saturation"].get(k) for k in keep},
+            "half_capacity": {k: srv["sweep"][0].get(k) for k in keep},
+            "capacity_req_per_s": srv["capacity_req_per_s"],
+            "server": srv["server"],
+        }
+
     cpu = None
-    if not args.no_cpu_baseline:
+    if src_host is not None:
         nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        cpu = cpu_baseline(nthreads)
+        cpu = cpu_baseline(src_host, nthreads)
 
     out = {
         "metric": "Gvoxel/s + % HBM peak, 3D separable/median filters; p50/p99 latency served",
@@ -592,11 +657,13 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(dom["GBps"] / HBM_PEAK_GBS, 4),
-            "traffic": committed_traffic(dom["launch"]) if (W, H, D) == (1024, 1024, 512) else None,
+            "traffic": traffic,
+            "traffic_source": traffic_source,
         },
         "cpu_baseline": cpu,
         "launches": launches,
     }
+    out.update(extra)
     print(json.dumps(out))
 
 

@@ -25,6 +25,7 @@ __all__ = [
     "median_box_2d", "gaussian_blur_2d", "mean_box_2d", "minimum_box_2d", "maximum_box_2d",
     "minimum_sphere_3d", "maximum_sphere_3d", "mean_sphere_3d", "median_sphere_3d",
     "median_slice_by_slice_box_3d", "copy_3d", "detect_maxima_3d", "maximum_z_projection",
+    "release_temporaries", "temporaries_held",
 ]
 
 # Kernel name -> declared parameters (reference .cl declarations).
@@ -314,6 +315,19 @@ def batch_filter_3d(jobs, stream=None):
             a.sigma[k] = float(j.sigma[k])
             a.radius[k] = int(j.radius[k])
     check(lib().clk_batch_filter_3d(_stream_ptr(stream), arr, len(jobs)))
+
+
+def release_temporaries(stream=None):
+    """Free the temporaries kept for `stream` (waits for it) and return the
+    library's unused pooled device memory to the driver."""
+    check(lib().clk_release_temporaries(_stream_ptr(stream)))
+
+
+def temporaries_held(stream=None):
+    """(bytes kept for `stream`, bytes the library's pool holds on the device)."""
+    a, b = ctypes.c_int64(), ctypes.c_int64()
+    check(lib().clk_temporaries_held(_stream_ptr(stream), ctypes.byref(a), ctypes.byref(b)))
+    return a.value, b.value
 
 
 def set_tuning(key, value):

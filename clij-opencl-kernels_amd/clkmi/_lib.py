@@ -22,6 +22,7 @@ ERROR_NAMES = {
     -46: "CL_INVALID_KERNEL_NAME",
     -52: "CL_INVALID_KERNEL_ARGS",
     -53: "CL_INVALID_WORK_DIMENSION",
+    -59: "CL_INVALID_OPERATION",
 }
 
 # Every symbol the header declares; tests check the library exports them all.
@@ -36,7 +37,7 @@ EXPORTED = (
     "clk_maximum_separable_2d", "clk_median_box_2d", "clk_gaussian_blur_2d", "clk_mean_box_2d",
     "clk_minimum_box_2d", "clk_maximum_box_2d", "clk_minimum_sphere_3d", "clk_maximum_sphere_3d",
     "clk_mean_sphere_3d", "clk_median_sphere_3d", "clk_median_slice_by_slice_box_3d", "clk_copy_3d",
-    "clk_detect_maxima_3d", "clk_maximum_z_projection",
+    "clk_detect_maxima_3d", "clk_maximum_z_projection", "clk_release_temporaries", "clk_temporaries_held",
 )
 
 CLK_JOB_GAUSSIAN_BLUR_3D, CLK_JOB_MEAN_BOX_3D, CLK_JOB_MINIMUM_BOX_3D, CLK_JOB_MAXIMUM_BOX_3D, \
@@ -141,6 +142,10 @@ def lib():
             fn.restype = i32
         L.clk_batch_filter_3d.argtypes = [vp, ctypes.POINTER(ClkFilterJob), i32]
         L.clk_batch_filter_3d.restype = i32
+        L.clk_release_temporaries.argtypes = [vp]
+        L.clk_release_temporaries.restype = i32
+        L.clk_temporaries_held.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]
+        L.clk_temporaries_held.restype = i32
         L.clk_set_tuning.argtypes = [ctypes.c_char_p, i32]
         L.clk_set_tuning.restype = i32
         L.clk_get_tuning.argtypes = [ctypes.c_char_p, ctypes.POINTER(i32)]

@@ -236,6 +236,17 @@ struct Single {
 }  // namespace
 
 int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std::string* err) {
+  {
+    // The launch tables go up from reused host staging memory: a captured
+    // copy would read whatever that memory holds at each replay.
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess) (void)hipGetLastError();
+    if (cs != hipStreamCaptureStatusNone) {
+      *err = "clk_batch_filter_3d: a stream under graph capture is not supported (launch tables are "
+             "uploaded from reused host staging memory)";
+      return CLK_INVALID_OPERATION;
+    }
+  }
   // ---- plan: passes per stage, temporaries --------------------------------
   std::vector<std::vector<Pass>> stage_passes(3);
   std::vector<std::pair<clk_filter_job, int>> medians;  // job, unused
@@ -301,8 +312,8 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
   std::vector<std::map<ClassKey, ClassWork>> classes(3);
   std::vector<Single> singles;
   void* temp_base = nullptr;
-  auto scratch_lk = stream_scratch_lock();  // until the last launch is issued
-  if (temp_bytes > 0 && stream_scratch(st, 1, (size_t)temp_bytes, &temp_base) != hipSuccess) {
+  CallScratch scratch(st);  // held until the last launch is issued
+  if (temp_bytes > 0 && scratch.get(1, (size_t)temp_bytes, &temp_base) != hipSuccess) {
     *err = "batch temporaries: allocation failed";
     return CLK_MEM_OBJECT_ALLOCATION_FAILURE;
   }
@@ -422,7 +433,7 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
     }
   char* dev_tables = nullptr;
   if (tbytes > 0) {
-    if (stream_scratch(st, 2, tbytes, (void**)&dev_tables) != hipSuccess) {
+    if (scratch.get(2, tbytes, (void**)&dev_tables) != hipSuccess) {
       *err = "batch tables: allocation failed";
       return CLK_MEM_OBJECT_ALLOCATION_FAILURE;
     }

@@ -1,6 +1,7 @@
 // Internal interface between the C-ABI layer (clkmi_api.cpp) and the kernel
 // launchers (sep_kernels.hip, median_kernels.hip).
 #pragma once
+#include <memory>
 #include <mutex>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -66,10 +67,12 @@ struct Status {
 float expf_weight(float x);
 
 // Single-axis separable launch; op is Op (clk_device.h).  Arguments are
-// already validated by the caller (types, sizes, no overlap).
+// already validated by the caller (types, sizes, no overlap).  With
+// allow_tune false the launch never times candidates (it uses a shape the
+// autotuner already chose, or the default): no events, no synchronisation.
 int32_t launch_separable(int op, hipStream_t st, const clk_image& dst,
                          const clk_image& src, int32_t dim, int32_t N, float s,
-                         std::string* err);
+                         std::string* err, bool allow_tune = true);
 
 // Tuned launch shapes of single launches: write them to / merge them from
 // a text file (see clk_autotune_save / clk_autotune_load).
@@ -85,21 +88,51 @@ int32_t launch_fill_nan(hipStream_t st, const clk_image& dst, std::string* err);
 // Stream-ordered temporaries (free them with hipFreeAsync) from a memory pool
 // of the library's own per device, which keeps freed memory for the next
 // call instead of handing it back to the driver at every synchronisation as
-// the default pool does (release threshold 0).  The pool holds at most the
-// largest set of temporaries in use at once until the process ends.
+// the default pool does (release threshold 0).  clk_release_temporaries
+// trims it.
 hipError_t temp_alloc(void** p, size_t bytes, hipStream_t st);
 
-// Scratch blocks kept per (device, stream, slot) and reused by the next call
+// The temporaries of one call (host chain, batch).  Normally they are
+// device blocks kept per (device, stream, slot) and reused by the next call
 // on that stream: work on one stream runs in issue order, so a call's
 // temporaries are free once its kernels are queued behind it.  A larger
-// request replaces the block (the old one is freed in stream order); blocks
-// live until the process ends.  A new allocation per call of a new size
-// cost about 0.45 ms of host time.  Hold stream_scratch_lock() from the
-// first stream_scratch() of a call until its last launch is issued, so that
+// request replaces the block (the old one is freed in stream order).  A new
+// allocation per call of a new size cost about 0.45 ms of host time.  The
+// scope holds that stream's lock from construction until destruction, so
 // two host threads on one stream cannot interleave launches that share a
-// block.
-std::unique_lock<std::recursive_mutex> stream_scratch_lock();
-hipError_t stream_scratch(hipStream_t st, int slot, size_t bytes, void** p);
+// block; calls on other streams are not blocked.
+//
+// Streams on which one handle does not name one stream in issue order get
+// transient blocks instead, allocated per call and freed in stream order
+// when the scope ends: a stream under graph capture (the graph then owns
+// its allocations; a kept block would be baked into the graph and could be
+// freed or shared by later eager calls) and hipStreamPerThread (one handle,
+// a different stream in every host thread).
+class CallScratch {
+ public:
+  explicit CallScratch(hipStream_t st);
+  ~CallScratch();
+  CallScratch(const CallScratch&) = delete;
+  CallScratch& operator=(const CallScratch&) = delete;
+  hipError_t get(int slot, size_t bytes, void** p);
+  bool transient() const { return transient_; }
+
+ private:
+  hipStream_t st_;
+  bool transient_ = false;
+  void* blocks_ = nullptr;  // the stream's kept blocks (cached mode)
+  std::unique_lock<std::recursive_mutex> lock_;
+  void* owned_[3] = {nullptr, nullptr, nullptr};  // transient mode
+};
+constexpr int kScratchSlots = 3;  // 0: host chain, 1: batch temporaries, 2: batch tables
+
+// Kept blocks of one stream on the current device: bytes held, and release
+// (synchronises the stream, frees the blocks, trims the pool).
+int64_t kept_bytes(hipStream_t st);
+hipError_t release_kept(hipStream_t st);
+// Memory the library's pool holds on the current device (kept blocks and
+// freed memory not yet returned to the driver).
+int64_t pool_reserved_bytes();
 
 // detect_maxima_3d (detect = true; dst and src of one size) or
 // maximum_z_projection (dst W x H x 1); arguments validated by the caller.

@@ -17,63 +17,163 @@
 
 namespace clk {
 
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_pool_mu;
+hipMemPool_t g_pools[kMaxDevices] = {};
+bool g_no_pool[kMaxDevices] = {};
+
+// The library's pool on device `dev` (created on first use), or nullptr
+// when the device cannot create one (the default pool then serves it).
+hipMemPool_t device_pool(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> g(g_pool_mu);
+  if (!g_pools[dev] && !g_no_pool[dev]) {
+    hipMemPoolProps props;
+    memset(&props, 0, sizeof(props));
+    props.allocType = hipMemAllocationTypePinned;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = dev;
+    hipMemPool_t np = nullptr;
+    if (hipMemPoolCreate(&np, &props) == hipSuccess) {
+      uint64_t keep = UINT64_MAX;
+      (void)hipMemPoolSetAttribute(np, hipMemPoolAttrReleaseThreshold, &keep);
+      g_pools[dev] = np;
+    } else {
+      (void)hipGetLastError();
+      g_no_pool[dev] = true;
+    }
+  }
+  return g_pools[dev];
+}
+
+// Kept blocks of one (device, stream): a lock held by the call that uses
+// them, and one block per slot.
+struct StreamBlocks {
+  std::recursive_mutex mu;
+  void* ptr[clk::kScratchSlots] = {nullptr, nullptr, nullptr};
+  size_t bytes[clk::kScratchSlots] = {0, 0, 0};
+};
+std::mutex g_blocks_mu;  // guards the map only (never held across HIP calls)
+std::map<std::pair<int, hipStream_t>, std::unique_ptr<StreamBlocks>> g_blocks;
+
+StreamBlocks* blocks_of(int dev, hipStream_t st, bool create) {
+  std::lock_guard<std::mutex> g(g_blocks_mu);
+  auto it = g_blocks.find({dev, st});
+  if (it != g_blocks.end()) return it->second.get();
+  if (!create) return nullptr;
+  auto& slot = g_blocks[{dev, st}];
+  slot.reset(new StreamBlocks);
+  return slot.get();
+}
+}  // namespace
+
 hipError_t temp_alloc(void** p, size_t bytes, hipStream_t st) {
-  constexpr int kMaxDevices = 64;
-  static std::mutex mu;
-  static hipMemPool_t pools[kMaxDevices] = {};
-  static bool no_pool[kMaxDevices] = {};
   int dev = 0;
   const hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
-  hipMemPool_t pool = nullptr;
-  if (dev >= 0 && dev < kMaxDevices) {
-    std::lock_guard<std::mutex> g(mu);
-    if (!pools[dev] && !no_pool[dev]) {
-      hipMemPoolProps props;
-      memset(&props, 0, sizeof(props));
-      props.allocType = hipMemAllocationTypePinned;
-      props.location.type = hipMemLocationTypeDevice;
-      props.location.id = dev;
-      hipMemPool_t np = nullptr;
-      if (hipMemPoolCreate(&np, &props) == hipSuccess) {
-        uint64_t keep = UINT64_MAX;
-        (void)hipMemPoolSetAttribute(np, hipMemPoolAttrReleaseThreshold, &keep);
-        pools[dev] = np;
-      } else {
-        (void)hipGetLastError();
-        no_pool[dev] = true;  // the default pool serves this device
-      }
-    }
-    pool = pools[dev];
-  }
+  hipMemPool_t pool = device_pool(dev);
   return pool ? hipMallocFromPoolAsync(p, bytes, pool, st) : hipMallocAsync(p, bytes, st);
 }
 
-namespace {
-std::recursive_mutex g_scratch_mu;
-std::map<std::tuple<int, hipStream_t, int>, std::pair<void*, size_t>> g_scratch;
-}  // namespace
-
-std::unique_lock<std::recursive_mutex> stream_scratch_lock() {
-  return std::unique_lock<std::recursive_mutex>(g_scratch_mu);
+CallScratch::CallScratch(hipStream_t st) : st_(st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  const hipError_t e = hipStreamIsCapturing(st, &cs);
+  if (e != hipSuccess) (void)hipGetLastError();
+  int dev = 0;
+  const hipError_t ed = hipGetDevice(&dev);
+  transient_ = st == hipStreamPerThread || e != hipSuccess || cs != hipStreamCaptureStatusNone ||
+               ed != hipSuccess;
+  if (!transient_) {
+    StreamBlocks* b = blocks_of(dev, st, true);
+    lock_ = std::unique_lock<std::recursive_mutex>(b->mu);
+    blocks_ = b;
+  }
 }
 
-hipError_t stream_scratch(hipStream_t st, int slot, size_t bytes, void** p) {
-  std::lock_guard<std::recursive_mutex> g(g_scratch_mu);
-  int dev = 0;
-  const hipError_t e = hipGetDevice(&dev);
-  if (e != hipSuccess) return e;
-  std::pair<void*, size_t>& b = g_scratch[std::make_tuple(dev, st, slot)];
-  if (b.second < bytes) {
-    if (b.first) (void)hipFreeAsync(b.first, st);
-    b = {nullptr, 0};
-    void* q = nullptr;
-    const hipError_t ea = temp_alloc(&q, bytes, st);
-    if (ea != hipSuccess) return ea;
-    b = {q, bytes};
+CallScratch::~CallScratch() {
+  // transient blocks go back in stream order, after the launches that use them
+  for (void*& q : owned_)
+    if (q) {
+      (void)hipFreeAsync(q, st_);
+      q = nullptr;
+    }
+}
+
+hipError_t CallScratch::get(int slot, size_t bytes, void** p) {
+  if (slot < 0 || slot >= kScratchSlots) return hipErrorInvalidValue;
+  if (transient_) {
+    if (owned_[slot]) {
+      (void)hipFreeAsync(owned_[slot], st_);
+      owned_[slot] = nullptr;
+    }
+    // under capture the allocation becomes a node of the graph; the
+    // default pool is the one capture supports
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    const bool cap = hipStreamIsCapturing(st_, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+    const hipError_t e = cap ? hipMallocAsync(p, bytes, st_) : temp_alloc(p, bytes, st_);
+    if (e == hipSuccess) owned_[slot] = *p;
+    return e;
   }
-  *p = b.first;
+  StreamBlocks& b = *static_cast<StreamBlocks*>(blocks_);
+  if (b.bytes[slot] < bytes) {
+    if (b.ptr[slot]) (void)hipFreeAsync(b.ptr[slot], st_);
+    b.ptr[slot] = nullptr;
+    b.bytes[slot] = 0;
+    void* q = nullptr;
+    const hipError_t ea = temp_alloc(&q, bytes, st_);
+    if (ea != hipSuccess) return ea;
+    b.ptr[slot] = q;
+    b.bytes[slot] = bytes;
+  }
+  *p = b.ptr[slot];
   return hipSuccess;
+}
+
+int64_t kept_bytes(hipStream_t st) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  StreamBlocks* b = blocks_of(dev, st, false);
+  if (!b) return 0;
+  std::lock_guard<std::recursive_mutex> g(b->mu);
+  int64_t n = 0;
+  for (size_t v : b->bytes) n += (int64_t)v;
+  return n;
+}
+
+hipError_t release_kept(hipStream_t st) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  StreamBlocks* b = blocks_of(dev, st, false);
+  if (b) {
+    std::lock_guard<std::recursive_mutex> g(b->mu);
+    bool any = false;
+    for (int s = 0; s < kScratchSlots; ++s)
+      if (b->ptr[s]) {
+        (void)hipFreeAsync(b->ptr[s], st);
+        b->ptr[s] = nullptr;
+        b->bytes[s] = 0;
+        any = true;
+      }
+    if (any) e = hipStreamSynchronize(st);
+  }
+  hipMemPool_t pool = device_pool(dev);
+  if (pool) (void)hipMemPoolTrimTo(pool, 0);  // gives back what no block holds
+  return e;
+}
+
+int64_t pool_reserved_bytes() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  hipMemPool_t pool = device_pool(dev);
+  if (!pool) return 0;
+  uint64_t v = 0;
+  if (hipMemPoolGetAttribute(pool, hipMemPoolAttrReservedMemCurrent, &v) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return (int64_t)v;
 }
 
 }  // namespace clk
@@ -170,55 +270,64 @@ struct PassSpec {
   float s;
 };
 
-// Runs the active passes in axis order x, y, z.  Intermediate images are
-// float32 and stream-ordered allocations; an inactive axis is skipped (the
-// next pass reads the previous image directly, which gives the same values
-// as a float copy); with no active axis the result is a converting copy.
+// The temporaries a chain needs: their pixel type, how many (0..2), and
+// the bytes of one (256-byte aligned).
+struct ChainTemps {
+  int nact = 0, tdt = CLK_FLOAT, ntmp = 0;
+  size_t one = 0;
+};
+
+ChainTemps chain_temps(int op, const clk_image& dst, const clk_image& src, const PassSpec (&ps)[3]) {
+  ChainTemps t;
+  for (int a = 0; a < 3; ++a) t.nact += ps[a].active;
+  // float for Gaussian and mean; the source type for min/max, whose values
+  // are exact in any type that holds the source
+  t.tdt = (op == clk::OP_MIN || op == clk::OP_MAX) ? src.dtype : CLK_FLOAT;
+  t.ntmp = t.nact - 1 < 2 ? (t.nact > 0 ? t.nact - 1 : 0) : 2;
+  t.one = ((size_t)voxels(dst) * clk_dtype_size(t.tdt) + 255) & ~(size_t)255;
+  return t;
+}
+
+// Runs the active passes in axis order x, y, z through the temporaries at
+// `blk` (t.ntmp of t.one bytes); an inactive axis is skipped (the next pass
+// reads the previous image directly, which gives the same values as a float
+// copy); with no active axis the result is a converting copy.
+int32_t run_chain(int op, hipStream_t st, const clk_image& dst, const clk_image& src, const PassSpec (&ps)[3],
+                  const ChainTemps& t, void* blk, bool allow_tune, std::string* err) {
+  if (t.nact == 0) return clk::launch_convert_copy(st, dst, src, err);
+  void* tmp[2] = {blk, t.ntmp > 1 ? static_cast<char*>(blk) + t.one : nullptr};
+  clk_image cur = src;
+  int done = 0, ti = 0;
+  int32_t rc = 0;
+  for (int a = 0; a < 3 && rc == 0; ++a) {
+    if (!ps[a].active) continue;
+    ++done;
+    clk_image out = dst;
+    if (done < t.nact) {
+      out.dtype = t.tdt;
+      out.ptr = tmp[ti];
+      ti ^= 1;
+    }
+    rc = clk::launch_separable(op, st, out, cur, a, ps[a].N, ps[a].s, err, allow_tune);
+    cur = out;
+  }
+  return rc;
+}
+
+// The host chain with its temporaries in the stream's kept blocks
+// (clk::CallScratch).
 int32_t chain3(int op, const char* name, void* stream, const clk_image* dst, const clk_image* src,
                const PassSpec (&ps)[3]) {
   int32_t rc = check_pair(name, dst, src);
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
   std::string err;
-  int nact = 0;
-  for (int a = 0; a < 3; ++a) nact += ps[a].active;
-  if (nact == 0) {
-    rc = clk::launch_convert_copy(st, *dst, *src, &err);
-    return rc ? fail(rc, std::string(name) + ": " + err) : ok();
-  }
-  const int64_t V = voxels(*dst);
-  // temporaries: float for Gaussian and mean; the source type for min/max,
-  // whose values are exact in any type that holds the source
-  const int tdt = (op == clk::OP_MIN || op == clk::OP_MAX) ? src->dtype : CLK_FLOAT;
-  void* tmp[2] = {nullptr, nullptr};
-  const int ntmp = nact - 1 < 2 ? nact - 1 : 2;
-  const size_t one = ((size_t)V * clk_dtype_size(tdt) + 255) & ~(size_t)255;
-  auto lk = clk::stream_scratch_lock();  // until the last pass is issued
-  if (ntmp > 0 && V > 0) {
-    void* blk = nullptr;
-    if (clk::stream_scratch(st, 0, ntmp * one, &blk) != hipSuccess)
-      return fail(CLK_MEM_OBJECT_ALLOCATION_FAILURE, std::string(name) + ": temporary allocation failed");
-    tmp[0] = blk;
-    if (ntmp > 1) tmp[1] = static_cast<char*>(blk) + one;
-  }
-  clk_image cur = *src;
-  int done = 0, ti = 0;
-  rc = 0;
-  for (int a = 0; a < 3 && rc == 0; ++a) {
-    if (!ps[a].active) continue;
-    ++done;
-    clk_image out;
-    if (done == nact) {
-      out = *dst;
-    } else {
-      out = *dst;
-      out.dtype = tdt;
-      out.ptr = tmp[ti];
-      ti ^= 1;
-    }
-    rc = clk::launch_separable(op, st, out, cur, a, ps[a].N, ps[a].s, &err);
-    cur = out;
-  }
+  const ChainTemps t = chain_temps(op, *dst, *src, ps);
+  clk::CallScratch scratch(st);  // held until the last pass is issued
+  void* blk = nullptr;
+  if (t.ntmp > 0 && voxels(*dst) > 0 && scratch.get(0, t.ntmp * t.one, &blk) != hipSuccess)
+    return fail(CLK_MEM_OBJECT_ALLOCATION_FAILURE, std::string(name) + ": temporary allocation failed");
+  rc = run_chain(op, st, *dst, *src, ps, t, blk, true, &err);
   return rc ? fail(rc, std::string(name) + ": " + err) : ok();
 }
 
@@ -582,6 +691,19 @@ int32_t clk_batch_filter_3d(void* stream, const clk_filter_job* jobs, int32_t nj
   return rc ? fail(rc, err) : ok();
 }
 
+int32_t clk_release_temporaries(void* stream) {
+  const hipError_t e = clk::release_kept((hipStream_t)stream);
+  if (e != hipSuccess)
+    return fail(CLK_OUT_OF_RESOURCES, std::string("clk_release_temporaries: ") + hipGetErrorString(e));
+  return ok();
+}
+
+int32_t clk_temporaries_held(void* stream, int64_t* stream_bytes, int64_t* pool_bytes) {
+  if (stream_bytes) *stream_bytes = clk::kept_bytes((hipStream_t)stream);
+  if (pool_bytes) *pool_bytes = clk::pool_reserved_bytes();
+  return ok();
+}
+
 int32_t clk_init(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(CLK_OUT_OF_RESOURCES, "no HIP device");
@@ -628,6 +750,8 @@ int32_t clk_stream_create(void** stream) {
 }
 
 int32_t clk_stream_destroy(void* stream) {
+  // its kept temporaries must not pass to a later stream with the same handle
+  if (stream) (void)clk::release_kept((hipStream_t)stream);
   if (stream && hipStreamDestroy((hipStream_t)stream) != hipSuccess)
     return fail(CLK_INVALID_VALUE, "clk_stream_destroy failed");
   return ok();

@@ -407,12 +407,12 @@ int32_t autotune_load(const char* path) {
 }
 
 int32_t launch_separable(int op, hipStream_t st, const clk_image& dst, const clk_image& src,
-                         int32_t dim, int32_t N, float s, std::string* err) {
+                         int32_t dim, int32_t N, float s, std::string* err, bool allow_tune) {
   const int64_t W = dst.width, H = dst.height, D = dst.depth;
   if (W * H * D == 0) return 0;
-  SepParams* pp = new (std::nothrow) SepParams;
-  if (!pp) { *err = "host allocation failed"; return CLK_OUT_OF_RESOURCES; }
-  SepParams& p = *pp;
+  // the launch parameters (weights included, ~2 KB) of this thread's launch
+  thread_local SepParams tl_params;
+  SepParams& p = tl_params;
   int32_t rc = 0;
   bool done = false;
   if (tuning(TUNE_AUTOTUNE) == 1 && W * H * D >= kTuneMinVoxels && N > 1) {
@@ -431,7 +431,7 @@ int32_t launch_separable(int op, hipStream_t st, const clk_image& dst, const clk
           have = true;
         }
       }
-      if (!have) {
+      if (!have && allow_tune) {
         rc = tune_signature(op, st, dst, src, dim, N, s, p, dim == 0, &c, err);
         if (rc == 0) {
           std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -445,7 +445,7 @@ int32_t launch_separable(int op, hipStream_t st, const clk_image& dst, const clk
         }
         done = true;  // the candidates already wrote dst
       }
-      if (!done) {
+      if (!done && have) {
         ScopedTuning scope(c.set, c.n);
         rc = launch_once(op, st, dst, src, dim, N, s, p, err);
         done = true;
@@ -453,7 +453,6 @@ int32_t launch_separable(int op, hipStream_t st, const clk_image& dst, const clk
     }
   }
   if (!done) rc = launch_once(op, st, dst, src, dim, N, s, p, err);
-  delete pp;
   return rc;
 }
 

@@ -49,6 +49,7 @@ typedef enum clk_dtype {
 #define CLK_INVALID_KERNEL_NAME            (-46)
 #define CLK_INVALID_KERNEL_ARGS            (-52)
 #define CLK_INVALID_WORK_DIMENSION         (-53)
+#define CLK_INVALID_OPERATION              (-59)
 
 /* Image descriptor: replaces a cl_mem buffer plus the compile-time
  * GET_IMAGE_WIDTH/HEIGHT/DEPTH constants (reference README.md:75-87). */
@@ -210,11 +211,20 @@ int32_t clk_execute(void* stream, const char* kernel_name, const clk_arg* args,
 /* ---- host chain: full 3D filters (three single-axis passes) ------------- */
 
 /* The reference leaves the 3-pass chain to its host.  These run the passes
- * x, y, z with float32 temporaries (device blocks of the library's, kept
- * per device and stream and reused by the next call on that stream); an
- * axis with sigma <= 0 (Gaussian) or radius 0 is a
- * converting copy.  N for a Gaussian axis is int(8*sigma) rounded up to
- * odd; for the box filters N = 2*radius+1. */
+ * x, y, z with float32 temporaries (the source type for minimum/maximum);
+ * an axis with sigma <= 0 (Gaussian) or radius 0 is skipped, and with no
+ * axis left the call is a converting copy.  N for a Gaussian axis is
+ * int(8*sigma) rounded up to odd; for the box filters N = 2*radius+1.
+ *
+ * Temporaries are device blocks of the library's, kept per (device, stream)
+ * and reused by the next call on that stream (grown when a call needs more)
+ * until clk_release_temporaries(stream).  Calls on one stream from several
+ * host threads are serialised while they issue their passes; calls on
+ * different streams are not.  On a stream under graph capture, and on
+ * hipStreamPerThread, each call allocates its temporaries in stream order
+ * and frees them after its last pass (under capture the graph owns them).
+ * Before destroying a stream of your own on which these calls ran, call
+ * clk_release_temporaries for it (clk_stream_destroy does). */
 int32_t clk_gaussian_blur_3d(void* stream, const clk_image* dst,
                              const clk_image* src, float sigma_x,
                              float sigma_y, float sigma_z);
@@ -269,10 +279,22 @@ typedef struct clk_filter_job {
  * kernel launch per kernel class and stage instead of one per volume and
  * axis.  Results equal those of the single calls bit for bit.  The batch's
  * temporaries and launch tables live in device blocks kept per (device,
- * stream) and reused by the next call on that stream (grown when a call
- * needs more; held until the process ends).  No reference counterpart: the
- * reference leaves launching to its host. */
+ * stream), as the host chain's (see above).  Not for a stream under graph
+ * capture (CLK_INVALID_OPERATION): the launch tables go up from reused host
+ * staging memory.  No reference counterpart: the reference leaves launching
+ * to its host. */
 int32_t clk_batch_filter_3d(void* stream, const clk_filter_job* jobs, int32_t njobs);
+
+/* Release the temporaries kept for `stream` on the current device: waits
+ * for the stream, frees its blocks, and returns the library's unused pooled
+ * device memory to the driver.  No reference counterpart. */
+int32_t clk_release_temporaries(void* stream);
+
+/* Bytes of device memory held for temporaries on the current device: kept
+ * for `stream` (*stream_bytes) and by the library's pool as a whole, kept
+ * blocks of every stream plus freed memory not yet returned to the driver
+ * (*pool_bytes).  Either pointer may be NULL.  No reference counterpart. */
+int32_t clk_temporaries_held(void* stream, int64_t* stream_bytes, int64_t* pool_bytes);
 
 /* Launch-shape knobs; results never depend on them.  Keys: "strided_rounds"
  * (y/z blocks per resident slot), "x_sub" (LDS x pass segments per wave),

@@ -16,6 +16,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "libclk_oracle.so")
+# the same source built -O3 for x86-64-v3 (bit-identical results): what
+# bench.py's cpu_baseline leg times
+_FAST_PATH = os.path.join(_HERE, "_build", "libclk_oracle_fast.so")
 
 DTYPE_CODES = {
     np.dtype(np.int8): 0,
@@ -35,6 +38,7 @@ OP_NAMES = {
 }
 
 _lib = None
+_fast = None
 
 
 def build():
@@ -45,38 +49,50 @@ def build():
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(_LIB_PATH):
-            build()
-        L = ctypes.CDLL(_LIB_PATH)
-        i64, i32, f32, vp = ctypes.c_int64, ctypes.c_int, ctypes.c_float, ctypes.c_void_p
-        L.clko_separable.argtypes = [i32, vp, i32, vp, i32, i64, i64, i64, i32, i32, f32, i32]
-        L.clko_separable.restype = i32
-        L.clko_median_box_3d.argtypes = [vp, i32, vp, i32, i64, i64, i64, i32, i32, i32, i32]
-        L.clko_median_box_3d.restype = i32
-        L.clko_sphere_3d.argtypes = [i32, vp, i32, vp, i32, i64, i64, i64, i32, i32, i32, i32]
-        L.clko_sphere_3d.restype = i32
-        L.clko_median_slice_by_slice_box_3d.argtypes = [vp, i32, vp, i32, i64, i64, i64, i32, i32, i32]
-        L.clko_median_slice_by_slice_box_3d.restype = i32
-        L.clko_copy_3d.argtypes = [vp, i32, vp, i32, i64]
-        L.clko_copy_3d.restype = i32
-        L.clko_detect_maxima_3d.argtypes = [vp, i32, vp, i32, i64, i64, i64]
-        L.clko_detect_maxima_3d.restype = i32
-        L.clko_maximum_z_projection.argtypes = [vp, i32, vp, i32, i64, i64, i64]
-        L.clko_maximum_z_projection.restype = i32
-        L.clko_separable_2d.argtypes = [i32, vp, i32, vp, i32, i64, i64, i32, i32, f32, i32]
-        L.clko_separable_2d.restype = i32
-        L.clko_median_box_2d.argtypes = [vp, i32, vp, i32, i64, i64, i32, i32, i32]
-        L.clko_median_box_2d.restype = i32
-        L.clko_expf.argtypes = [f32]
-        L.clko_expf.restype = f32
-        L.clko_convert.argtypes = [f32, i32]
-        L.clko_convert.restype = f32
-        L.clko_div_uniform.argtypes = [f32, f32, f32]
-        L.clko_div_uniform.restype = f32
-        L.clko_num_threads.argtypes = []
-        L.clko_num_threads.restype = i32
-        _lib = L
+        _lib = _load(_LIB_PATH)
     return _lib
+
+
+def fast_lib():
+    """The -O3 x86-64-v3 build of the same restatement (for timing)."""
+    global _fast
+    if _fast is None:
+        _fast = _load(_FAST_PATH)
+    return _fast
+
+
+def _load(path):
+    if not os.path.exists(path):
+        build()
+    L = ctypes.CDLL(path)
+    i64, i32, f32, vp = ctypes.c_int64, ctypes.c_int, ctypes.c_float, ctypes.c_void_p
+    L.clko_separable.argtypes = [i32, vp, i32, vp, i32, i64, i64, i64, i32, i32, f32, i32]
+    L.clko_separable.restype = i32
+    L.clko_median_box_3d.argtypes = [vp, i32, vp, i32, i64, i64, i64, i32, i32, i32, i32]
+    L.clko_median_box_3d.restype = i32
+    L.clko_sphere_3d.argtypes = [i32, vp, i32, vp, i32, i64, i64, i64, i32, i32, i32, i32]
+    L.clko_sphere_3d.restype = i32
+    L.clko_median_slice_by_slice_box_3d.argtypes = [vp, i32, vp, i32, i64, i64, i64, i32, i32, i32]
+    L.clko_median_slice_by_slice_box_3d.restype = i32
+    L.clko_copy_3d.argtypes = [vp, i32, vp, i32, i64]
+    L.clko_copy_3d.restype = i32
+    L.clko_detect_maxima_3d.argtypes = [vp, i32, vp, i32, i64, i64, i64]
+    L.clko_detect_maxima_3d.restype = i32
+    L.clko_maximum_z_projection.argtypes = [vp, i32, vp, i32, i64, i64, i64]
+    L.clko_maximum_z_projection.restype = i32
+    L.clko_separable_2d.argtypes = [i32, vp, i32, vp, i32, i64, i64, i32, i32, f32, i32]
+    L.clko_separable_2d.restype = i32
+    L.clko_median_box_2d.argtypes = [vp, i32, vp, i32, i64, i64, i32, i32, i32]
+    L.clko_median_box_2d.restype = i32
+    L.clko_expf.argtypes = [f32]
+    L.clko_expf.restype = f32
+    L.clko_convert.argtypes = [f32, i32]
+    L.clko_convert.restype = f32
+    L.clko_div_uniform.argtypes = [f32, f32, f32]
+    L.clko_div_uniform.restype = f32
+    L.clko_num_threads.argtypes = []
+    L.clko_num_threads.restype = i32
+    return L
 
 
 def default_threads():
@@ -92,14 +108,15 @@ def _vol(a):
     return a
 
 
-def separable(op, src, dst_dtype, dim, N, s=0.0, nthreads=None):
-    """One single-axis launch of gaussian/mean/minimum/maximum_separable_3d."""
+def separable(op, src, dst_dtype, dim, N, s=0.0, nthreads=None, fast=False):
+    """One single-axis launch of gaussian/mean/minimum/maximum_separable_3d
+    (fast=True: through the -O3 x86-64-v3 build, same results)."""
     if isinstance(op, str):
         op = OP_NAMES[op]
     src = _vol(src)
     dst = np.empty(src.shape, dtype=np.dtype(dst_dtype))
     D, H, W = src.shape
-    rc = lib().clko_separable(
+    rc = (fast_lib() if fast else lib()).clko_separable(
         op, dst.ctypes.data, DTYPE_CODES[dst.dtype], src.ctypes.data, DTYPE_CODES[src.dtype],
         W, H, D, int(dim), int(N), float(s), int(nthreads or default_threads()))
     if rc != 0:

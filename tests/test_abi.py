@@ -127,3 +127,4 @@ def test_autotune_cache_round_trip(tmp_path):
     assert e.value.code == -30
     with pytest.raises(clkmi.ClkError):
         clkmi.autotune_load(tmp_path / "missing.txt")
+

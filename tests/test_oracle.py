@@ -497,3 +497,28 @@ def test_maximum_z_projection_equals_numpy(src_t, dst_t):
     a = random_volume((7, 5, 6), src_t, seed=3)
     want = O.convert(a.max(axis=0, keepdims=True).astype(np.float32), dst_t)
     assert same_bits_or_value(O.maximum_z_projection(a, dst_t), want)
+
+
+@pytest.mark.parametrize("op", [O.OP_GAUSS, O.OP_MEAN, O.OP_MIN, O.OP_MAX])
+def test_fast_build_gives_the_checker_bits(op):
+    """The -O3 x86-64-v3 build that bench.py times computes exactly what the
+    checker build computes (no contraction, no fast-math), specials
+    included: the same bits everywhere except that a NaN may carry another
+    sign or payload."""
+    rng = np.random.default_rng(11 + op)
+    srcs = [rng.integers(0, 65535, (12, 17, 29)).astype(np.uint16),
+            rng.uniform(-1e3, 1e3, (12, 17, 29)).astype(np.float32)]
+    srcs[1].flat[::97] = np.nan
+    srcs[1].flat[5::89] = -0.0
+    srcs[1].flat[7::83] = np.inf
+    for src in srcs:
+        for dst_t in (np.float32, np.uint16, np.int8):
+            for dim in (0, 1, 2):
+                for N, s in ((25, 3.0), (7, 0.0), (13, 1.5)):
+                    a = O.separable(op, src, dst_t, dim, N, s)
+                    b = O.separable(op, src, dst_t, dim, N, s, fast=True)
+                    if a.dtype.kind == "f":
+                        na, nb = np.isnan(a), np.isnan(b)
+                        assert np.array_equal(na, nb), (op, src.dtype, dst_t, dim, N)
+                        a, b = a[~na], b[~nb]
+                    assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), (op, src.dtype, dst_t, dim, N)

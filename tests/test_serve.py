@@ -128,6 +128,67 @@ def test_failed_batch_is_isolated():
     assert srv.used == 0
 
 
+class TempExecutor(HostExecutor):
+    """A host executor that keeps per-stream temporaries the way the library
+    does (grown to the largest batch call run on the stream, held until
+    released) and records the HBM in use at every batch."""
+
+    def __init__(self, server_ref, **kw):
+        super().__init__(server_ref, **kw)
+        self.kept = {}
+        self.peaks = []
+        self.n_release = 0
+
+    def run(self, batch, stream):
+        from clkmi.serve import batch_temp_bytes
+        srv = self.server_ref[0]
+        with self.lock:
+            self.kept[id(stream)] = max(self.kept.get(id(stream), 0), batch_temp_bytes(batch))
+            self.peaks.append(srv.used + sum(self.kept.values()))
+        return super().run(batch, stream)
+
+    def held(self, stream):
+        with self.lock:
+            return self.kept.get(id(stream), 0), sum(self.kept.values())
+
+    def release(self, stream):
+        with self.lock:
+            self.kept[id(stream)] = 0
+            self.n_release += 1
+
+
+def test_kept_temporaries_count_against_the_budget():
+    """Outputs in flight plus the temporaries kept per stream never exceed
+    the budget; idle streams give theirs back when the head request would
+    not fit otherwise, and close() releases every stream's."""
+    from clkmi.serve import FilterServer, batch_temp_bytes
+    one = small_request(0)
+    per = one.footprint() + batch_temp_bytes([one])
+    budget = int(2.5 * per)  # two streams' temporaries leave no room for a third's
+    ref = [None]
+    ex = TempExecutor(ref, delay=0.02)
+    # at least four batches: the third stream's block only fits once an
+    # idle stream has given its block back
+    srv = FilterServer(ex, budget_bytes=budget, max_batch=4, n_streams=3)
+    ref[0] = srv
+    reqs = [srv.submit(small_request(i)) for i in range(16)]
+    assert srv.drain(timeout=60)
+    srv.close()
+    assert all(r.error is None for r in reqs)
+    assert max(ex.peaks) <= budget
+    assert srv.releases > 0
+    assert sum(ex.kept.values()) == 0
+    for r in reqs:
+        assert np.array_equal(r.result, oracle_chain(r.src.data, r.chain))
+
+
+def test_size_groups_complete_small_requests_first():
+    from clkmi.serve import size_groups
+    reqs = [small_request(i, shape=s) for i, s in enumerate([(40, 40, 40), (4, 4, 4), (9, 9, 9), (5, 5, 5)])]
+    g = size_groups(reqs)
+    assert [[r.voxels for r in grp] for grp in g] == [[64, 125], [729], [64000]]
+
+
 def test_latency_stats():
     from clkmi.serve import latency_stats
     srv, ex = make_server(budget=10**9)
