@@ -38,6 +38,7 @@ EXPORTED = (
     "clk_minimum_box_2d", "clk_maximum_box_2d", "clk_minimum_sphere_3d", "clk_maximum_sphere_3d",
     "clk_mean_sphere_3d", "clk_median_sphere_3d", "clk_median_slice_by_slice_box_3d", "clk_copy_3d",
     "clk_detect_maxima_3d", "clk_maximum_z_projection", "clk_release_temporaries", "clk_temporaries_held",
+    "clk_filter_3d_noalloc",
 )
 
 CLK_JOB_GAUSSIAN_BLUR_3D, CLK_JOB_MEAN_BOX_3D, CLK_JOB_MINIMUM_BOX_3D, CLK_JOB_MAXIMUM_BOX_3D, \
@@ -142,6 +143,8 @@ def lib():
             fn.restype = i32
         L.clk_batch_filter_3d.argtypes = [vp, ctypes.POINTER(ClkFilterJob), i32]
         L.clk_batch_filter_3d.restype = i32
+        L.clk_filter_3d_noalloc.argtypes = [vp, ctypes.POINTER(ClkFilterJob)]
+        L.clk_filter_3d_noalloc.restype = i32
         L.clk_release_temporaries.argtypes = [vp]
         L.clk_release_temporaries.restype = i32
         L.clk_temporaries_held.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]

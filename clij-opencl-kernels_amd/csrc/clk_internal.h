@@ -134,6 +134,11 @@ hipError_t release_kept(hipStream_t st);
 // freed memory not yet returned to the driver).
 int64_t pool_reserved_bytes();
 
+// The fixed device array of clk_filter_3d_noalloc (noalloc.hip): its size,
+// and its address on the current device (resolved once per device).
+constexpr size_t kNoAllocScratchBytes = CLK_NOALLOC_SCRATCH_BYTES;
+hipError_t noalloc_scratch(void** ptr);
+
 // detect_maxima_3d (detect = true; dst and src of one size) or
 // maximum_z_projection (dst W x H x 1); arguments validated by the caller.
 int32_t launch_consumer(bool detect, hipStream_t st, const clk_image& dst, const clk_image& src, std::string* err);

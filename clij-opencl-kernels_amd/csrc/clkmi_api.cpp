@@ -691,6 +691,54 @@ int32_t clk_batch_filter_3d(void* stream, const clk_filter_job* jobs, int32_t nj
   return rc ? fail(rc, err) : ok();
 }
 
+int32_t clk_filter_3d_noalloc(void* stream, const clk_filter_job* job) {
+  static const char* kName = "clk_filter_3d_noalloc";
+  if (!job) return fail(CLK_INVALID_VALUE, std::string(kName) + ": job is NULL");
+  if (job->kind < CLK_JOB_GAUSSIAN_BLUR_3D || job->kind > CLK_JOB_MEDIAN_BOX_3D || job->reserved != 0)
+    return fail(CLK_INVALID_VALUE, std::string(kName) + ": invalid kind");
+  int32_t rc = check_pair(kName, &job->dst, &job->src);
+  if (rc) return rc;
+  const clk_image& dst = job->dst;
+  const clk_image& src = job->src;
+  hipStream_t st = (hipStream_t)stream;
+  std::string err;
+  if (job->kind == CLK_JOB_MEDIAN_BOX_3D) {  // one launch, no temporaries
+    for (int a = 0; a < 3; ++a)
+      if (job->radius[a] < 0) return fail(CLK_INVALID_VALUE, std::string(kName) + ": negative radius");
+    rc = clk::launch_median(st, dst, src, 2 * job->radius[0] + 1, 2 * job->radius[1] + 1,
+                            2 * job->radius[2] + 1, &err);
+    return rc ? fail(rc, std::string(kName) + ": " + err) : ok();
+  }
+  PassSpec ps[3];
+  const int op = job->kind;  // the separable CLK_JOB_* values equal the op codes
+  for (int a = 0; a < 3; ++a) {
+    if (op == clk::OP_GAUSS) {
+      const float sg = job->sigma[a];
+      ps[a] = {sg > 0, sigma_to_n(sg), sg};
+      if (sg > 0 && ps[a].N > CLK_GAUSSIAN_MAX_TAPS)
+        return fail(CLK_INVALID_VALUE, std::string(kName) + ": sigma too large");
+    } else {
+      const int32_t r = job->radius[a];
+      if (r < 0) return fail(CLK_INVALID_VALUE, std::string(kName) + ": negative radius");
+      ps[a] = {r > 0, 2 * r + 1, 0.f};
+    }
+  }
+  const ChainTemps t = chain_temps(op, dst, src, ps);
+  const size_t need = (size_t)t.ntmp * t.one;
+  if (need > clk::kNoAllocScratchBytes)
+    return fail(CLK_INVALID_VALUE, std::string(kName) + ": the filter needs " + std::to_string(need) +
+                                       " bytes of temporaries, more than the " +
+                                       std::to_string(clk::kNoAllocScratchBytes) + " bytes of its fixed array");
+  void* blk = nullptr;
+  if (need > 0 && voxels(dst) > 0) {
+    const hipError_t e = clk::noalloc_scratch(&blk);
+    if (e != hipSuccess)
+      return fail(CLK_OUT_OF_RESOURCES, std::string(kName) + ": device array not found: " + hipGetErrorString(e));
+  }
+  rc = run_chain(op, st, dst, src, ps, t, blk, false, &err);
+  return rc ? fail(rc, std::string(kName) + ": " + err) : ok();
+}
+
 int32_t clk_release_temporaries(void* stream) {
   const hipError_t e = clk::release_kept((hipStream_t)stream);
   if (e != hipSuccess)

@@ -285,6 +285,31 @@ typedef struct clk_filter_job {
  * to its host. */
 int32_t clk_batch_filter_3d(void* stream, const clk_filter_job* jobs, int32_t njobs);
 
+/* Size of the fixed device array that holds the temporaries of
+ * clk_filter_3d_noalloc (one per GPU the library runs on, held until the
+ * process ends): 128 MiB. */
+#define CLK_NOALLOC_SCRATCH_BYTES ((int64_t)128 << 20)
+
+/* clk_filter_3d_noalloc(stream, job): the 3D filter of one clk_filter_job
+ * (kinds as for clk_batch_filter_3d), with the results of the matching
+ * clk_gaussian_blur_3d / clk_mean_box_3d / clk_minimum_box_3d /
+ * clk_maximum_box_3d / clk_median_box_3d call, bit for bit, but calling no
+ * allocator: the temporaries (two images of float32, or of the source type
+ * for minimum/maximum, when three axes are filtered; one for two axes) live
+ * in a fixed device array of CLK_NOALLOC_SCRATCH_BYTES per GPU, and no
+ * launch shape is timed (tuned shapes chosen earlier are used).  A job whose
+ * temporaries exceed the array is rejected with CLK_INVALID_VALUE (float32
+ * temporaries of a 256^3 volume fit exactly).  The array's address on each
+ * GPU is resolved on the first call there.  Every call on one GPU shares
+ * that GPU's array and leaves its contents behind; nothing in it needs to
+ * start at zero (each pass writes all of its output).  Calls on one stream
+ * are ordered; calls on different streams of one GPU whose work overlaps in
+ * time race on the array and must not be made.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). */
+int32_t clk_filter_3d_noalloc(void* stream, const clk_filter_job* job);
+
 /* Release the temporaries kept for `stream` on the current device: waits
  * for the stream, frees its blocks, and returns the library's unused pooled
  * device memory to the driver.  No reference counterpart. */

@@ -128,3 +128,24 @@ def test_autotune_cache_round_trip(tmp_path):
     with pytest.raises(clkmi.ClkError):
         clkmi.autotune_load(tmp_path / "missing.txt")
 
+
+def test_noalloc_validation_without_device():
+    """clk_filter_3d_noalloc checks its job on the host: NULL, bad kind,
+    overlap, and temporaries beyond its fixed array (checked before any
+    device call)."""
+    from clkmi import _lib
+    L = _lib.lib()
+    assert L.clk_filter_3d_noalloc(None, None) == -30
+    j = _lib.ClkFilterJob()
+    j.kind = 0
+    j.src = _lib.ClkImage(0x100000000, 3, 0, 256, 256, 257)
+    j.dst = _lib.ClkImage(0x200000000, 4, 0, 256, 256, 257)
+    for k in range(3):
+        j.sigma[k] = 1.0
+    assert L.clk_filter_3d_noalloc(None, ctypes.byref(j)) == -30
+    assert b"fixed array" in L.clk_last_error_string()
+    j.kind = 9
+    assert L.clk_filter_3d_noalloc(None, ctypes.byref(j)) == -30
+    j.kind = 2
+    j.dst = j.src
+    assert L.clk_filter_3d_noalloc(None, ctypes.byref(j)) == -8
