@@ -256,6 +256,7 @@ __host__ __device__ __forceinline__ int lds_pad(int e) { return e + e / kXK; }
 }  // namespace clk
 #include "sep_xreg.h"
 #include "sep_ystatic.h"
+#include "sep_yring.h"
 namespace clk {
 
 // The taps of 8 consecutive outputs whose windows start at lb (LDS, one
@@ -446,8 +447,9 @@ namespace clk {
 
 // ---- host-side launch of one (op, src type, dst type) ----------------------
 
-// Resident blocks of a kernel on the current device, cached per kernel.
-inline int resident_blocks(const void* kernel, size_t shmem) {
+// Resident blocks of a kernel (launched with `threads` per block) on the
+// current device, cached per kernel.
+inline int resident_blocks(const void* kernel, size_t shmem, int threads = 256) {
   static std::mutex mu;
   static std::map<std::pair<const void*, size_t>, int> cache;
   static int cus = 0;
@@ -462,7 +464,7 @@ inline int resident_blocks(const void* kernel, size_t shmem) {
   auto it = cache.find(key);
   if (it != cache.end()) return it->second;
   int bpc = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kernel, 256, shmem) != hipSuccess || bpc <= 0)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kernel, threads, shmem) != hipSuccess || bpc <= 0)
     bpc = 1;
   cache[key] = bpc * cus;
   return bpc * cus;
@@ -532,10 +534,57 @@ int32_t launch_ystatic_cc(hipStream_t st, SepParams& p, std::string* err) {
   return launch_ystatic_k<OP, TS, TD, C, COLS, 8>(st, p, err);
 }
 
+// The lockstep ring pass (float -> float): K per column width so that the
+// ring and a step's outputs fit the 128 registers a 1024-thread block
+// allows.  Segments: about rounds/8 blocks per resident slot (one block per
+// CU fits), i.e. whole axes for the y passes of large volumes by default.
+template <int OP, class TS, class TD, int C, int COLS>
+int32_t launch_yring_cc(hipStream_t st, SepParams& p, std::string* err) {
+  constexpr int K = COLS == 1 ? 16 : (COLS == 2 ? (C <= 6 ? 8 : 4) : 4);
+  p.ncolgroups = p.inner / COLS;
+  p.ncols_total = p.outer * p.ncolgroups;
+  const int64_t nblk = (p.ncols_total + kYRingThreads - 1) / kYRingThreads;
+  const auto kern = sep_yring_kernel<OP, TS, TD, C, COLS, K>;
+  int64_t target = (int64_t)resident_blocks((const void*)kern, 0, kYRingThreads) * strided_rounds() / 8;
+  plan_segments<K>(p, nblk, target > 0 ? target : 1);
+  const int64_t blocks = nblk * p.nseg;
+  if (blocks > 0x7fffffffLL) {
+    *err = "volume too large for the strided pass grid";
+    return CLK_INVALID_VALUE;
+  }
+  p.order = tuning(TUNE_STRIDED_ORDER);
+  hipLaunchKernelGGL(kern, dim3((uint32_t)blocks), dim3(kYRingThreads), 0, st, p);
+  return 0;
+}
+
+// The lockstep ring pass for float -> float when the yring knob asks for it
+// (2) or by default (0) on y passes (outer > 1), where it measured faster
+// on every placement; one column per lane there, else four where they fit.
+// Returns false when it does not apply.
+template <int OP, class TS, class TD, int C>
+bool launch_yring_c(hipStream_t st, SepParams& p, bool v4, bool v2, bool off32_1024, int32_t* rc,
+                    std::string* err) {
+  if constexpr (std::is_same<TS, float>::value && std::is_same<TD, float>::value) {
+    const int yr = tuning(TUNE_YRING);
+    if (!off32_1024 || !(yr == 2 || (yr == 0 && p.outer > 1))) return false;
+    int want = tuning(TUNE_YSTATIC_COLS);
+    if (want == 0) want = p.outer > 1 ? 1 : 4;
+    if constexpr (C <= 6) {
+      if (want == 4 && v4) { *rc = launch_yring_cc<OP, TS, TD, C, 4>(st, p, err); return true; }
+    }
+    if (want >= 2 && v2) { *rc = launch_yring_cc<OP, TS, TD, C, 2>(st, p, err); return true; }
+    *rc = launch_yring_cc<OP, TS, TD, C, 1>(st, p, err);
+    return true;
+  }
+  return false;
+}
+
 // Wider column groups while the window registers stay within ~112; the
 // ystatic_cols knob can ask for fewer (or, where they fit, four).
 template <int OP, class TS, class TD, int C>
-int32_t launch_ystatic_c(hipStream_t st, SepParams& p, bool v4, bool v2, std::string* err) {
+int32_t launch_ystatic_c(hipStream_t st, SepParams& p, bool v4, bool v2, bool off32_1024, std::string* err) {
+  int32_t rrc = 0;
+  if (launch_yring_c<OP, TS, TD, C>(st, p, v4, v2, off32_1024, &rrc, err)) return rrc;
   constexpr int WN = 8 + 2 * C;
   const int want = tuning(TUNE_YSTATIC_COLS);
   if constexpr ((WN + 8) * 4 <= 112) {
@@ -546,24 +595,24 @@ int32_t launch_ystatic_c(hipStream_t st, SepParams& p, bool v4, bool v2, std::st
 }
 
 template <int OP, class TS, class TD>
-int32_t launch_ystatic(hipStream_t st, SepParams& p, bool v4, bool v2, std::string* err) {
+int32_t launch_ystatic(hipStream_t st, SepParams& p, bool v4, bool v2, bool off32_1024, std::string* err) {
   if constexpr (std::is_same<TS, TD>::value || std::is_same<TD, float>::value) {
     switch (p.c) {
-      case 1: return launch_ystatic_c<OP, TS, TD, 1>(st, p, v4, v2, err);
-      case 2: return launch_ystatic_c<OP, TS, TD, 2>(st, p, v4, v2, err);
-      case 3: return launch_ystatic_c<OP, TS, TD, 3>(st, p, v4, v2, err);
-      case 4: return launch_ystatic_c<OP, TS, TD, 4>(st, p, v4, v2, err);
-      case 5: return launch_ystatic_c<OP, TS, TD, 5>(st, p, v4, v2, err);
-      case 6: return launch_ystatic_c<OP, TS, TD, 6>(st, p, v4, v2, err);
-      case 7: return launch_ystatic_c<OP, TS, TD, 7>(st, p, v4, v2, err);
-      case 8: return launch_ystatic_c<OP, TS, TD, 8>(st, p, v4, v2, err);
+      case 1: return launch_ystatic_c<OP, TS, TD, 1>(st, p, v4, v2, off32_1024, err);
+      case 2: return launch_ystatic_c<OP, TS, TD, 2>(st, p, v4, v2, off32_1024, err);
+      case 3: return launch_ystatic_c<OP, TS, TD, 3>(st, p, v4, v2, off32_1024, err);
+      case 4: return launch_ystatic_c<OP, TS, TD, 4>(st, p, v4, v2, off32_1024, err);
+      case 5: return launch_ystatic_c<OP, TS, TD, 5>(st, p, v4, v2, off32_1024, err);
+      case 6: return launch_ystatic_c<OP, TS, TD, 6>(st, p, v4, v2, off32_1024, err);
+      case 7: return launch_ystatic_c<OP, TS, TD, 7>(st, p, v4, v2, off32_1024, err);
+      case 8: return launch_ystatic_c<OP, TS, TD, 8>(st, p, v4, v2, off32_1024, err);
     }
     if constexpr (OP == OP_GAUSS) {
       switch (p.c) {
-        case 9: return launch_ystatic_c<OP, TS, TD, 9>(st, p, v4, v2, err);
-        case 10: return launch_ystatic_c<OP, TS, TD, 10>(st, p, v4, v2, err);
-        case 11: return launch_ystatic_c<OP, TS, TD, 11>(st, p, v4, v2, err);
-        case 12: return launch_ystatic_c<OP, TS, TD, 12>(st, p, v4, v2, err);
+        case 9: return launch_ystatic_c<OP, TS, TD, 9>(st, p, v4, v2, off32_1024, err);
+        case 10: return launch_ystatic_c<OP, TS, TD, 10>(st, p, v4, v2, off32_1024, err);
+        case 11: return launch_ystatic_c<OP, TS, TD, 11>(st, p, v4, v2, off32_1024, err);
+        case 12: return launch_ystatic_c<OP, TS, TD, 12>(st, p, v4, v2, off32_1024, err);
       }
     }
   }
@@ -724,9 +773,13 @@ int32_t launch_sep_typed(hipStream_t st, SepParams& p, int axis_is_x, std::strin
     int64_t extra = (256 + ncg_min - 1) / ncg_min;
     if (extra > p.outer - 1) extra = p.outer - 1;
     const bool off32 = (extra * p.L * p.inner + p.inner) * maxbytes < (int64_t(1) << 31);
+    // the same bound for the 1024 lanes of a lockstep ring block
+    int64_t extra1k = (kYRingThreads + ncg_min - 1) / ncg_min;
+    if (extra1k > p.outer - 1) extra1k = p.outer - 1;
+    const bool off32_1024 = (extra1k * p.L * p.inner + p.inner) * maxbytes < (int64_t(1) << 31);
     if (kStaticPair && p.c >= 1 && p.c <= smaxc && off32 && p.L < (int64_t(1) << 30) &&
         strided_variant() == 0) {
-      rc = launch_ystatic<OP, TS, TD>(st, p, v4, v2, err);
+      rc = launch_ystatic<OP, TS, TD>(st, p, v4, v2, off32_1024, err);
       if (rc) return rc;
       const hipError_t e = hipGetLastError();
       if (e != hipSuccess) {

@@ -264,6 +264,7 @@ struct TuneDim {
   int32_t vals[6];
 };
 const TuneDim kYZDims[] = {
+    {TUNE_YRING, 2, {1, 2}},  // float -> float only: the lockstep ring pass or not
     {TUNE_STRIDED_ROUNDS, 6, {2, 4, 8, 16, 32, 64}},
     {TUNE_STRIDED_ORDER, 4, {0, 1, 2, 3}},
     {TUNE_YSTATIC_COLS, 3, {0, 1, 2}},
@@ -328,11 +329,15 @@ int32_t tune_signature(int op, hipStream_t st, const clk_image& dst, const clk_i
     if (rc == 0) *best = kXCands[bi];
     return rc;
   }
-  // current point: rounds, order, cols (the launcher's defaults)
-  Candidate cur{{{TUNE_STRIDED_ROUNDS, 16}, {TUNE_STRIDED_ORDER, 0}, {TUNE_YSTATIC_COLS, 0}}, 3};
+  // current point: rounds, order, cols, ring (the launcher's defaults)
+  const bool float_pair = src.dtype == CLK_FLOAT && dst.dtype == CLK_FLOAT;
+  Candidate cur{{{TUNE_STRIDED_ROUNDS, 16}, {TUNE_STRIDED_ORDER, 0}, {TUNE_YSTATIC_COLS, 0}, {TUNE_YRING, 0}},
+                4};
   for (const TuneDim& d : kYZDims) {
+    if (d.key == TUNE_YRING && !float_pair) continue;  // no ring pass for other pixel types
     Candidate c[6];
-    const int slot = d.key == TUNE_STRIDED_ROUNDS ? 0 : (d.key == TUNE_STRIDED_ORDER ? 1 : 2);
+    const int slot = d.key == TUNE_STRIDED_ROUNDS ? 0
+                     : (d.key == TUNE_STRIDED_ORDER ? 1 : (d.key == TUNE_YSTATIC_COLS ? 2 : 3));
     for (int i = 0; i < d.n; ++i) {
       c[i] = cur;
       c[i].set[slot].value = d.vals[i];

@@ -94,7 +94,7 @@ def test_tuning_knobs_validate_values():
     import clkmi
     for key, good, bad in [("autotune", 1, 2), ("strided_order", 3, 4), ("ystatic_cols", 4, 3),
                            ("xreg_maxc", 16, 17), ("strided_rounds", 64, 0), ("batch_seg", 8, 7),
-                           ("xreg_units", 4, 3), ("x_tasks", 2, 3), ("sphere_direct", 2, 3)]:
+                           ("xreg_units", 4, 3), ("x_tasks", 2, 3), ("sphere_direct", 2, 3), ("yring", 2, 3)]:
         old = clkmi.get_tuning(key)
         try:
             clkmi.set_tuning(key, good)

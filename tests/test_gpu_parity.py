@@ -860,3 +860,40 @@ def test_consumers_by_name_and_errors(gpu):
     with pytest.raises(clkmi.ClkError) as e:   # projection output is named dst_max
         clkmi.execute("maximum_z_projection", {"dst": dp, "src": si})
     assert e.value.code == -52
+
+
+RING_SETTINGS = [
+    (("yring", 2), ("ystatic_cols", 1)),
+    (("yring", 2), ("ystatic_cols", 2), ("strided_order", 1)),
+    (("yring", 2), ("ystatic_cols", 4), ("strided_rounds", 64), ("strided_order", 3)),
+    (("yring", 2), ("ystatic_cols", 0), ("strided_rounds", 16), ("strided_order", 2)),
+]
+
+
+@pytest.mark.parametrize("shape", [(37, 41, 8), (20, 300, 12), (70, 9, 6), (6, 70, 1030), (130, 33, 40)])
+def test_lockstep_ring_pass(gpu, shape):
+    """The float32 y/z pass in 1024-thread lockstep blocks: every op, half
+    widths up to its limit, one, two or four columns per lane, blocks with
+    idle lanes (fewer columns than 1024 per block), several segments per
+    axis and every block order, along y and z; bit for bit as the oracle."""
+    import clkmi
+    src = random_volume(shape, np.float32, seed=shape[0] * 7 + shape[2])
+    src.flat[::131] = -0.0
+    for name in SEP_OPS:
+        cmax = 12 if name.startswith("gaussian") else 8
+        for c in (1, 2, 3, 6, 7, 8, 12):
+            if c > cmax:
+                continue
+            N = 2 * c + 1
+            for dim in (1, 2):
+                want = O.separable(name, src, np.float32, dim, N, 0.4 * c + 0.3)
+                for settings in RING_SETTINGS:
+                    old = {k: clkmi.get_tuning(k) for k, _ in settings}
+                    try:
+                        for k, v in settings:
+                            clkmi.set_tuning(k, v)
+                        got = gpu_separable(name, src, np.float32, dim, N, 0.4 * c + 0.3)
+                    finally:
+                        for k, v in old.items():
+                            clkmi.set_tuning(k, v)
+                    assert_parity(got, want, "%s %s dim=%d c=%d %s" % (name, shape, dim, c, settings))
