@@ -21,18 +21,16 @@
 #include <vector>
 
 #include "../../include/clkmi.h"
+#include "batch.h"
 #include "clk_device.h"
 #include "clk_internal.h"
 #include "clk_tuning.h"
 #include "median_impl.h"
 #include "sep_impl.h"
+#include "batch_x.h"
 
 namespace clk {
 
-struct BatchTile {
-  uint32_t job;   // index into the class's parameter array
-  uint32_t vblk;  // block index within that job's launch
-};
 
 template <int OP, class TS, class TD>
 __global__ __launch_bounds__(256) void batch_x_kernel(const SepParams* __restrict__ ps,
@@ -47,7 +45,7 @@ __global__ __launch_bounds__(256) void batch_xpack_kernel(const SepParams* __res
                                                           const BatchTile* __restrict__ tiles) {
   extern __shared__ uint32_t lds[];
   const BatchTile t = tiles[blockIdx.x];
-  sep_xpack_body<OP, TS, TD>(ps[t.job], t.vblk, lds);
+  sep_xrows_body<OP, TS, TD>(ps[t.job], t.vblk, lds);
 }
 
 template <int OP, class TS, class TD, int COLS>
@@ -85,15 +83,16 @@ __global__ __launch_bounds__(64) void batch_hist8_kernel(const MedParams* __rest
 
 namespace {
 
-enum ClassKind : int { K_X = 0, K_STRIDED4 = 1, K_STRIDED1 = 2, K_MEDIAN = 3, K_XPACK = 4 };
+enum ClassKind : int { K_X = 0, K_STRIDED4 = 1, K_STRIDED1 = 2, K_MEDIAN = 3, K_XPACK = 4, K_YDYN = 5 };
 
 // Pixel-type pairs with batched kernels: same type, to float, from float.
 bool batch_pair(int ts, int td) { return ts == td || td == CLK_FLOAT || ts == CLK_FLOAT; }
 
 struct ClassKey {
   int kind, op, ts, td;  // op: separable op, or MedKind for medians
+  int tier = 0, cols = 0;  // K_YDYN: half-width tier and columns per lane
   bool operator<(const ClassKey& o) const {
-    return std::tie(kind, op, ts, td) < std::tie(o.kind, o.op, o.ts, o.td);
+    return std::tie(kind, op, ts, td, tier, cols) < std::tie(o.kind, o.op, o.ts, o.td, o.tier, o.cols);
   }
 };
 
@@ -196,7 +195,11 @@ void med_dst(hipStream_t st, const ClassKey& k, const ClassWork& w, char* dev) {
   }
 }
 
-void launch_class(hipStream_t st, const ClassKey& k, const ClassWork& w, char* dev) {
+int32_t launch_class(hipStream_t st, const ClassKey& k, const ClassWork& w, char* dev, std::string* err) {
+  if (k.kind == K_YDYN)
+    return launch_ydyn_class(st, k.op, k.ts, k.td, k.tier, k.cols,
+                             reinterpret_cast<const SepParams*>(dev + w.params_off),
+                             reinterpret_cast<const BatchTile*>(dev + w.tiles_off), (uint32_t)w.tiles.size(), err);
   if (k.kind == K_MEDIAN) {
     switch (k.ts) {
       case CLK_CHAR: med_dst<int8_t>(st, k, w, dev); break;
@@ -205,7 +208,7 @@ void launch_class(hipStream_t st, const ClassKey& k, const ClassWork& w, char* d
       case CLK_USHORT: med_dst<uint16_t>(st, k, w, dev); break;
       case CLK_FLOAT: med_dst<float>(st, k, w, dev); break;
     }
-    return;
+    return 0;
   }
   switch (k.op) {
     case OP_GAUSS: sep_src<OP_GAUSS>(st, k, w, dev); break;
@@ -213,6 +216,7 @@ void launch_class(hipStream_t st, const ClassKey& k, const ClassWork& w, char* d
     case OP_MIN: sep_src<OP_MIN>(st, k, w, dev); break;
     case OP_MAX: sep_src<OP_MAX>(st, k, w, dev); break;
   }
+  return 0;
 }
 
 // One single-axis pass of a job.  src_tmp/dst_tmp >= 0 are byte offsets
@@ -341,10 +345,10 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
       uint32_t blocks = 0;
       size_t lds = 0;
       if (axis_is_x && p.L <= 256 && p.c <= 256 && tuning(TUNE_NO_XPACK) == 0) {
-        key.kind = K_XPACK;  // short rows: several per wave
+        key.kind = K_XPACK;  // short rows: several per wave, vector loads and stores of their span
         plan_xpack(p);
         blocks = (uint32_t)((p.ntasks + 3) / 4);
-        lds = (size_t)4 * p.lds_stride * 4;
+        lds = (size_t)4 * p.lds_stride * 4 + 4 * (512 * (size_t)clk_dtype_size(ps.dst.dtype) + 16);
       } else if (axis_is_x && p.c <= kMaxCX) {
         key.kind = K_X;
         const int64_t W = p.L;
@@ -356,6 +360,29 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
         p.vec_st = (W % kXK == 0) && ((uintptr_t)p.dst % (kXK * ds) == 0);
         blocks = (uint32_t)((p.ntasks + 3) / 4);
         lds = (size_t)4 * p.lds_stride * 4;
+      } else if (ydyn_tier(p.c) > 0 && ydyn_pair(ps.op, ps.src.dtype, ps.dst.dtype) && tuning(TUNE_STRIDED_VARIANT) == 0) {
+        // y/z: the ring body for a run-time half width (one class per tier
+        // and column width, whatever each job's c)
+        const int ss = clk_dtype_size(ps.src.dtype), ds = clk_dtype_size(ps.dst.dtype);
+        const bool v4 = (p.inner % 4 == 0) && ((uintptr_t)p.src % (4 * ss) == 0) &&
+                        ((uintptr_t)p.dst % (4 * ds) == 0);
+        const bool v2 = (p.inner % 2 == 0) && ((uintptr_t)p.src % (2 * ss) == 0) &&
+                        ((uintptr_t)p.dst % (2 * ds) == 0);
+        key.kind = K_YDYN;
+        key.tier = ydyn_tier(p.c);
+        key.cols = ydyn_cols(key.tier, v4, v2);
+        const int K = ydyn_k(key.tier, key.cols);
+        if (ps.op == OP_GAUSS) set_tier_weights(p, key.tier);
+        p.ncolgroups = p.inner / key.cols;
+        p.ncols_total = p.outer * p.ncolgroups;
+        const int64_t nblk = (p.ncols_total + 255) / 256;
+        int64_t seg = ((int64_t)bseg + K - 1) / K * K;
+        if (seg > (p.L + K - 1) / K * K) seg = (p.L + K - 1) / K * K;
+        const int64_t nseg = (p.L + seg - 1) / seg;
+        p.seg = (int32_t)seg;
+        set_nseg(p, nseg);
+        p.nblk = (uint32_t)nblk;
+        blocks = (uint32_t)(nblk * nseg);
       } else {
         const int ss = clk_dtype_size(ps.src.dtype), ds = clk_dtype_size(ps.dst.dtype);
         const bool v4 = (p.inner % 4 == 0) && ((uintptr_t)p.src % (4 * ss) == 0) &&
@@ -484,7 +511,8 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
         rc = CLK_INVALID_VALUE;
         break;
       }
-      launch_class(st, kv.first, kv.second, dev_tables);
+      rc = launch_class(st, kv.first, kv.second, dev_tables, err);
+      if (rc) break;
       const hipError_t e = hipGetLastError();
       if (e != hipSuccess) {
         *err = std::string("batch launch failed: ") + hipGetErrorString(e);

@@ -1,0 +1,30 @@
+// Dispatch of a batch y/z tier class to the per-op compilation units.
+#include <string>
+
+#include "batch.h"
+#include "clk_device.h"
+
+namespace clk {
+
+bool launch_ydyn_gauss(hipStream_t, int, int, int, int, const SepParams*, const BatchTile*, uint32_t);
+bool launch_ydyn_mean(hipStream_t, int, int, int, int, const SepParams*, const BatchTile*, uint32_t);
+bool launch_ydyn_min(hipStream_t, int, int, int, int, const SepParams*, const BatchTile*, uint32_t);
+bool launch_ydyn_max(hipStream_t, int, int, int, int, const SepParams*, const BatchTile*, uint32_t);
+
+int32_t launch_ydyn_class(hipStream_t st, int op, int ts, int td, int tier, int cols, const SepParams* ps,
+                          const BatchTile* tiles, uint32_t ntiles, std::string* err) {
+  bool ok = false;
+  switch (op) {
+    case OP_GAUSS: ok = launch_ydyn_gauss(st, ts, td, tier, cols, ps, tiles, ntiles); break;
+    case OP_MEAN: ok = launch_ydyn_mean(st, ts, td, tier, cols, ps, tiles, ntiles); break;
+    case OP_MIN: ok = launch_ydyn_min(st, ts, td, tier, cols, ps, tiles, ntiles); break;
+    case OP_MAX: ok = launch_ydyn_max(st, ts, td, tier, cols, ps, tiles, ntiles); break;
+  }
+  if (!ok) {
+    *err = "internal: no y/z tier class for this pass";
+    return CLK_INVALID_VALUE;
+  }
+  return 0;
+}
+
+}  // namespace clk

@@ -1,0 +1,12 @@
+// Batch y/z tier classes of one op (split per op so the units compile in
+// parallel).
+#include "batch_ydyn.h"
+
+namespace clk {
+
+bool launch_ydyn_mean(hipStream_t st, int ts, int td, int tier, int cols, const SepParams* ps, const BatchTile* tiles,
+                    uint32_t n) {
+  return launch_ydyn_op<OP_MEAN>(st, ts, td, tier, cols, ps, tiles, n);
+}
+
+}  // namespace clk

@@ -257,6 +257,7 @@ __host__ __device__ __forceinline__ int lds_pad(int e) { return e + e / kXK; }
 #include "sep_xreg.h"
 #include "sep_ystatic.h"
 #include "sep_yring.h"
+#include "sep_ydyn.h"
 namespace clk {
 
 // The taps of 8 consecutive outputs whose windows start at lb (LDS, one

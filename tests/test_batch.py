@@ -157,3 +157,34 @@ def test_batch_errors(gpu):
         clkmi.batch_filter_3d([clkmi.FilterJob("minimum_box_3d", di, si, radius=(1, -1, 1))])
     assert e.value.code == -30
     clkmi.batch_filter_3d([])
+
+
+def test_batch_yz_tier_classes(gpu):
+    """The y/z tier classes (a run-time half width up to 5, 8 or 16, one
+    launch per tier) give the generic strided classes' bits and the
+    oracle's: half widths 1..16 and beyond the last tier, odd, even and
+    multiple-of-four inner extents (1, 2 or 4 columns per lane), several
+    segments along an axis."""
+    import clkmi
+    rng = np.random.default_rng(21)
+    jobs = []
+    pairs = [(np.uint16, np.float32), (np.float32, np.float32), (np.uint8, np.uint8), (np.float32, np.uint16),
+             (np.int16, np.int16), (np.uint16, np.uint16)]
+    for i in range(40):
+        shape = (int(rng.integers(3, 90)), int(rng.integers(3, 50)), int(rng.integers(3, 70)))
+        kind = ["gaussian_blur_3d", "minimum_box_3d", "maximum_box_3d", "mean_box_3d"][i % 4]
+        st, dt = pairs[i % len(pairs)]
+        src = random_volume(shape, st, seed=2100 + i, lo=0 if np.dtype(st).kind == "u" else None)
+        sigma = tuple(float(v) for v in rng.uniform(0.3, 8.6, 3))
+        radius = tuple(int(v) for v in rng.integers(1, 19, 3))
+        jobs.append((kind, src, dt, sigma, radius))
+    got = run_batch(jobs)
+    old = clkmi.get_tuning("strided_variant")
+    try:
+        clkmi.set_tuning("strided_variant", 1)
+        generic = run_batch(jobs)
+    finally:
+        clkmi.set_tuning("strided_variant", old)
+    for i, (job, g, h) in enumerate(zip(jobs, got, generic)):
+        assert np.array_equal(g.view(np.uint8), h.view(np.uint8)), "job %d tiers != generic" % i
+        assert same_bits_or_value(g, oracle_job(*job)), "job %d %s %s" % (i, job[0], job[1].shape)
