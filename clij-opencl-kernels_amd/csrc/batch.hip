@@ -25,6 +25,7 @@
 #include "clk_device.h"
 #include "clk_internal.h"
 #include "clk_tuning.h"
+#include "median_exact.h"
 #include "median_impl.h"
 #include "sep_impl.h"
 #include "batch_x.h"
@@ -523,6 +524,28 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
     for (const Single& sg : singles) {
       if (sg.stage != s || rc) continue;
       rc = launch_separable(sg.pass.op, st, sg.pass.dst, sg.pass.src, sg.pass.dim, sg.pass.N, sg.pass.s, err);
+    }
+  }
+  // float -> float medians: the NaN / -0 fix-up per job, one flag buffer
+  // reused in stream order
+  int64_t flag_bytes = 0;
+  for (auto& mj : medians) {
+    const clk_filter_job& jb = mj.first;
+    if (jb.src.dtype == CLK_FLOAT && jb.dst.dtype == CLK_FLOAT) {
+      const int64_t b = nanzero_flag_bytes(jb.dst);
+      if (b > flag_bytes) flag_bytes = b;
+    }
+  }
+  if (rc == 0 && flag_bytes > 0) {
+    void* flags = nullptr;
+    if (scratch.get(0, (size_t)flag_bytes, &flags) != hipSuccess) {
+      *err = "median brick flags: allocation failed";
+      return CLK_MEM_OBJECT_ALLOCATION_FAILURE;
+    }
+    for (auto& mj : medians) {
+      const clk_filter_job& jb = mj.first;
+      rc = launch_median_nanzero(st, jb.dst, jb.src, jb.radius[0], jb.radius[1], jb.radius[2], false, flags, err);
+      if (rc) break;
     }
   }
   return rc;

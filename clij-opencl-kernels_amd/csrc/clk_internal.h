@@ -124,7 +124,7 @@ class CallScratch {
   std::unique_lock<std::recursive_mutex> lock_;
   void* owned_[3] = {nullptr, nullptr, nullptr};  // transient mode
 };
-constexpr int kScratchSlots = 3;  // 0: host chain, 1: batch temporaries, 2: batch tables
+constexpr int kScratchSlots = 3;  // 0: host chain / median brick flags, 1: batch temporaries, 2: batch tables
 
 // Kept blocks of one stream on the current device: bytes held, and release
 // (synchronises the stream, frees the blocks, trims the pool).

@@ -14,6 +14,7 @@
 #include "../../include/clkmi.h"
 #include "clk_device.h"
 #include "clk_internal.h"
+#include "median_exact.h"
 
 namespace clk {
 
@@ -394,6 +395,21 @@ int32_t clk_maximum_separable_3d(void* stream, const clk_image* dst, const clk_i
   return separable(clk::OP_MAX, "maximum_separable_3d", stream, dst, src, dim, N, s);
 }
 
+// Float -> float medians: the NaN / -0 scan and fix-up after the median
+// launch (median_exact.h), its brick flags in the stream's kept blocks.
+// (ex, ey, ez) are the half widths (N - 1) / 2 of the launch.
+static int32_t median_nanzero(const char* name, hipStream_t st, const clk_image& dst, const clk_image& src, int ex,
+                              int ey, int ez, bool sphere) {
+  if (src.dtype != CLK_FLOAT || dst.dtype != CLK_FLOAT || voxels(dst) == 0) return ok();
+  clk::CallScratch scratch(st);  // held until the fix-up is issued
+  void* flags = nullptr;
+  if (scratch.get(0, (size_t)clk::nanzero_flag_bytes(dst), &flags) != hipSuccess)
+    return fail(CLK_MEM_OBJECT_ALLOCATION_FAILURE, std::string(name) + ": temporary allocation failed");
+  std::string err;
+  const int32_t rc = clk::launch_median_nanzero(st, dst, src, ex, ey, ez, sphere, flags, &err);
+  return rc ? fail(rc, std::string(name) + ": " + err) : ok();
+}
+
 int32_t clk_median_box_3d(void* stream, const clk_image* dst, const clk_image* src, int32_t Nx,
                           int32_t Ny, int32_t Nz) {
   int32_t rc = check_pair("median_box_3d", dst, src);
@@ -401,7 +417,8 @@ int32_t clk_median_box_3d(void* stream, const clk_image* dst, const clk_image* s
   std::string err;
   rc = clk::launch_median((hipStream_t)stream, *dst, *src, Nx, Ny, Nz, &err);
   if (rc) return fail(rc, err);
-  return ok();
+  return median_nanzero("median_box_3d", (hipStream_t)stream, *dst, *src, (Nx - 1) / 2, (Ny - 1) / 2, (Nz - 1) / 2,
+                        false);
 }
 
 int32_t clk_gaussian_blur_separable_2d(void* stream, const clk_image* dst, const clk_image* src,
@@ -433,7 +450,7 @@ int32_t clk_median_box_2d(void* stream, const clk_image* dst, const clk_image* s
   std::string err;
   rc = clk::launch_median((hipStream_t)stream, *dst, *src, Nx, Ny, 1, &err, true);
   if (rc) return fail(rc, err);
-  return ok();
+  return median_nanzero("median_box_2d", (hipStream_t)stream, *dst, *src, (Nx - 1) / 2, (Ny - 1) / 2, 0, false);
 }
 
 static int32_t sphere(int op, const char* name, void* stream, const clk_image* dst, const clk_image* src,
@@ -443,6 +460,8 @@ static int32_t sphere(int op, const char* name, void* stream, const clk_image* d
   std::string err;
   rc = clk::launch_sphere(op, (hipStream_t)stream, *dst, *src, Nx, Ny, Nz, &err);
   if (rc) return fail(rc, std::string(name) + ": " + err);
+  if (op == 3)
+    return median_nanzero(name, (hipStream_t)stream, *dst, *src, (Nx - 1) / 2, (Ny - 1) / 2, (Nz - 1) / 2, true);
   return ok();
 }
 
@@ -475,7 +494,8 @@ int32_t clk_median_slice_by_slice_box_3d(void* stream, const clk_image* dst, con
   // (median_slice_by_slice_box_3d_x.cl:46)
   rc = clk::launch_median((hipStream_t)stream, *dst, *src, Nx, Ny, 1, &err, true);
   if (rc) return fail(rc, "median_slice_by_slice_box_3d: " + err);
-  return ok();
+  return median_nanzero("median_slice_by_slice_box_3d", (hipStream_t)stream, *dst, *src, (Nx - 1) / 2, (Ny - 1) / 2,
+                        0, false);
 }
 
 int32_t clk_copy_3d(void* stream, const clk_image* dst, const clk_image* src) {
@@ -702,11 +722,24 @@ int32_t clk_filter_3d_noalloc(void* stream, const clk_filter_job* job) {
   const clk_image& src = job->src;
   hipStream_t st = (hipStream_t)stream;
   std::string err;
-  if (job->kind == CLK_JOB_MEDIAN_BOX_3D) {  // one launch, no temporaries
+  if (job->kind == CLK_JOB_MEDIAN_BOX_3D) {
     for (int a = 0; a < 3; ++a)
       if (job->radius[a] < 0) return fail(CLK_INVALID_VALUE, std::string(kName) + ": negative radius");
     rc = clk::launch_median(st, dst, src, 2 * job->radius[0] + 1, 2 * job->radius[1] + 1,
                             2 * job->radius[2] + 1, &err);
+    if (rc) return fail(rc, std::string(kName) + ": " + err);
+    if (src.dtype != CLK_FLOAT || dst.dtype != CLK_FLOAT || voxels(dst) == 0) return ok();
+    // float -> float: the NaN / -0 fix-up, its brick flags in the fixed array
+    const size_t need = (size_t)clk::nanzero_flag_bytes(dst);
+    if (need > clk::kNoAllocScratchBytes)
+      return fail(CLK_INVALID_VALUE, std::string(kName) + ": the median needs " + std::to_string(need) +
+                                         " bytes of brick flags, more than the " +
+                                         std::to_string(clk::kNoAllocScratchBytes) + " bytes of its fixed array");
+    void* flags = nullptr;
+    const hipError_t e = clk::noalloc_scratch(&flags);
+    if (e != hipSuccess)
+      return fail(CLK_OUT_OF_RESOURCES, std::string(kName) + ": device array not found: " + hipGetErrorString(e));
+    rc = clk::launch_median_nanzero(st, dst, src, job->radius[0], job->radius[1], job->radius[2], false, flags, &err);
     return rc ? fail(rc, std::string(kName) + ": " + err) : ok();
   }
   PassSpec ps[3];

@@ -98,7 +98,10 @@ int32_t clk_maximum_separable_3d(void* stream, const clk_image* dst,
 
 /* median_box_3d(dst, src, Nx, Ny, Nz)
  * replaces kernels/median_box_3d_x.cl:23-30.  The host define
- * MAX_ARRAY_SIZE (median_box_3d_x.cl:36) is not needed: any box works. */
+ * MAX_ARRAY_SIZE (median_box_3d_x.cl:36) is not needed: any box works.
+ * Float -> float medians keep the reference's insertion-sort result where
+ * windows hold NaN or -0 (a NaN splits the sort, ties keep gather order),
+ * as do the 2D, slice-by-slice and ellipsoid medians below. */
 int32_t clk_median_box_3d(void* stream, const clk_image* dst,
                           const clk_image* src, int32_t Nx, int32_t Ny,
                           int32_t Nz);
@@ -302,7 +305,8 @@ int32_t clk_batch_filter_3d(void* stream, const clk_filter_job* jobs, int32_t nj
  * temporaries of a 256^3 volume fit exactly).  The array's address on each
  * GPU is resolved on the first call there.  Every call on one GPU shares
  * that GPU's array and leaves its contents behind; nothing in it needs to
- * start at zero (each pass writes all of its output).  Calls on one stream
+ * start at zero (each pass writes all of its output; a float -> float
+ * median keeps one flag byte per 8x8x8 brick there, all rewritten first).  Calls on one stream
  * are ordered; calls on different streams of one GPU whose work overlaps in
  * time race on the array and must not be made.
  * No counterpart in the reference: for callers that cannot allocate device

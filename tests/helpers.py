@@ -57,3 +57,32 @@ def same_bits_or_value(a, b):
     if a.dtype.kind == "f":
         return np.array_equal(a, b, equal_nan=True)
     return np.array_equal(a, b)
+
+
+def nanzero_volume(shape, kind, seed):
+    """Float volumes for the insertion-sort median rules on NaN and -0:
+    'ties' draws from {-0, +0, -1, 1, 2, NaN} (zeros at the median position,
+    windows split by NaNs); 'sparse' is uniform with a few -0 and NaN (most
+    8x8x8 bricks hold neither); 'slab' has NaN over a z slab (windows whose
+    middle member is NaN) and -0 / +0 runs elsewhere."""
+    rng = np.random.default_rng(seed)
+    if kind == "ties":
+        vals = np.array([-0.0, 0.0, -1.0, 1.0, 2.0, np.nan], dtype=np.float32)
+        return vals[rng.choice(6, size=shape, p=[0.25, 0.2, 0.15, 0.15, 0.1, 0.15])]
+    a = rng.uniform(-1000, 1000, size=shape).astype(np.float32)
+    flat = a.reshape(-1)
+    if kind == "sparse":
+        flat[rng.choice(flat.size, max(1, flat.size // 900), replace=False)] = np.nan
+        flat[rng.choice(flat.size, max(1, flat.size // 700), replace=False)] = -0.0
+        return a
+    z = shape[0]
+    a[z // 3: z // 3 + 3] = np.nan
+    zeros = rng.random(shape) < 0.4
+    a[zeros & (a == a)] = np.where(rng.random(int((zeros & (a == a)).sum())) < 0.5, -0.0, 0.0)
+    return a
+
+
+def same_float_bits(a, b):
+    """Floats equal bit for bit (sign of zero included), NaN == NaN."""
+    an, bn = np.isnan(a), np.isnan(b)
+    return bool(np.array_equal(an, bn) and np.array_equal(a.view(np.uint32)[~an], b.view(np.uint32)[~bn]))

@@ -12,7 +12,7 @@ from scipy import ndimage
 import clk_oracle as O
 import zlib
 
-from helpers import DTYPES, random_volume, same_bits_or_value
+from helpers import DTYPES, nanzero_volume, random_volume, same_bits_or_value, same_float_bits
 
 SHAPES = [(11, 13, 17), (5, 1, 9), (1, 1, 23)]
 
@@ -257,6 +257,88 @@ def test_median_converts_to_dst_before_ranking():
 def test_median_rejects_empty_box():
     with pytest.raises(ValueError):
         O.median_box_3d(np.zeros((2, 2, 2), np.uint8), np.uint8, -1, 3, 3)
+
+
+# ---- float medians over NaN and -0: the insertion sort's closed form --------------
+# The reference sorts with `temp < a[j]` (median_box_3d_x.cl:3-14).  A NaN
+# compares false both ways, so it never moves and nothing moves past it; -0
+# and +0 compare equal, so they keep their order.  The sorted array is the
+# window split at its NaNs, each run sorted stably.  These tests build that
+# array with Python's stable sort, not with an insertion sort.
+
+def _closed_form_median(vals):
+    out, run = [], []
+    for v in vals:
+        if v != v:
+            out += sorted(run)
+            out.append(v)
+            run = []
+        else:
+            run.append(v)
+    out += sorted(run)
+    return out[len(vals) // 2]
+
+
+def _median_by_closed_form(a, offsets):
+    """offsets: (dx, dy, dz) in the reference's gather order (x outer, z
+    inner); clamped reads."""
+    D, H, W = a.shape
+    out = np.empty_like(a)
+    for k in range(D):
+        for j in range(H):
+            for i in range(W):
+                vals = [a[min(max(k + dz, 0), D - 1), min(max(j + dy, 0), H - 1), min(max(i + dx, 0), W - 1)]
+                        for dx, dy, dz in offsets]
+                out[k, j, i] = _closed_form_median(vals)
+    return out
+
+
+def _box_offsets(ex, ey, ez):
+    return [(x, y, z) for x in range(-ex, ex + 1) for y in range(-ey, ey + 1) for z in range(-ez, ez + 1)]
+
+
+@pytest.mark.parametrize("kind", ["ties", "sparse", "slab"])
+@pytest.mark.parametrize("Ns", [(3, 3, 3), (5, 3, 1), (1, 1, 3), (4, 2, 5)])
+def test_median_nan_negzero_closed_form(kind, Ns):
+    a = nanzero_volume((6, 7, 9), kind, seed=sum(Ns))
+    ex, ey, ez = ((n - 1) // 2 for n in Ns)
+    want = _median_by_closed_form(a, _box_offsets(ex, ey, ez))
+    assert same_float_bits(O.median_box_3d(a, np.float32, *Ns), want)
+    want2 = _median_by_closed_form(a, _box_offsets(ex, ey, 0))
+    assert same_float_bits(O.median_slice_by_slice_box_3d(a, np.float32, Ns[0], Ns[1]), want2)
+    assert same_float_bits(O.median_box_2d(a[2], np.float32, Ns[0], Ns[1]), want2[2])
+
+
+@pytest.mark.parametrize("Ns", [(3, 3, 3), (5, 5, 5), (5, 3, 1)])
+def test_sphere_median_nan_negzero_closed_form(Ns):
+    a = nanzero_volume((6, 7, 9), "ties", seed=7 + sum(Ns))
+    ex, ey, ez = ((n - 1) // 2 for n in Ns)
+    fp = _sphere_footprint(*Ns)
+    offs = [(x, y, z) for x, y, z in _box_offsets(ex, ey, ez) if fp[z + ez, y + ey, x + ex]]
+    want = _median_by_closed_form(a, offs)
+    assert same_float_bits(O.sphere_3d("median_sphere_3d", a, np.float32, *Ns), want)
+
+
+def test_median_signed_zero_known_answers():
+    """Worked by hand: all -0 gives -0 everywhere; in a row of +0 with one
+    -0 the -0 is the median only where it is the middle member of the
+    window (ties keep gather order); NaNs stay where they are."""
+    a = np.full((2, 3, 4), -0.0, dtype=np.float32)
+    got = O.median_box_3d(a, np.float32, 3, 3, 3)
+    assert np.all(np.signbit(got)) and np.all(got == 0)
+    # row [+0, +0, -0, +0, +0], window 3x1x1 (members x-1, x, x+1; t = 1):
+    # x=1: [+0, +0, -0] -> +0; x=2: [+0, -0, +0] -> -0; x=3: [-0, +0, +0] -> +0
+    row = np.array([[[0.0, 0.0, -0.0, 0.0, 0.0]]], dtype=np.float32)
+    got = O.median_box_3d(row, np.float32, 3, 1, 1)[0, 0]
+    assert list(np.signbit(got)) == [False, False, True, False, False]
+    # a NaN splits the window: [5, NaN, 1] stays as it is -> NaN at t = 1;
+    # [3, 1, NaN, 2, 0] -> [1, 3, NaN, 0, 2] -> NaN; [2, 1, 0, NaN, 9] -> [0, 1, 2, NaN, 9] -> 2
+    r = np.array([[[5.0, np.nan, 1.0]]], dtype=np.float32)
+    assert np.isnan(O.median_box_3d(r, np.float32, 3, 1, 1)[0, 0, 1])
+    r = np.array([[[3.0, 1.0, np.nan, 2.0, 0.0]]], dtype=np.float32)
+    assert np.isnan(O.median_box_3d(r, np.float32, 5, 1, 1)[0, 0, 2])
+    r = np.array([[[2.0, 1.0, 0.0, np.nan, 9.0]]], dtype=np.float32)
+    assert O.median_box_3d(r, np.float32, 5, 1, 1)[0, 0, 2] == 2.0
 
 
 # ---- the GPU's divide formula equals the IEEE divide -----------------------------------

@@ -9,7 +9,7 @@ import ctypes
 import numpy as np
 import pytest
 
-from helpers import empty_device, from_device, random_volume, to_device
+from helpers import empty_device, from_device, nanzero_volume, random_volume, same_float_bits, to_device
 
 pytestmark = pytest.mark.gpu
 
@@ -91,3 +91,26 @@ def test_noalloc_rejects_temporaries_beyond_the_array(gpu):
     job = c_job("gaussian_blur_3d", di, si, sigma=(1.0, 0.0, 1.0))
     assert L.clk_filter_3d_noalloc(stream, ctypes.byref(job)) == 0, L.clk_last_error_string()
     torch.cuda.synchronize()
+
+
+def test_noalloc_float_median_nan_negzero(gpu):
+    """Float -> float medians with NaN and -0: the brick flags of the fix-up
+    live in the fixed array too; two calls, the second smaller, bit for bit
+    (sign of zero included) against the allocating entry point."""
+    import torch
+    from clkmi import _lib
+    L = _lib.lib()
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    radius = (1, 2, 1)
+    for i, shape in enumerate([(40, 50, 60), (9, 11, 13)]):
+        a = nanzero_volume(shape, "ties", seed=40 + i)
+        ts, si = to_device(a)
+        ta, da = empty_device(shape, np.float32)
+        tb, db = empty_device(shape, np.float32)
+        assert allocating(L, stream, "median_box_3d", da, si, (0.0, 0.0, 0.0), radius) == 0, L.clk_last_error_string()
+        job = c_job("median_box_3d", db, si, (0.0, 0.0, 0.0), radius)
+        assert L.clk_filter_3d_noalloc(stream, ctypes.byref(job)) == 0, L.clk_last_error_string()
+        want = from_device(ta, shape, np.float32)
+        got = from_device(tb, shape, np.float32)
+        assert same_float_bits(got, want), "call %d %s" % (i, shape)
+        assert np.any(np.signbit(got) & (got == 0)), "no -0 medians in the sample"
