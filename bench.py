@@ -237,19 +237,35 @@ def measure_batched(torch, clkmi, steps, warmup, autotune_on=True):
     against one 3D call per volume, each timed with HIP events on the
     launch stream (mean of `steps` runs after `warmup`)."""
     from clkmi import autotune
+    import ctypes
+    from clkmi import _lib
     jobs, bufs = make_c4_jobs(torch, clkmi)
     V = sum(int(j.src.width * j.src.height * j.src.depth) for j in jobs)
     stream = torch.cuda.current_stream()
+    # both sides call the C-ABI with their arguments built beforehand, so the
+    # comparison is of the library's work, not of Python argument packing
+    L = _lib.lib()
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    calls, images = [], []
+    for j in jobs:
+        di, si = clkmi.to_clk_image(j.dst), clkmi.to_clk_image(j.src)
+        images += [di, si]
+        d, s_ = ctypes.byref(di), ctypes.byref(si)  # the C-ABI takes (stream, dst, src, ...)
+        if j.kind == "gaussian_blur_3d":
+            calls.append((L.clk_gaussian_blur_3d, (sp, d, s_) + tuple(ctypes.c_float(v) for v in j.sigma)))
+        else:
+            calls.append((getattr(L, "clk_" + j.kind), (sp, d, s_) + tuple(ctypes.c_int32(v) for v in j.radius)))
+    packed = clkmi.pack_jobs(jobs)
+    jp = ctypes.cast(packed.ptr, ctypes.POINTER(_lib.ClkFilterJob))
 
     def plain():
-        for j in jobs:
-            if j.kind == "gaussian_blur_3d":
-                clkmi.gaussian_blur_3d(j.src, j.dst, *j.sigma, stream=stream)
-            else:
-                getattr(clkmi, j.kind)(j.src, j.dst, *j.radius, stream=stream)
+        for f, a in calls:
+            if f(*a):
+                raise RuntimeError(L.clk_last_error_string())
 
     def batched():
-        clkmi.batch_filter_3d(jobs, stream=stream)
+        if L.clk_batch_filter_3d(sp, jp, packed.n):
+            raise RuntimeError(L.clk_last_error_string())
 
     tuned = []
     if autotune_on:

@@ -26,7 +26,7 @@ __all__ = [
     "minimum_sphere_3d", "maximum_sphere_3d", "mean_sphere_3d", "median_sphere_3d",
     "median_slice_by_slice_box_3d", "copy_3d", "detect_maxima_3d", "maximum_z_projection",
     "release_temporaries", "temporaries_held",
-]
+ "pack_jobs", "PackedJobs",]
 
 # Kernel name -> declared parameters (reference .cl declarations).
 KERNELS = {
@@ -300,21 +300,51 @@ class FilterJob(namedtuple("FilterJob", "kind dst src sigma radius")):
         return super().__new__(cls, kind, dst, src, tuple(sigma), tuple(radius))
 
 
+def _image_fields(img):
+    """(ptr, dtype code, width, height, depth) of an Image or a tensor."""
+    if isinstance(img, Image):
+        return img.ptr, int(img.dtype), int(img.width), int(img.height), int(img.depth)
+    c = to_clk_image(img)
+    return c.ptr or 0, c.dtype, c.width, c.height, c.depth
+
+
+class PackedJobs:
+    """A batch's clk_filter_job array, packed once (numpy, in the C layout)
+    and reusable for every call on the same images."""
+
+    def __init__(self, jobs):
+        import numpy as np
+        n = len(jobs)
+        a = np.zeros(max(1, n), dtype=np.dtype(_lib.ClkFilterJob))
+        if n:
+            a["kind"][:n] = [JOB_KINDS[j.kind] if isinstance(j.kind, str) else int(j.kind) for j in jobs]
+            for side in ("dst", "src"):
+                f = np.array([_image_fields(getattr(j, side)) for j in jobs], dtype=np.int64)
+                col = a[side]
+                col["ptr"][:n] = f[:, 0].astype(np.uint64)
+                col["dtype"][:n] = f[:, 1]
+                col["width"][:n] = f[:, 2]
+                col["height"][:n] = f[:, 3]
+                col["depth"][:n] = f[:, 4]
+            a["sigma"][:n] = [j.sigma for j in jobs]
+            a["radius"][:n] = [j.radius for j in jobs]
+        self.array = a
+        self.n = n
+        self.ptr = ctypes.c_void_p(a.ctypes.data)
+
+
+def pack_jobs(jobs):
+    """Pack a list of FilterJob once for repeated batch_filter_3d calls."""
+    return PackedJobs(jobs)
+
+
 def batch_filter_3d(jobs, stream=None):
     """Run many independent 3D filters as one batch (a few launches per
     pass stage instead of one per volume and axis); results equal the single
-    calls bit for bit."""
-    arr = (_lib.ClkFilterJob * max(1, len(jobs)))()
-    for i, j in enumerate(jobs):
-        a = arr[i]
-        a.kind = JOB_KINDS[j.kind] if isinstance(j.kind, str) else int(j.kind)
-        a.reserved = 0
-        a.dst = to_clk_image(j.dst)
-        a.src = to_clk_image(j.src)
-        for k in range(3):
-            a.sigma[k] = float(j.sigma[k])
-            a.radius[k] = int(j.radius[k])
-    check(lib().clk_batch_filter_3d(_stream_ptr(stream), arr, len(jobs)))
+    calls bit for bit.  `jobs` is a list of FilterJob or a PackedJobs."""
+    packed = jobs if isinstance(jobs, PackedJobs) else PackedJobs(jobs)
+    check(lib().clk_batch_filter_3d(_stream_ptr(stream), ctypes.cast(packed.ptr, ctypes.POINTER(_lib.ClkFilterJob)),
+                                    packed.n))
 
 
 def release_temporaries(stream=None):

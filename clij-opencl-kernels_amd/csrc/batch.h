@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 #include "clk_internal.h"
 
 namespace clk {
@@ -14,6 +16,27 @@ struct BatchTile {
   uint32_t job;
   uint32_t vblk;
 };
+
+// The blocks of a class launch: job j owns blocks off[j] .. off[j+1] - 1
+// (off[0] = 0, off[njobs] = the grid), so the table the host uploads grows
+// with the jobs, not with the blocks.
+struct BatchGrid {
+  const uint32_t* off;
+  uint32_t njobs;
+};
+
+// Job and job-local block of this block: a binary search over off (the
+// same for the whole block, so it stays on the scalar unit).
+__device__ __forceinline__ BatchTile batch_tile(const BatchGrid g) {
+  const uint32_t b = blockIdx.x;
+  uint32_t lo = 0, hi = g.njobs;  // off[lo] <= b < off[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (g.off[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  return {lo, b - g.off[lo]};
+}
 
 // y/z passes with a half width known at run time (sep_ydyn.h): the tier
 // (largest half width compiled: 5, 8 or 16; 0 = none fits), the columns
@@ -28,11 +51,24 @@ inline bool ydyn_pair(int op, int ts, int td) {
   return ts == td || ts == CLK_FLOAT;
 }
 
-// Launch one y/z tier class (a launch of ntiles 256-thread blocks);
+// x passes with a half width known at run time (sep_xtier.h): the tier
+// (largest half width compiled: 4, 8 or 16; 0 = none fits), the pixel-type
+// pairs compiled (to float, or to the same type), and the 64-chunk units
+// each wave filters.
+inline int xtier_tier(int c) { return c <= 4 ? 4 : (c <= 8 ? 8 : (c <= 16 ? 16 : 0)); }
+inline bool xtier_pair(int ts, int td) { return td == CLK_FLOAT || ts == td; }
+constexpr int kXTierUnits = 2;
+
+// Launch one x tier class (nblocks 256-thread blocks); CLK_INVALID_VALUE
+// when no kernel exists for it.
+int32_t launch_xtier_class(hipStream_t st, int op, int ts, int td, int tier, const SepParams* ps,
+                           const BatchGrid g, uint32_t nblocks, std::string* err);
+
+// Launch one y/z tier class (a launch of nblocks 256-thread blocks);
 // CLK_INVALID_VALUE when no kernel exists for the pair (the planner only
 // forms classes that have one: ydyn_pair, and float sources for the
 // Gaussian and the mean).
 int32_t launch_ydyn_class(hipStream_t st, int op, int ts, int td, int tier, int cols, const SepParams* ps,
-                          const BatchTile* tiles, uint32_t ntiles, std::string* err);
+                          const BatchGrid g, uint32_t nblocks, std::string* err);
 
 }  // namespace clk

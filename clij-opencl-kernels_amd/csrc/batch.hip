@@ -5,9 +5,10 @@
 // and pass order as the single calls), then runs "stage" 0, 1, 2 -- the
 // first, second and third active pass of every job.  Within a stage the
 // passes are grouped by kernel class (kernel kind, op, pixel types, column
-// width); each class is ONE launch whose blocks come from a device tile
-// table (job, virtual block index), and each block runs the same kernel
-// body as the single launch with that job's parameters.  So a batch of 256
+// width); each class is ONE launch whose blocks are numbered job after job
+// (a device table holds each job's first block; a block finds its job by
+// binary search), and each block runs the same kernel body as the single
+// launch with that job's parameters.  So a batch of 256
 // uneven volumes costs a few launches per stage instead of ~768, and the
 // results equal the single calls bit for bit.
 #include <hip/hip_runtime.h>
@@ -35,31 +36,31 @@ namespace clk {
 
 template <int OP, class TS, class TD>
 __global__ __launch_bounds__(256) void batch_x_kernel(const SepParams* __restrict__ ps,
-                                                      const BatchTile* __restrict__ tiles) {
+                                                      const BatchGrid grid) {
   extern __shared__ uint32_t lds[];
-  const BatchTile t = tiles[blockIdx.x];
+  const BatchTile t = batch_tile(grid);
   sep_x_body<OP, TS, TD, 1>(ps[t.job], t.vblk, lds);
 }
 
 template <int OP, class TS, class TD>
 __global__ __launch_bounds__(256) void batch_xpack_kernel(const SepParams* __restrict__ ps,
-                                                          const BatchTile* __restrict__ tiles) {
+                                                          const BatchGrid grid) {
   extern __shared__ uint32_t lds[];
-  const BatchTile t = tiles[blockIdx.x];
+  const BatchTile t = batch_tile(grid);
   sep_xrows_body<OP, TS, TD>(ps[t.job], t.vblk, lds);
 }
 
 template <int OP, class TS, class TD, int COLS>
 __global__ __launch_bounds__(256) void batch_strided_kernel(const SepParams* __restrict__ ps,
-                                                            const BatchTile* __restrict__ tiles) {
-  const BatchTile t = tiles[blockIdx.x];
+                                                            const BatchGrid grid) {
+  const BatchTile t = batch_tile(grid);
   sep_strided_body<OP, TS, TD, 8, COLS>(ps[t.job], t.vblk);
 }
 
 template <class TS, class TD, int KIND>
 __global__ __launch_bounds__(256) void batch_median_kernel(const MedParams* __restrict__ ms,
-                                                           const BatchTile* __restrict__ tiles) {
-  const BatchTile t = tiles[blockIdx.x];
+                                                           const BatchGrid grid) {
+  const BatchTile t = batch_tile(grid);
   const MedParams& p = ms[t.job];
   const uint32_t bx = t.vblk % p.gx;
   const uint32_t r = t.vblk / p.gx;
@@ -74,8 +75,8 @@ __global__ __launch_bounds__(256) void batch_median_kernel(const MedParams* __re
 
 template <class TS, class TD>
 __global__ __launch_bounds__(64) void batch_hist8_kernel(const MedParams* __restrict__ ms,
-                                                         const BatchTile* __restrict__ tiles) {
-  const BatchTile t = tiles[blockIdx.x];
+                                                         const BatchGrid grid) {
+  const BatchTile t = batch_tile(grid);
   const MedParams& p = ms[t.job];
   const uint32_t bx = t.vblk % p.gx;
   const uint32_t r = t.vblk / p.gx;
@@ -84,14 +85,14 @@ __global__ __launch_bounds__(64) void batch_hist8_kernel(const MedParams* __rest
 
 namespace {
 
-enum ClassKind : int { K_X = 0, K_STRIDED4 = 1, K_STRIDED1 = 2, K_MEDIAN = 3, K_XPACK = 4, K_YDYN = 5 };
+enum ClassKind : int { K_X = 0, K_STRIDED4 = 1, K_STRIDED1 = 2, K_MEDIAN = 3, K_XPACK = 4, K_YDYN = 5, K_XTIER = 6 };
 
 // Pixel-type pairs with batched kernels: same type, to float, from float.
 bool batch_pair(int ts, int td) { return ts == td || td == CLK_FLOAT || ts == CLK_FLOAT; }
 
 struct ClassKey {
   int kind, op, ts, td;  // op: separable op, or MedKind for medians
-  int tier = 0, cols = 0;  // K_YDYN: half-width tier and columns per lane
+  int tier = 0, cols = 0;  // K_YDYN, K_XTIER: half-width tier; K_YDYN: columns per lane
   bool operator<(const ClassKey& o) const {
     return std::tie(kind, op, ts, td, tier, cols) < std::tie(o.kind, o.op, o.ts, o.td, o.tier, o.cols);
   }
@@ -100,10 +101,18 @@ struct ClassKey {
 struct ClassWork {
   std::vector<SepParams> sep;
   std::vector<MedParams> med;
-  std::vector<BatchTile> tiles;
+  std::vector<uint32_t> off{0};  // first block of each job, then the grid size
+  uint64_t nblocks = 0;
   size_t lds = 0;
   // device offsets inside the table area
-  size_t params_off = 0, tiles_off = 0;
+  size_t params_off = 0, off_off = 0;
+  void add_job(uint64_t blocks) {
+    nblocks += blocks;
+    off.push_back((uint32_t)(nblocks < 0xffffffffULL ? nblocks : 0xffffffffULL));
+  }
+  BatchGrid grid(char* dev) const {
+    return {reinterpret_cast<const uint32_t*>(dev + off_off), (uint32_t)(off.size() - 1)};
+  }
 };
 
 // Pinned staging buffers for the launch tables, reused round robin; a slot
@@ -127,37 +136,37 @@ int32_t sigma_n(float sigma) {
 
 template <int OP, class TS, class TD>
 void launch_class_typed(hipStream_t st, const ClassKey& k, const ClassWork& w, char* dev) {
-  const uint32_t nt = (uint32_t)w.tiles.size();
-  const BatchTile* tiles = reinterpret_cast<const BatchTile*>(dev + w.tiles_off);
+  const uint32_t nt = (uint32_t)w.nblocks;
+  const BatchGrid grid = w.grid(dev);
   const SepParams* ps = reinterpret_cast<const SepParams*>(dev + w.params_off);
   if (k.kind == K_X)
-    hipLaunchKernelGGL((batch_x_kernel<OP, TS, TD>), dim3(nt), dim3(256), w.lds, st, ps, tiles);
+    hipLaunchKernelGGL((batch_x_kernel<OP, TS, TD>), dim3(nt), dim3(256), w.lds, st, ps, grid);
   else if (k.kind == K_XPACK)
-    hipLaunchKernelGGL((batch_xpack_kernel<OP, TS, TD>), dim3(nt), dim3(256), w.lds, st, ps, tiles);
+    hipLaunchKernelGGL((batch_xpack_kernel<OP, TS, TD>), dim3(nt), dim3(256), w.lds, st, ps, grid);
   else if (k.kind == K_STRIDED4)
-    hipLaunchKernelGGL((batch_strided_kernel<OP, TS, TD, 4>), dim3(nt), dim3(256), 0, st, ps, tiles);
+    hipLaunchKernelGGL((batch_strided_kernel<OP, TS, TD, 4>), dim3(nt), dim3(256), 0, st, ps, grid);
   else
-    hipLaunchKernelGGL((batch_strided_kernel<OP, TS, TD, 1>), dim3(nt), dim3(256), 0, st, ps, tiles);
+    hipLaunchKernelGGL((batch_strided_kernel<OP, TS, TD, 1>), dim3(nt), dim3(256), 0, st, ps, grid);
 }
 
 template <class TS, class TD>
 void launch_median_class(hipStream_t st, const ClassKey& k, const ClassWork& w, char* dev) {
-  const uint32_t nt = (uint32_t)w.tiles.size();
-  const BatchTile* tiles = reinterpret_cast<const BatchTile*>(dev + w.tiles_off);
+  const uint32_t nt = (uint32_t)w.nblocks;
+  const BatchGrid grid = w.grid(dev);
   const MedParams* ms = reinterpret_cast<const MedParams*>(dev + w.params_off);
   switch (k.op) {
-    case MED_PK8: hipLaunchKernelGGL((batch_median_kernel<TS, TD, MED_PK8>), dim3(nt), dim3(256), 0, st, ms, tiles); break;
-    case MED_PK9: hipLaunchKernelGGL((batch_median_kernel<TS, TD, MED_PK9>), dim3(nt), dim3(256), 0, st, ms, tiles); break;
-    case MED_HIST8: hipLaunchKernelGGL((batch_hist8_kernel<TS, TD>), dim3(nt), dim3(64), 0, st, ms, tiles); break;
-    case MED_27: hipLaunchKernelGGL((batch_median_kernel<TS, TD, MED_27>), dim3(nt), dim3(256), 0, st, ms, tiles); break;
+    case MED_PK8: hipLaunchKernelGGL((batch_median_kernel<TS, TD, MED_PK8>), dim3(nt), dim3(256), 0, st, ms, grid); break;
+    case MED_PK9: hipLaunchKernelGGL((batch_median_kernel<TS, TD, MED_PK9>), dim3(nt), dim3(256), 0, st, ms, grid); break;
+    case MED_HIST8: hipLaunchKernelGGL((batch_hist8_kernel<TS, TD>), dim3(nt), dim3(64), 0, st, ms, grid); break;
+    case MED_27: hipLaunchKernelGGL((batch_median_kernel<TS, TD, MED_27>), dim3(nt), dim3(256), 0, st, ms, grid); break;
 #define CLK_MED_TIER(I)                                                                                     \
   case MED_REGS + I:                                                                                        \
     if constexpr (sizeof(TD) >= 2)                                                                          \
-      hipLaunchKernelGGL((batch_median_kernel<TS, TD, MED_REGS + I>), dim3(nt), dim3(256), 0, st, ms, tiles); \
+      hipLaunchKernelGGL((batch_median_kernel<TS, TD, MED_REGS + I>), dim3(nt), dim3(256), 0, st, ms, grid); \
     break;
     CLK_MED_TIER(0) CLK_MED_TIER(1) CLK_MED_TIER(2) CLK_MED_TIER(3) CLK_MED_TIER(4) CLK_MED_TIER(5)
 #undef CLK_MED_TIER
-    default: hipLaunchKernelGGL((batch_median_kernel<TS, TD, MED_GENERIC>), dim3(nt), dim3(256), 0, st, ms, tiles); break;
+    default: hipLaunchKernelGGL((batch_median_kernel<TS, TD, MED_GENERIC>), dim3(nt), dim3(256), 0, st, ms, grid); break;
   }
 }
 
@@ -197,10 +206,13 @@ void med_dst(hipStream_t st, const ClassKey& k, const ClassWork& w, char* dev) {
 }
 
 int32_t launch_class(hipStream_t st, const ClassKey& k, const ClassWork& w, char* dev, std::string* err) {
+  if (k.kind == K_XTIER)
+    return launch_xtier_class(st, k.op, k.ts, k.td, k.tier, reinterpret_cast<const SepParams*>(dev + w.params_off),
+                              w.grid(dev), (uint32_t)w.nblocks, err);
   if (k.kind == K_YDYN)
     return launch_ydyn_class(st, k.op, k.ts, k.td, k.tier, k.cols,
                              reinterpret_cast<const SepParams*>(dev + w.params_off),
-                             reinterpret_cast<const BatchTile*>(dev + w.tiles_off), (uint32_t)w.tiles.size(), err);
+                             w.grid(dev), (uint32_t)w.nblocks, err);
   if (k.kind == K_MEDIAN) {
     switch (k.ts) {
       case CLK_CHAR: med_dst<int8_t>(st, k, w, dev); break;
@@ -345,7 +357,23 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
       ClassKey key{0, ps.op, ps.src.dtype, ps.dst.dtype};
       uint32_t blocks = 0;
       size_t lds = 0;
-      if (axis_is_x && p.L <= 256 && p.c <= 256 && tuning(TUNE_NO_XPACK) == 0) {
+      const int64_t per_row = (p.L + 7) / 8;
+      if (axis_is_x && xtier_tier(p.c) > 0 && xtier_pair(ps.src.dtype, ps.dst.dtype) &&
+          p.outer * per_row < (int64_t(1) << 31) - 64 && tuning(TUNE_NO_XTIER) == 0) {
+        // register windows over per-row 8-pixel chunks, one class per tier
+        // whatever each job's row length and c
+        key.kind = K_XTIER;
+        key.tier = xtier_tier(p.c);
+        set_nseg(p, per_row);
+        p.ntasks = p.outer * per_row;
+        const int ss = clk_dtype_size(ps.src.dtype), ds = clk_dtype_size(ps.dst.dtype);
+        p.vec_ld = ((p.L * ss) % 16 == 0) && ((uintptr_t)p.src % 16 == 0);
+        p.vec_st = ((p.L * ds) % 16 == 0) && ((uintptr_t)p.dst % 16 == 0);
+        if (ps.op == OP_GAUSS) set_tier_weights(p, key.tier);
+        const int64_t units = (p.ntasks + 63) / 64;
+        const int64_t waves = (units + kXTierUnits - 1) / kXTierUnits;
+        blocks = (uint32_t)((waves + 3) / 4);
+      } else if (axis_is_x && p.L <= 256 && p.c <= 256 && tuning(TUNE_NO_XPACK) == 0) {
         key.kind = K_XPACK;  // short rows: several per wave, vector loads and stores of their span
         plan_xpack(p);
         blocks = (uint32_t)((p.ntasks + 3) / 4);
@@ -402,10 +430,9 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
         blocks = (uint32_t)(nblk * nseg);
       }
       ClassWork& w = classes[s][key];
-      const uint32_t jid = (uint32_t)w.sep.size();
       w.sep.push_back(p);
       if (lds > w.lds) w.lds = lds;
-      for (uint32_t b = 0; b < blocks; ++b) w.tiles.push_back({jid, b});
+      w.add_job(blocks);
     }
   }
   for (auto& mj : medians) {
@@ -443,10 +470,8 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
     }
     ClassKey key{K_MEDIAN, (int)mk, jb.src.dtype, jb.dst.dtype};
     ClassWork& w = classes[0][key];
-    const uint32_t jid = (uint32_t)w.med.size();
     w.med.push_back(p);
-    const uint64_t blocks = (uint64_t)grid.x * grid.y * grid.z;
-    for (uint64_t b = 0; b < blocks; ++b) w.tiles.push_back({jid, (uint32_t)b});
+    w.add_job((uint64_t)grid.x * grid.y * grid.z);
   }
 
   // ---- upload the launch tables -----------------------------------------------
@@ -456,8 +481,8 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
       ClassWork& w = kv.second;
       w.params_off = tbytes;
       tbytes += align256(w.sep.size() * sizeof(SepParams) + w.med.size() * sizeof(MedParams));
-      w.tiles_off = tbytes;
-      tbytes += align256(w.tiles.size() * sizeof(BatchTile));
+      w.off_off = tbytes;
+      tbytes += align256(w.off.size() * sizeof(uint32_t));
     }
   char* dev_tables = nullptr;
   if (tbytes > 0) {
@@ -489,7 +514,7 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
         const ClassWork& w = kv.second;
         if (!w.sep.empty()) memcpy(h + w.params_off, w.sep.data(), w.sep.size() * sizeof(SepParams));
         if (!w.med.empty()) memcpy(h + w.params_off, w.med.data(), w.med.size() * sizeof(MedParams));
-        memcpy(h + w.tiles_off, w.tiles.data(), w.tiles.size() * sizeof(BatchTile));
+        memcpy(h + w.off_off, w.off.data(), w.off.size() * sizeof(uint32_t));
       }
     if (hipMemcpyAsync(dev_tables, h, tbytes, hipMemcpyHostToDevice, st) != hipSuccess) {
       *err = "batch tables: upload failed";
@@ -506,8 +531,8 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
   }
   for (int s = 0; s < 3 && rc == 0; ++s) {
     for (auto& kv : classes[s]) {
-      if (kv.second.tiles.empty()) continue;
-      if (kv.second.tiles.size() > 0x7fffffffULL) {
+      if (kv.second.nblocks == 0) continue;
+      if (kv.second.nblocks > 0x7fffffffULL) {
         *err = "batch too large for one launch";
         rc = CLK_INVALID_VALUE;
         break;

@@ -1,4 +1,5 @@
-// Dispatch of a batch y/z tier class to the per-op compilation units.
+// Dispatch of the batch tier classes (x and y/z) to the per-op compilation
+// units.
 #include <string>
 
 #include "batch.h"
@@ -6,19 +7,39 @@
 
 namespace clk {
 
-bool launch_ydyn_gauss(hipStream_t, int, int, int, int, const SepParams*, const BatchTile*, uint32_t);
-bool launch_ydyn_mean(hipStream_t, int, int, int, int, const SepParams*, const BatchTile*, uint32_t);
-bool launch_ydyn_min(hipStream_t, int, int, int, int, const SepParams*, const BatchTile*, uint32_t);
-bool launch_ydyn_max(hipStream_t, int, int, int, int, const SepParams*, const BatchTile*, uint32_t);
+bool launch_ydyn_gauss(hipStream_t, int, int, int, int, const SepParams*, const BatchGrid, uint32_t);
+bool launch_ydyn_mean(hipStream_t, int, int, int, int, const SepParams*, const BatchGrid, uint32_t);
+bool launch_ydyn_min(hipStream_t, int, int, int, int, const SepParams*, const BatchGrid, uint32_t);
+bool launch_ydyn_max(hipStream_t, int, int, int, int, const SepParams*, const BatchGrid, uint32_t);
+bool launch_xtier_gauss(hipStream_t, int, int, int, const SepParams*, const BatchGrid, uint32_t);
+bool launch_xtier_mean(hipStream_t, int, int, int, const SepParams*, const BatchGrid, uint32_t);
+bool launch_xtier_min(hipStream_t, int, int, int, const SepParams*, const BatchGrid, uint32_t);
+bool launch_xtier_max(hipStream_t, int, int, int, const SepParams*, const BatchGrid, uint32_t);
 
-int32_t launch_ydyn_class(hipStream_t st, int op, int ts, int td, int tier, int cols, const SepParams* ps,
-                          const BatchTile* tiles, uint32_t ntiles, std::string* err) {
+int32_t launch_xtier_class(hipStream_t st, int op, int ts, int td, int tier, const SepParams* ps,
+                           const BatchGrid grid, uint32_t nblocks, std::string* err) {
   bool ok = false;
   switch (op) {
-    case OP_GAUSS: ok = launch_ydyn_gauss(st, ts, td, tier, cols, ps, tiles, ntiles); break;
-    case OP_MEAN: ok = launch_ydyn_mean(st, ts, td, tier, cols, ps, tiles, ntiles); break;
-    case OP_MIN: ok = launch_ydyn_min(st, ts, td, tier, cols, ps, tiles, ntiles); break;
-    case OP_MAX: ok = launch_ydyn_max(st, ts, td, tier, cols, ps, tiles, ntiles); break;
+    case OP_GAUSS: ok = launch_xtier_gauss(st, ts, td, tier, ps, grid, nblocks); break;
+    case OP_MEAN: ok = launch_xtier_mean(st, ts, td, tier, ps, grid, nblocks); break;
+    case OP_MIN: ok = launch_xtier_min(st, ts, td, tier, ps, grid, nblocks); break;
+    case OP_MAX: ok = launch_xtier_max(st, ts, td, tier, ps, grid, nblocks); break;
+  }
+  if (!ok) {
+    *err = "internal: no x tier class for this pass";
+    return CLK_INVALID_VALUE;
+  }
+  return 0;
+}
+
+int32_t launch_ydyn_class(hipStream_t st, int op, int ts, int td, int tier, int cols, const SepParams* ps,
+                          const BatchGrid grid, uint32_t nblocks, std::string* err) {
+  bool ok = false;
+  switch (op) {
+    case OP_GAUSS: ok = launch_ydyn_gauss(st, ts, td, tier, cols, ps, grid, nblocks); break;
+    case OP_MEAN: ok = launch_ydyn_mean(st, ts, td, tier, cols, ps, grid, nblocks); break;
+    case OP_MIN: ok = launch_ydyn_min(st, ts, td, tier, cols, ps, grid, nblocks); break;
+    case OP_MAX: ok = launch_ydyn_max(st, ts, td, tier, cols, ps, grid, nblocks); break;
   }
   if (!ok) {
     *err = "internal: no y/z tier class for this pass";

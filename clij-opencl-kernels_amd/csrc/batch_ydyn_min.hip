@@ -4,9 +4,9 @@
 
 namespace clk {
 
-bool launch_ydyn_min(hipStream_t st, int ts, int td, int tier, int cols, const SepParams* ps, const BatchTile* tiles,
+bool launch_ydyn_min(hipStream_t st, int ts, int td, int tier, int cols, const SepParams* ps, const BatchGrid grid,
                     uint32_t n) {
-  return launch_ydyn_op<OP_MIN>(st, ts, td, tier, cols, ps, tiles, n);
+  return launch_ydyn_op<OP_MIN>(st, ts, td, tier, cols, ps, grid, n);
 }
 
 }  // namespace clk

@@ -16,8 +16,8 @@ const char* kNames[TUNE_COUNT] = {"strided_rounds", "x_sub", "batch_seg",
                                   "no_xreg", "strided_variant", "no_hist8",
                                   "xreg_maxc", "ystatic_cols", "strided_order",
                                   "autotune", "xreg_units", "x_dynamic", "x_tasks", "sphere_direct",
-                                  "no_xpack", "yring"};
-const int32_t kDefaults[TUNE_COUNT] = {16, 0, 64, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+                                  "no_xpack", "yring", "no_xtier"};
+const int32_t kDefaults[TUNE_COUNT] = {16, 0, 64, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 std::atomic<int32_t> g_val[TUNE_COUNT];
 std::once_flag g_once;
 
@@ -86,6 +86,7 @@ extern "C" int32_t clk_set_tuning(const char* key, int32_t value) {
   if (k == clk::TUNE_X_TASKS && (value < 0 || value > 2)) return CLK_INVALID_VALUE;
   if (k == clk::TUNE_SPHERE_DIRECT && (value < 0 || value > 2)) return CLK_INVALID_VALUE;
   if (k == clk::TUNE_YRING && (value < 0 || value > 2)) return CLK_INVALID_VALUE;
+  if (k == clk::TUNE_NO_XTIER && value != 0 && value != 1) return CLK_INVALID_VALUE;
   if (k == clk::TUNE_XREG_UNITS && value != 0 && value != 1 && value != 2 && value != 4)
     return CLK_INVALID_VALUE;
   clk::g_val[k].store(value);

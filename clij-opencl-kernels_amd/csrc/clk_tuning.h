@@ -23,7 +23,8 @@ enum TuneKey : int {
   TUNE_SPHERE_DIRECT = 13,   // ellipsoid filters: 0 = auto, 1 = per-voxel gathers, 2 = LDS row tiles (no z sliding)
   TUNE_NO_XPACK = 14,        // 1 = short rows take one row per wave in the LDS x pass
   TUNE_YRING = 15,           // float y/z passes: 0 = lockstep ring on y passes, 1 = never, 2 = always
-  TUNE_COUNT = 16
+  TUNE_NO_XTIER = 16,        // 1 = batched x passes never take the register-window tier classes
+  TUNE_COUNT = 17
 };
 
 int32_t tuning(TuneKey k);

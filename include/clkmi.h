@@ -329,7 +329,9 @@ int32_t clk_temporaries_held(void* stream, int64_t* stream_bytes, int64_t* pool_
  * (y/z blocks per resident slot), "x_sub" (LDS x pass segments per wave),
  * "batch_seg", "no_xreg", "strided_variant", "no_hist8", "xreg_maxc",
  * "ystatic_cols", "strided_order", "xreg_units", "x_dynamic", "x_tasks",
- * "no_xpack", "sphere_direct" (ellipsoid filters: 0 = auto, 1 = per-voxel
+ * "no_xpack", "yring" (float y/z passes: 0 = lockstep ring on y passes,
+ * 1 = never, 2 = always), "no_xtier" (1 = batched x passes keep the LDS row
+ * classes), "sphere_direct" (ellipsoid filters: 0 = auto, 1 = per-voxel
  * gathers, 2 = LDS row tiles without z sliding), and
  * "autotune": 1 = the first single-axis launch of each signature (pixel
  * types, axis, N, volume size; volumes of 2^22 voxels and more) times a set

@@ -162,7 +162,7 @@ def test_batch_errors(gpu):
 def test_batch_yz_tier_classes(gpu):
     """The y/z tier classes (a run-time half width up to 5, 8 or 16, one
     launch per tier) give the generic strided classes' bits and the
-    oracle's: half widths 1..16 and beyond the last tier, odd, even and
+    oracle's: half widths 1..18 (beyond the last tier), odd, even and
     multiple-of-four inner extents (1, 2 or 4 columns per lane), several
     segments along an axis."""
     import clkmi
@@ -188,3 +188,70 @@ def test_batch_yz_tier_classes(gpu):
     for i, (job, g, h) in enumerate(zip(jobs, got, generic)):
         assert np.array_equal(g.view(np.uint8), h.view(np.uint8)), "job %d tiers != generic" % i
         assert same_bits_or_value(g, oracle_job(*job)), "job %d %s %s" % (i, job[0], job[1].shape)
+
+
+def _offset_device(a, offset):
+    """Upload a (z, y, x) volume `offset` bytes into a fresh buffer (rows
+    that do not start 16-byte aligned); returns (keepalive, Image)."""
+    import torch
+    import clkmi
+    from helpers import CODES
+    raw = np.ascontiguousarray(a).view(np.uint8).reshape(-1)
+    t = torch.zeros(raw.size + offset + 64, dtype=torch.uint8, device="cuda")
+    t[offset:offset + raw.size] = torch.from_numpy(raw.copy()).cuda()
+    d, h, w = a.shape
+    return t, clkmi.Image(t.data_ptr() + offset, CODES[a.dtype], w, h, d)
+
+
+def test_batch_x_tier_classes(gpu):
+    """The x tier classes (register windows over per-row 8-pixel chunks, a
+    run-time half width up to 4, 8 or 16) against the oracle and against the
+    LDS row classes they replace: row lengths 1..1000 (partial last chunks,
+    rows shorter than a chunk), half widths 0..17 (the last beyond the
+    tiers), x-only jobs (the pass writes the job's dst) and full chains,
+    aligned and unaligned rows."""
+    import clkmi
+    rng = np.random.default_rng(31)
+    widths = [1, 3, 7, 8, 9, 15, 16, 17, 31, 33, 64, 100, 255, 256, 257, 300, 513, 1000]
+    pairs = [(np.uint16, np.float32), (np.uint16, np.uint16), (np.uint8, np.uint8), (np.float32, np.float32),
+             (np.int16, np.float32), (np.int8, np.int8)]
+    jobs, offsets = [], []
+    for i, W in enumerate(widths * 2):
+        kind = ["gaussian_blur_3d", "minimum_box_3d", "maximum_box_3d", "mean_box_3d"][i % 4]
+        st, dt = pairs[i % len(pairs)]
+        shape = (int(rng.integers(1, 6)), int(rng.integers(1, 9)), W)
+        src = random_volume(shape, st, seed=3100 + i, lo=0 if np.dtype(st).kind == "u" else None)
+        c = i % 18
+        if kind == "gaussian_blur_3d":
+            s = {0: 0.1}.get(c, (2 * c + 0.5) / 8.0)  # int(8 s) = 2c: N = 2c + 1
+            sigma = (s, 0.0, 0.0) if i < len(widths) else (s, 1.0, 0.6)
+            radius = (0, 0, 0)
+        else:
+            sigma = (0.0, 0.0, 0.0)
+            radius = (max(c, 1), 0, 0) if i < len(widths) else (max(c, 1), 1, 2)
+        jobs.append((kind, src, dt, sigma, radius))
+        offsets.append(0 if i % 3 else (np.dtype(st).itemsize * 3))
+
+    def run(no_xtier):
+        keep, descs, outs = [], [], []
+        for (kind, src, dst_t, sigma, radius), off in zip(jobs, offsets):
+            ts, si = _offset_device(src, off)
+            td, di = empty_device(src.shape, dst_t)
+            keep += [ts, td]
+            outs.append((td, src.shape, dst_t))
+            descs.append(clkmi.FilterJob(kind, di, si, sigma, radius))
+        old = clkmi.get_tuning("no_xtier")
+        try:
+            clkmi.set_tuning("no_xtier", no_xtier)
+            clkmi.batch_filter_3d(descs)
+        finally:
+            clkmi.set_tuning("no_xtier", old)
+        return [from_device(*o) for o in outs]
+
+    print("x tier batch", len(jobs), flush=True)
+    got = run(0)
+    rows = run(1)
+    for i, (job, g, h) in enumerate(zip(jobs, got, rows)):
+        ctx = "job %d %s %s %s sigma=%s radius=%s" % (i, job[0], job[1].dtype, job[1].shape, job[3], job[4])
+        assert np.array_equal(g.view(np.uint8), h.view(np.uint8)), ctx + ": tiers != LDS rows"
+        assert same_bits_or_value(g, oracle_job(*job)), ctx + ": != oracle"
