@@ -38,6 +38,55 @@ __device__ __forceinline__ BatchTile batch_tile(const BatchGrid g) {
   return {lo, b - g.off[lo]};
 }
 
+// Fused x+y classes (batch_xy.h): the x pass computed in LDS on the way
+// into the y pass, so the x temporary never goes to memory.
+constexpr int kXYMaxC = 16;     // largest half width along x and y
+constexpr int kXYK = 4;         // y outputs per step
+constexpr int kXYSlots = 4;     // staged source pixels per thread and row (at most)
+
+// One fused job (device table).  Plain data, filled by the batch planner.
+struct XYParams {
+  const void* src;
+  void* dst;
+  int32_t W, H, D;
+  int32_t cx, cy;
+  int32_t ng;       // column groups per row: ceil(W / COLS)
+  int32_t ngt;      // D * ng
+  int32_t rstr;     // staged words per plane row (xy_rstr)
+  int32_t seg;      // y outputs per segment (multiple of kXYK)
+  int32_t nseg;     // segments along y
+  int32_t vec_st;   // W % COLS == 0 and dst aligned: COLS-wide stores
+  float divx, rdivx, divy, rdivy;
+  int32_t fastx, fasty;
+  float wx[2 * kXYMaxC + 1];  // x weights in tap order t = 0..2cx
+  float wy[2 * kXYMaxC + 1];  // y weights at CMAX - cy .. CMAX + cy (tier order)
+};
+
+// Staged words per plane row: the padded row rounded up to whole COLS-word
+// vectors, so every thread's window starts COLS-aligned.
+inline int32_t xy_rstr(int64_t ng, int cols, int cx) {
+  return (int32_t)((ng * cols + 2 * cx + cols - 1) / cols * cols);
+}
+
+// Staged words (of one row) a block of a job touches at most: the planes a
+// run of 256 groups can reach times the padded row, plus COLS - 1 words
+// that the last window's final vector may read past it (never used).
+inline int64_t xy_nslot(int64_t ng, int64_t D, int cols, int cx) {
+  int64_t np = 255 / ng + 2;
+  if (np > D) np = D;
+  return np * xy_rstr(ng, cols, cx) + cols;
+}
+
+// LDS bytes of a fused class launch: K staged rows of nslot words, K x
+// results of COLS words per thread.
+inline size_t xy_lds_bytes(int64_t nslot, int cols) { return (size_t)4 * (kXYK * nslot + kXYK * cols * 256); }
+
+// Launch one fused class (nblocks 256-thread blocks, lds bytes of dynamic
+// LDS, nslot staged words per row); CLK_INVALID_VALUE when no kernel exists
+// for it.
+int32_t launch_xy_class(hipStream_t st, int op, int ts, int tier, int cols, const XYParams* ps,
+                        const BatchGrid g, uint32_t nblocks, size_t lds, int nslot, std::string* err);
+
 // y/z passes with a half width known at run time (sep_ydyn.h): the tier
 // (largest half width compiled: 5, 8 or 16; 0 = none fits), the columns
 // per lane the tier allows, and the outputs per step.

@@ -15,6 +15,7 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <string>
@@ -85,14 +86,14 @@ __global__ __launch_bounds__(64) void batch_hist8_kernel(const MedParams* __rest
 
 namespace {
 
-enum ClassKind : int { K_X = 0, K_STRIDED4 = 1, K_STRIDED1 = 2, K_MEDIAN = 3, K_XPACK = 4, K_YDYN = 5, K_XTIER = 6 };
+enum ClassKind : int { K_X = 0, K_STRIDED4 = 1, K_STRIDED1 = 2, K_MEDIAN = 3, K_XPACK = 4, K_YDYN = 5, K_XTIER = 6, K_XY = 7 };
 
 // Pixel-type pairs with batched kernels: same type, to float, from float.
 bool batch_pair(int ts, int td) { return ts == td || td == CLK_FLOAT || ts == CLK_FLOAT; }
 
 struct ClassKey {
   int kind, op, ts, td;  // op: separable op, or MedKind for medians
-  int tier = 0, cols = 0;  // K_YDYN, K_XTIER: half-width tier; K_YDYN: columns per lane
+  int tier = 0, cols = 0;  // K_YDYN, K_XTIER, K_XY: half-width tier (K_XY: of y); K_YDYN, K_XY: columns per lane
   bool operator<(const ClassKey& o) const {
     return std::tie(kind, op, ts, td, tier, cols) < std::tie(o.kind, o.op, o.ts, o.td, o.tier, o.cols);
   }
@@ -101,9 +102,11 @@ struct ClassKey {
 struct ClassWork {
   std::vector<SepParams> sep;
   std::vector<MedParams> med;
+  std::vector<XYParams> xy;
   std::vector<uint32_t> off{0};  // first block of each job, then the grid size
   uint64_t nblocks = 0;
   size_t lds = 0;
+  int64_t nslot = 0;  // K_XY: staged words per row (largest job)
   // device offsets inside the table area
   size_t params_off = 0, off_off = 0;
   void add_job(uint64_t blocks) {
@@ -206,6 +209,9 @@ void med_dst(hipStream_t st, const ClassKey& k, const ClassWork& w, char* dev) {
 }
 
 int32_t launch_class(hipStream_t st, const ClassKey& k, const ClassWork& w, char* dev, std::string* err) {
+  if (k.kind == K_XY)
+    return launch_xy_class(st, k.op, k.ts, k.tier, k.cols, reinterpret_cast<const XYParams*>(dev + w.params_off),
+                           w.grid(dev), (uint32_t)w.nblocks, w.lds, (int)w.nslot, err);
   if (k.kind == K_XTIER)
     return launch_xtier_class(st, k.op, k.ts, k.td, k.tier, reinterpret_cast<const SepParams*>(dev + w.params_off),
                               w.grid(dev), (uint32_t)w.nblocks, err);
@@ -243,6 +249,40 @@ struct Pass {
   int64_t src_tmp, dst_tmp;
 };
 
+// The fused x+y pass of a job (batch_xy.h): src -> temporary of type tt at
+// byte offset dst_tmp, the y pass's output; the job's z pass reads it.
+struct XYPass {
+  int op;
+  clk_image dst, src;  // dst: the job's image (sizes)
+  int Nx, Ny;
+  float sx, sy;
+  int tt;
+  int64_t dst_tmp;
+  int cols;
+};
+
+// Columns per thread of a fused x+y pass for a job, or 0 when the job takes
+// the separate passes: half widths up to 16 (c = (N-1)/2 as the kernels
+// truncate), integer sources for min/max, 32-bit offsets, and a padded
+// staging row that fits the block's slots.  Wider columns cut LDS reads;
+// the y tier bounds them by registers as in the y/z tier classes.
+int xy_cols(int op, const clk_filter_job& jb, const int (&Ns)[3]) {
+  if (tuning(TUNE_NO_XY) != 0) return 0;
+  const int cx = (Ns[0] - 1) / 2, cy = (Ns[1] - 1) / 2;
+  if (Ns[0] < 1 || Ns[1] < 1 || cx > kXYMaxC || cy > kXYMaxC) return 0;
+  if ((op == OP_MIN || op == OP_MAX) && jb.src.dtype == CLK_FLOAT) return 0;
+  const int64_t W = jb.dst.width, H = jb.dst.height, D = jb.dst.depth;
+  if (W <= 0 || H <= 0 || D <= 0 || W * H * D >= (int64_t(1) << 31) - 1) return 0;
+  const int tier = ydyn_tier(cy);
+  const int cands[3] = {tier == 5 ? 4 : 2, tier == 5 ? 2 : 1, tier == 5 ? 1 : 0};
+  for (int cols : cands) {
+    if (cols == 0) break;
+    const int64_t ng = (W + cols - 1) / cols;
+    if (xy_nslot(ng, D, cols, cx) - cols <= (int64_t)kXYSlots * 256) return cols;
+  }
+  return 0;
+}
+
 // A pass that does not go through a batched class (rare pixel-type pairs,
 // degenerate parameters) runs as a single launch, in stage order.
 struct Single {
@@ -266,6 +306,7 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
   }
   // ---- plan: passes per stage, temporaries --------------------------------
   std::vector<std::vector<Pass>> stage_passes(3);
+  std::vector<XYPass> xy_passes;  // stage 0
   std::vector<std::pair<clk_filter_job, int>> medians;  // job, unused
   std::vector<std::pair<clk_image, clk_image>> copies;  // no active axis: converting copy
 
@@ -297,6 +338,22 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
     // whose values are exact in any type that holds the source
     const int tdt = (op == OP_MIN || op == OP_MAX) ? jb.src.dtype : CLK_FLOAT;
     const int64_t V = jb.dst.width * jb.dst.height * jb.dst.depth;
+    if (nact == 3) {
+      const int cols = xy_cols(op, jb, Ns);
+      if (cols > 0) {
+        // x and y in one pass into one temporary, then the z pass
+        const int64_t toff = temp_bytes;
+        temp_bytes += (int64_t)align256((size_t)(V * clk_dtype_size(tdt)));
+        const bool g = op == OP_GAUSS;
+        xy_passes.push_back({op, jb.dst, jb.src, Ns[0], Ns[1], g ? jb.sigma[0] : 0.f, g ? jb.sigma[1] : 0.f,
+                             tdt, toff, cols});
+        clk_image tmp = jb.dst;
+        tmp.dtype = tdt;
+        tmp.ptr = nullptr;
+        stage_passes[1].push_back({op, jb.dst, tmp, 2, Ns[2], g ? jb.sigma[2] : 0.f, toff, -1});
+        continue;
+      }
+    }
     const int ntmp = nact - 1 < 2 ? nact - 1 : 2;
     std::vector<int64_t> toff(ntmp);
     for (int i = 0; i < ntmp; ++i) {
@@ -399,7 +456,8 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
                         ((uintptr_t)p.dst % (2 * ds) == 0);
         key.kind = K_YDYN;
         key.tier = ydyn_tier(p.c);
-        key.cols = ydyn_cols(key.tier, v4, v2);
+        const int ycap = tuning(TUNE_BATCH_YCOLS);
+        key.cols = ydyn_cols(key.tier, v4 && (ycap == 0 || ycap >= 4), v2 && (ycap == 0 || ycap >= 2));
         const int K = ydyn_k(key.tier, key.cols);
         if (ps.op == OP_GAUSS) set_tier_weights(p, key.tier);
         p.ncolgroups = p.inner / key.cols;
@@ -434,6 +492,60 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
       if (lds > w.lds) w.lds = lds;
       w.add_job(blocks);
     }
+  }
+  const int xyseg = tuning(TUNE_BATCH_XYSEG) > 0 ? tuning(TUNE_BATCH_XYSEG) : 64;
+  for (const XYPass& xp : xy_passes) {
+    clk_image tmp = xp.dst;
+    tmp.dtype = xp.tt;
+    tmp.ptr = static_cast<char*>(temp_base) + xp.dst_tmp;
+    SepParams px, py;
+    int ax = 0;
+    bool nf = false;
+    int32_t rc = prepare_separable(xp.op, tmp, xp.src, 0, xp.Nx, xp.sx, px, &ax, &nf, err);
+    if (rc == 0 && !nf) rc = prepare_separable(xp.op, tmp, tmp, 1, xp.Ny, xp.sy, py, &ax, &nf, err);
+    if (rc) return rc;
+    if (nf) {
+      *err = "internal: fused x+y pass planned for an empty tap window";
+      return CLK_INVALID_VALUE;
+    }
+    const int cols = xp.cols;
+    const int tier = ydyn_tier(py.c);
+    XYParams q;
+    memset(&q, 0, sizeof(q));
+    q.src = xp.src.ptr;
+    q.dst = tmp.ptr;
+    q.W = (int32_t)xp.dst.width;
+    q.H = (int32_t)xp.dst.height;
+    q.D = (int32_t)xp.dst.depth;
+    q.cx = px.c;
+    q.cy = py.c;
+    q.ng = (int32_t)((q.W + cols - 1) / cols);
+    q.ngt = q.D * q.ng;
+    q.rstr = xy_rstr(q.ng, cols, q.cx);
+    int seg = (xyseg + kXYK - 1) / kXYK * kXYK;
+    const int hk = (q.H + kXYK - 1) / kXYK * kXYK;
+    if (seg > hk) seg = hk;
+    q.seg = seg;
+    q.nseg = (q.H + seg - 1) / seg;
+    const int ds = clk_dtype_size(xp.tt);
+    q.vec_st = (q.W % cols == 0) && ((uintptr_t)tmp.ptr % (size_t)(cols * ds) == 0);
+    q.divx = px.div;
+    q.rdivx = px.rdiv;
+    q.fastx = px.fastdiv;
+    q.divy = py.div;
+    q.rdivy = py.rdiv;
+    q.fasty = py.fastdiv;
+    if (xp.op == OP_GAUSS) {
+      for (int t = 0; t <= 2 * px.c; ++t) q.wx[t] = px.w[t];
+      for (int t = 0; t <= 2 * py.c; ++t) q.wy[t + tier - py.c] = py.w[t];
+    }
+    ClassKey key{K_XY, xp.op, xp.src.dtype, xp.tt, tier, cols};
+    ClassWork& w = classes[0][key];
+    w.xy.push_back(q);
+    const int64_t nsl = (xy_nslot(q.ng, q.D, cols, q.cx) + 3) / 4 * 4;
+    if (nsl > w.nslot) w.nslot = nsl;
+    w.lds = xy_lds_bytes(w.nslot, cols);
+    w.add_job((uint64_t)((q.ngt + 255) / 256) * (uint64_t)q.nseg);
   }
   for (auto& mj : medians) {
     const clk_filter_job& jb = mj.first;
@@ -480,7 +592,8 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
     for (auto& kv : classes[s]) {
       ClassWork& w = kv.second;
       w.params_off = tbytes;
-      tbytes += align256(w.sep.size() * sizeof(SepParams) + w.med.size() * sizeof(MedParams));
+      tbytes += align256(w.sep.size() * sizeof(SepParams) + w.med.size() * sizeof(MedParams) +
+                         w.xy.size() * sizeof(XYParams));
       w.off_off = tbytes;
       tbytes += align256(w.off.size() * sizeof(uint32_t));
     }
@@ -514,6 +627,7 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
         const ClassWork& w = kv.second;
         if (!w.sep.empty()) memcpy(h + w.params_off, w.sep.data(), w.sep.size() * sizeof(SepParams));
         if (!w.med.empty()) memcpy(h + w.params_off, w.med.data(), w.med.size() * sizeof(MedParams));
+        if (!w.xy.empty()) memcpy(h + w.params_off, w.xy.data(), w.xy.size() * sizeof(XYParams));
         memcpy(h + w.off_off, w.off.data(), w.off.size() * sizeof(uint32_t));
       }
     if (hipMemcpyAsync(dev_tables, h, tbytes, hipMemcpyHostToDevice, st) != hipSuccess) {
@@ -529,15 +643,50 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
     rc = launch_convert_copy(st, c.first, c.second, err);
     if (rc) break;
   }
+  // The classes of one stage touch disjoint jobs, so they run side by side
+  // on the stream's side streams (forked from st after the table upload and
+  // the previous stage, joined back into st): a class launch of a few
+  // thousand blocks no longer leaves the GPU idle during its ramp and tail.
+  // The largest classes go first, one per side stream in turn.
+  hipStream_t* aux = nullptr;
+  hipEvent_t* joins = nullptr;
+  hipEvent_t fork = nullptr;
+  const int naux = tuning(TUNE_BATCH_SERIAL) ? 0 : scratch.aux(&aux, &joins, &fork);
   for (int s = 0; s < 3 && rc == 0; ++s) {
-    for (auto& kv : classes[s]) {
-      if (kv.second.nblocks == 0) continue;
-      if (kv.second.nblocks > 0x7fffffffULL) {
+    std::vector<std::pair<const ClassKey*, const ClassWork*>> order;
+    for (auto& kv : classes[s])
+      if (kv.second.nblocks > 0) order.push_back({&kv.first, &kv.second});
+    std::stable_sort(order.begin(), order.end(), [](const auto& a, const auto& b) {
+      return a.second->nblocks > b.second->nblocks;
+    });
+    const bool side = naux > 0 && order.size() > 1;
+    int used = 0;
+    if (side && hipEventRecord(fork, st) != hipSuccess) {
+      *err = "batch: fork event record failed";
+      rc = CLK_OUT_OF_RESOURCES;
+      break;
+    }
+    for (size_t i = 0; i < order.size() && rc == 0; ++i) {
+      const ClassWork& w = *order[i].second;
+      if (w.nblocks > 0x7fffffffULL) {
         *err = "batch too large for one launch";
         rc = CLK_INVALID_VALUE;
         break;
       }
-      rc = launch_class(st, kv.first, kv.second, dev_tables, err);
+      hipStream_t q = st;
+      if (side) {
+        const int a = (int)(i % (size_t)naux);
+        q = aux[a];
+        if (a >= used) {
+          if (hipStreamWaitEvent(q, fork, 0) != hipSuccess) {
+            *err = "batch: side stream wait failed";
+            rc = CLK_OUT_OF_RESOURCES;
+            break;
+          }
+          used = a + 1;
+        }
+      }
+      rc = launch_class(q, *order[i].first, w, dev_tables, err);
       if (rc) break;
       const hipError_t e = hipGetLastError();
       if (e != hipSuccess) {
@@ -546,6 +695,15 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
         break;
       }
     }
+    // join even after a failed launch, so st never runs ahead of side work
+    for (int a = 0; a < used; ++a)
+      if (hipEventRecord(joins[a], aux[a]) != hipSuccess || hipStreamWaitEvent(st, joins[a], 0) != hipSuccess) {
+        (void)hipStreamSynchronize(aux[a]);
+        if (rc == 0) {
+          *err = "batch: side stream join failed";
+          rc = CLK_OUT_OF_RESOURCES;
+        }
+      }
     for (const Single& sg : singles) {
       if (sg.stage != s || rc) continue;
       rc = launch_separable(sg.pass.op, st, sg.pass.dst, sg.pass.src, sg.pass.dim, sg.pass.N, sg.pass.s, err);

@@ -1,4 +1,4 @@
-// Dispatch of the batch tier classes (x and y/z) to the per-op compilation
+// Dispatch of the batch tier classes (x, y/z and fused x+y) to the per-op compilation
 // units.
 #include <string>
 
@@ -15,6 +15,10 @@ bool launch_xtier_gauss(hipStream_t, int, int, int, const SepParams*, const Batc
 bool launch_xtier_mean(hipStream_t, int, int, int, const SepParams*, const BatchGrid, uint32_t);
 bool launch_xtier_min(hipStream_t, int, int, int, const SepParams*, const BatchGrid, uint32_t);
 bool launch_xtier_max(hipStream_t, int, int, int, const SepParams*, const BatchGrid, uint32_t);
+bool launch_xy_gauss(hipStream_t, int, int, int, const XYParams*, const BatchGrid, uint32_t, size_t, int);
+bool launch_xy_mean(hipStream_t, int, int, int, const XYParams*, const BatchGrid, uint32_t, size_t, int);
+bool launch_xy_min(hipStream_t, int, int, int, const XYParams*, const BatchGrid, uint32_t, size_t, int);
+bool launch_xy_max(hipStream_t, int, int, int, const XYParams*, const BatchGrid, uint32_t, size_t, int);
 
 int32_t launch_xtier_class(hipStream_t st, int op, int ts, int td, int tier, const SepParams* ps,
                            const BatchGrid grid, uint32_t nblocks, std::string* err) {
@@ -43,6 +47,22 @@ int32_t launch_ydyn_class(hipStream_t st, int op, int ts, int td, int tier, int 
   }
   if (!ok) {
     *err = "internal: no y/z tier class for this pass";
+    return CLK_INVALID_VALUE;
+  }
+  return 0;
+}
+
+int32_t launch_xy_class(hipStream_t st, int op, int ts, int tier, int cols, const XYParams* ps,
+                        const BatchGrid grid, uint32_t nblocks, size_t lds, int nslot, std::string* err) {
+  bool ok = false;
+  switch (op) {
+    case OP_GAUSS: ok = launch_xy_gauss(st, ts, tier, cols, ps, grid, nblocks, lds, nslot); break;
+    case OP_MEAN: ok = launch_xy_mean(st, ts, tier, cols, ps, grid, nblocks, lds, nslot); break;
+    case OP_MIN: ok = launch_xy_min(st, ts, tier, cols, ps, grid, nblocks, lds, nslot); break;
+    case OP_MAX: ok = launch_xy_max(st, ts, tier, cols, ps, grid, nblocks, lds, nslot); break;
+  }
+  if (!ok) {
+    *err = "internal: no fused x+y class for this job";
     return CLK_INVALID_VALUE;
   }
   return 0;

@@ -158,6 +158,52 @@ __device__ __forceinline__ void load_nt_words(const void* __restrict__ p, uint32
   }
 }
 
+// The same streaming loads and stores at any byte alignment: the GPU runs
+// in unaligned access mode (the ROCm default for gfx9 targets, which the
+// compiler also assumes for amdhsa), so one vector instruction moves a
+// chunk of pixels wherever a row starts.  Used where rows of arbitrary
+// length make chunk addresses unaligned.
+typedef u32x4 u32x4_ua __attribute__((aligned(1)));
+typedef u32x2 u32x2_ua __attribute__((aligned(1)));
+
+template <int B>
+__device__ __forceinline__ void load_nt_words_ua(const void* __restrict__ p, uint32_t (&w)[B / 4]) {
+  static_assert(B == 8 || B % 16 == 0, "8 bytes or whole 16-byte vectors");
+  if constexpr (B == 8) {
+    const u32x2 t = __builtin_nontemporal_load(static_cast<const u32x2_ua*>(p));
+    w[0] = t.x; w[1] = t.y;
+  } else {
+#pragma unroll
+    for (int k = 0; k < B / 16; ++k) {
+      const u32x4 t = __builtin_nontemporal_load(static_cast<const u32x4_ua*>(p) + k);
+      w[4 * k] = t.x; w[4 * k + 1] = t.y; w[4 * k + 2] = t.z; w[4 * k + 3] = t.w;
+    }
+  }
+}
+
+// Streaming store of N pixels (N * sizeof(T) = 8 or a multiple of 16 bytes)
+// at any alignment.
+template <class T, int N>
+__device__ __forceinline__ void store_nt_ua(T* __restrict__ p, const T (&in)[N]) {
+  constexpr int B = N * (int)sizeof(T);
+  static_assert(B == 8 || B % 16 == 0, "8 bytes or whole 16-byte vectors");
+  PixVec<T, N> v;
+#pragma unroll
+  for (int i = 0; i < N; ++i) v.v[i] = in[i];
+  uint32_t w[B / 4];
+  __builtin_memcpy(w, &v, B);
+  if constexpr (B == 8) {
+    const u32x2 t = {w[0], w[1]};
+    __builtin_nontemporal_store(t, reinterpret_cast<u32x2_ua*>(p));
+  } else {
+#pragma unroll
+    for (int k = 0; k < B / 16; ++k) {
+      const u32x4 t = {w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]};
+      __builtin_nontemporal_store(t, reinterpret_cast<u32x4_ua*>(p) + k);
+    }
+  }
+}
+
 __device__ __forceinline__ int64_t clamp64(int64_t v, int64_t lo, int64_t hi) {
   return v < lo ? lo : (v > hi ? hi : v);
 }

@@ -116,6 +116,11 @@ class CallScratch {
   CallScratch& operator=(const CallScratch&) = delete;
   hipError_t get(int slot, size_t bytes, void** p);
   bool transient() const { return transient_; }
+  // Side streams of this (device, stream) for independent launches of one
+  // call (created on first use, kept with the blocks, default priority),
+  // with one event each and one for the fork: 0 when the call must stay on
+  // its own stream (transient mode, or creation failed).
+  int aux(hipStream_t** streams, hipEvent_t** joins, hipEvent_t* fork);
 
  private:
   hipStream_t st_;
@@ -125,6 +130,7 @@ class CallScratch {
   void* owned_[3] = {nullptr, nullptr, nullptr};  // transient mode
 };
 constexpr int kScratchSlots = 3;  // 0: host chain / median brick flags, 1: batch temporaries, 2: batch tables
+constexpr int kAuxStreams = 4;    // side streams per stream (one per hardware queue a process gets by default)
 
 // Kept blocks of one stream on the current device: bytes held, and release
 // (synchronises the stream, frees the blocks, trims the pool).

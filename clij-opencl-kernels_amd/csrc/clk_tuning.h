@@ -24,7 +24,11 @@ enum TuneKey : int {
   TUNE_NO_XPACK = 14,        // 1 = short rows take one row per wave in the LDS x pass
   TUNE_YRING = 15,           // float y/z passes: 0 = lockstep ring on y passes, 1 = never, 2 = always
   TUNE_NO_XTIER = 16,        // 1 = batched x passes never take the register-window tier classes
-  TUNE_COUNT = 17
+  TUNE_NO_XY = 17,           // 1 = batched 3D filters never fuse their x and y passes
+  TUNE_BATCH_XYSEG = 18,     // fused x+y batch pass: y outputs per block (0 = 64)
+  TUNE_BATCH_SERIAL = 19,    // 1 = batch classes of a stage run one after another on the caller's stream
+  TUNE_BATCH_YCOLS = 20,     // batch y/z tier classes: at most this many columns per lane (0 = tier's own)
+  TUNE_COUNT = 21
 };
 
 int32_t tuning(TuneKey k);

@@ -54,7 +54,24 @@ struct StreamBlocks {
   std::recursive_mutex mu;
   void* ptr[clk::kScratchSlots] = {nullptr, nullptr, nullptr};
   size_t bytes[clk::kScratchSlots] = {0, 0, 0};
+  // side streams (CallScratch::aux): created on first use
+  bool aux_tried = false, aux_ok = false;
+  hipStream_t aux[clk::kAuxStreams] = {};
+  hipEvent_t join[clk::kAuxStreams] = {};
+  hipEvent_t fork = nullptr;
 };
+
+void destroy_aux(StreamBlocks& b) {
+  for (int i = 0; i < clk::kAuxStreams; ++i) {
+    if (b.aux[i]) (void)hipStreamDestroy(b.aux[i]);
+    if (b.join[i]) (void)hipEventDestroy(b.join[i]);
+    b.aux[i] = nullptr;
+    b.join[i] = nullptr;
+  }
+  if (b.fork) (void)hipEventDestroy(b.fork);
+  b.fork = nullptr;
+  b.aux_tried = b.aux_ok = false;
+}
 std::mutex g_blocks_mu;  // guards the map only (never held across HIP calls)
 std::map<std::pair<int, hipStream_t>, std::unique_ptr<StreamBlocks>> g_blocks;
 
@@ -131,6 +148,30 @@ hipError_t CallScratch::get(int slot, size_t bytes, void** p) {
   return hipSuccess;
 }
 
+int CallScratch::aux(hipStream_t** streams, hipEvent_t** joins, hipEvent_t* fork) {
+  if (transient_ || !blocks_) return 0;
+  StreamBlocks& b = *static_cast<StreamBlocks*>(blocks_);
+  if (!b.aux_tried) {
+    b.aux_tried = true;
+    bool good = hipEventCreateWithFlags(&b.fork, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; good && i < kAuxStreams; ++i)
+      good = hipStreamCreateWithFlags(&b.aux[i], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&b.join[i], hipEventDisableTiming) == hipSuccess;
+    if (!good) {
+      (void)hipGetLastError();
+      destroy_aux(b);
+      b.aux_tried = true;
+      return 0;
+    }
+    b.aux_ok = true;
+  }
+  if (!b.aux_ok) return 0;
+  *streams = b.aux;
+  *joins = b.join;
+  *fork = b.fork;
+  return kAuxStreams;
+}
+
 int64_t kept_bytes(hipStream_t st) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
@@ -157,7 +198,12 @@ hipError_t release_kept(hipStream_t st) {
         b->bytes[s] = 0;
         any = true;
       }
+    if (b->aux_tried) {
+      // the side streams' work is joined into st; wait for it, then drop them
+      any = true;
+    }
     if (any) e = hipStreamSynchronize(st);
+    if (b->aux_tried) destroy_aux(*b);
   }
   hipMemPool_t pool = device_pool(dev);
   if (pool) (void)hipMemPoolTrimTo(pool, 0);  // gives back what no block holds

@@ -122,21 +122,33 @@ __device__ __forceinline__ TD clamp_int(int32_t v) {
 // box means of integer sources never need it, and Gaussians of integer
 // sources follow the flag (set from the smallest weight).
 template <int OP, class TS>
-__device__ __forceinline__ float div_planned(float a, const SepParams& p) {
-  if constexpr (!Pix<TS>::kInt) return div_uniform(a, p.div, p.rdiv);
-  else if constexpr (OP == OP_MEAN) return div_uniform_fast(a, p.div, p.rdiv);
-  else return p.fastdiv ? div_uniform_fast(a, p.div, p.rdiv) : div_uniform(a, p.div, p.rdiv);
+__device__ __forceinline__ float div_planned_dr(float a, float div, float rdiv, int32_t fastdiv) {
+  if constexpr (!Pix<TS>::kInt) return div_uniform(a, div, rdiv);
+  else if constexpr (OP == OP_MEAN) return div_uniform_fast(a, div, rdiv);
+  else return fastdiv ? div_uniform_fast(a, div, rdiv) : div_uniform(a, div, rdiv);
 }
 
+template <int OP, class TS>
+__device__ __forceinline__ float div_planned(float a, const SepParams& p) {
+  return div_planned_dr<OP, TS>(a, p.div, p.rdiv, p.fastdiv);
+}
+
+// The pass's output value: Gaussian/mean divided as planned, min/max
+// saturated, then CONVERT_dst.  (div, rdiv, fastdiv) as in SepParams.
 template <int OP, class TS, class TD, class VT>
-__device__ __forceinline__ TD finish(VT acc, const SepParams& p) {
+__device__ __forceinline__ TD finish_dr(VT acc, float div, float rdiv, int32_t fastdiv) {
   if constexpr (OP == OP_GAUSS || OP == OP_MEAN) {
-    return convert_sat<TD>(div_planned<OP, TS>(static_cast<float>(acc), p));
+    return convert_sat<TD>(div_planned_dr<OP, TS>(static_cast<float>(acc), div, rdiv, fastdiv));
   } else if constexpr (std::is_same<VT, int32_t>::value) {
     return clamp_int<TD>(acc);
   } else {
     return convert_sat<TD>(acc);
   }
+}
+
+template <int OP, class TS, class TD, class VT>
+__device__ __forceinline__ TD finish(VT acc, const SepParams& p) {
+  return finish_dr<OP, TS, TD, VT>(acc, p.div, p.rdiv, p.fastdiv);
 }
 
 // ---- y / z (strided) pass ---------------------------------------------------

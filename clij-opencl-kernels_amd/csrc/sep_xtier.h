@@ -17,8 +17,9 @@
 //     [CMAX - c, CMAX + c] are skipped by wave-uniform branches, in the
 //     reference's order (v = -c .. c), so the results are those of the
 //     single launches bit for bit.
-// Full chunks of rows that start 16-byte aligned load and store with one
-// vector access; other chunks one pixel at a time (clamped indices).
+// Whole chunks load and store with one vector access (an unaligned one
+// where rows do not start 16-byte aligned); a row's partial last chunk one
+// pixel at a time (clamped indices).
 #pragma once
 
 namespace clk {
@@ -26,8 +27,11 @@ namespace clk {
 // Chunk j of the row at rp (pixels 8j .. 8j+7, each clamped to the row).
 template <class TS>
 __device__ __forceinline__ void xt_load_chunk(const TS* __restrict__ rp, int x0, int W, bool vec, Chunk<TS>& c) {
-  if (vec && x0 + 8 <= W) {
-    load_nt_words<8 * (int)sizeof(TS)>(rp + x0, c.w);
+  if (x0 + 8 <= W) {
+    // a whole chunk: one vector load, aligned or not (inside the row, so
+    // inside the volume)
+    if (vec) load_nt_words<8 * (int)sizeof(TS)>(rp + x0, c.w);
+    else load_nt_words_ua<8 * (int)sizeof(TS)>(rp + x0, c.w);
     return;
   }
   TS v[8];
@@ -167,8 +171,9 @@ __device__ __forceinline__ void xt_compute(const SepParams& p, const XtUnit<TS, 
   const int W = (int)p.L;
   const int x0 = 8 * (int)j;
   TD* __restrict__ dp = static_cast<TD*>(p.dst) + (uint64_t)u.row * (uint64_t)W + x0;
-  if (p.vec_st && x0 + 8 <= W) {
-    store_nt<TD, 8>(dp, out);
+  if (x0 + 8 <= W) {
+    if (p.vec_st) store_nt<TD, 8>(dp, out);
+    else store_nt_ua<TD, 8>(dp, out);
   } else {
 #pragma unroll
     for (int q = 0; q < 8; ++q)

@@ -331,7 +331,13 @@ int32_t clk_temporaries_held(void* stream, int64_t* stream_bytes, int64_t* pool_
  * "ystatic_cols", "strided_order", "xreg_units", "x_dynamic", "x_tasks",
  * "no_xpack", "yring" (float y/z passes: 0 = lockstep ring on y passes,
  * 1 = never, 2 = always), "no_xtier" (1 = batched x passes keep the LDS row
- * classes), "sphere_direct" (ellipsoid filters: 0 = auto, 1 = per-voxel
+ * classes), "no_xy" (1 = batched 3D filters keep separate x and y passes),
+ * "batch_xyseg" (fused x+y batch pass: y outputs per block, 0 = 64),
+ * "batch_serial" (1 = the classes of a batch stage run one after another on
+ * the caller's stream instead of side by side on side streams),
+ * "batch_ycols" (batch y/z passes: at most 1, 2 or 4 columns per lane; 0 =
+ * as the half-width tier allows),
+ * "sphere_direct" (ellipsoid filters: 0 = auto, 1 = per-voxel
  * gathers, 2 = LDS row tiles without z sliding), and
  * "autotune": 1 = the first single-axis launch of each signature (pixel
  * types, axis, N, volume size; volumes of 2^22 voxels and more) times a set
