@@ -87,6 +87,13 @@ inline size_t xy_lds_bytes(int64_t nslot, int cols) { return (size_t)4 * (kXYK *
 int32_t launch_xy_class(hipStream_t st, int op, int ts, int tier, int cols, const XYParams* ps,
                         const BatchGrid g, uint32_t nblocks, size_t lds, int nslot, std::string* err);
 
+// Fused x+y minimum/maximum of 16-bit integer pixels in registers
+// (batch_xymm.h): half widths up to kXYMMMaxC along x and y; y tiers 5, 8.
+// ng = chunks of 8 pixels per row, ngt = D * ng.
+constexpr int kXYMMMaxC = 8;
+int32_t launch_xymm_class(hipStream_t st, int op, int ts, int tier, const XYParams* ps, const BatchGrid g,
+                          uint32_t nblocks, std::string* err);
+
 // y/z passes with a half width known at run time (sep_ydyn.h): the tier
 // (largest half width compiled: 5, 8 or 16; 0 = none fits), the columns
 // per lane the tier allows, and the outputs per step.

@@ -86,7 +86,8 @@ __global__ __launch_bounds__(64) void batch_hist8_kernel(const MedParams* __rest
 
 namespace {
 
-enum ClassKind : int { K_X = 0, K_STRIDED4 = 1, K_STRIDED1 = 2, K_MEDIAN = 3, K_XPACK = 4, K_YDYN = 5, K_XTIER = 6, K_XY = 7 };
+enum ClassKind : int { K_X = 0, K_STRIDED4 = 1, K_STRIDED1 = 2, K_MEDIAN = 3, K_XPACK = 4, K_YDYN = 5, K_XTIER = 6, K_XY = 7,
+                       K_XYMM = 8 };
 
 // Pixel-type pairs with batched kernels: same type, to float, from float.
 bool batch_pair(int ts, int td) { return ts == td || td == CLK_FLOAT || ts == CLK_FLOAT; }
@@ -209,6 +210,9 @@ void med_dst(hipStream_t st, const ClassKey& k, const ClassWork& w, char* dev) {
 }
 
 int32_t launch_class(hipStream_t st, const ClassKey& k, const ClassWork& w, char* dev, std::string* err) {
+  if (k.kind == K_XYMM)
+    return launch_xymm_class(st, k.op, k.ts, k.tier, reinterpret_cast<const XYParams*>(dev + w.params_off),
+                             w.grid(dev), (uint32_t)w.nblocks, err);
   if (k.kind == K_XY)
     return launch_xy_class(st, k.op, k.ts, k.tier, k.cols, reinterpret_cast<const XYParams*>(dev + w.params_off),
                            w.grid(dev), (uint32_t)w.nblocks, w.lds, (int)w.nslot, err);
@@ -258,8 +262,20 @@ struct XYPass {
   float sx, sy;
   int tt;
   int64_t dst_tmp;
-  int cols;
+  int cols;  // K_XY: columns per thread; 0: K_XYMM (registers, 8-pixel chunks)
 };
+
+// Minimum/maximum of 16-bit integer pixels with both half widths up to
+// kXYMMMaxC take the register-only fused class (batch_xymm.h).
+bool xymm_ok(int op, const clk_filter_job& jb, const int (&Ns)[3]) {
+  if (tuning(TUNE_NO_XYMM) != 0) return false;
+  if (op != OP_MIN && op != OP_MAX) return false;
+  if (jb.src.dtype != CLK_USHORT && jb.src.dtype != CLK_SHORT) return false;
+  const int cx = (Ns[0] - 1) / 2, cy = (Ns[1] - 1) / 2;
+  if (Ns[0] < 1 || Ns[1] < 1 || cx > kXYMMMaxC || cy > kXYMMMaxC) return false;
+  const int64_t W = jb.dst.width, H = jb.dst.height, D = jb.dst.depth;
+  return W > 0 && H > 0 && D > 0 && W * H * D < (int64_t(1) << 31) - 1;
+}
 
 // Columns per thread of a fused x+y pass for a job, or 0 when the job takes
 // the separate passes: half widths up to 16 (c = (N-1)/2 as the kernels
@@ -339,8 +355,9 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
     const int tdt = (op == OP_MIN || op == OP_MAX) ? jb.src.dtype : CLK_FLOAT;
     const int64_t V = jb.dst.width * jb.dst.height * jb.dst.depth;
     if (nact == 3) {
-      const int cols = xy_cols(op, jb, Ns);
-      if (cols > 0) {
+      const bool mm = xymm_ok(op, jb, Ns);
+      const int cols = mm ? 0 : xy_cols(op, jb, Ns);
+      if (mm || cols > 0) {
         // x and y in one pass into one temporary, then the z pass
         const int64_t toff = temp_bytes;
         temp_bytes += (int64_t)align256((size_t)(V * clk_dtype_size(tdt)));
@@ -509,7 +526,7 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
       return CLK_INVALID_VALUE;
     }
     const int cols = xp.cols;
-    const int tier = ydyn_tier(py.c);
+    const int tier = cols == 0 ? (py.c <= 5 ? 5 : 8) : ydyn_tier(py.c);
     XYParams q;
     memset(&q, 0, sizeof(q));
     q.src = xp.src.ptr;
@@ -519,16 +536,16 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
     q.D = (int32_t)xp.dst.depth;
     q.cx = px.c;
     q.cy = py.c;
-    q.ng = (int32_t)((q.W + cols - 1) / cols);
+    q.ng = (int32_t)((q.W + (cols ? cols : 8) - 1) / (cols ? cols : 8));
     q.ngt = q.D * q.ng;
-    q.rstr = xy_rstr(q.ng, cols, q.cx);
+    q.rstr = cols ? xy_rstr(q.ng, cols, q.cx) : 0;
     int seg = (xyseg + kXYK - 1) / kXYK * kXYK;
     const int hk = (q.H + kXYK - 1) / kXYK * kXYK;
     if (seg > hk) seg = hk;
     q.seg = seg;
     q.nseg = (q.H + seg - 1) / seg;
     const int ds = clk_dtype_size(xp.tt);
-    q.vec_st = (q.W % cols == 0) && ((uintptr_t)tmp.ptr % (size_t)(cols * ds) == 0);
+    q.vec_st = cols && (q.W % cols == 0) && ((uintptr_t)tmp.ptr % (size_t)(cols * ds) == 0);
     q.divx = px.div;
     q.rdivx = px.rdiv;
     q.fastx = px.fastdiv;
@@ -538,6 +555,14 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
     if (xp.op == OP_GAUSS) {
       for (int t = 0; t <= 2 * px.c; ++t) q.wx[t] = px.w[t];
       for (int t = 0; t <= 2 * py.c; ++t) q.wy[t + tier - py.c] = py.w[t];
+    }
+    if (cols == 0) {
+      ClassKey key{K_XYMM, xp.op, xp.src.dtype, xp.tt, tier, 0};
+      ClassWork& w = classes[0][key];
+      w.xy.push_back(q);
+      const uint64_t waves = (uint64_t)((q.ngt + 61) / 62) * (uint64_t)q.nseg;  // 62 chunks per wave
+      w.add_job((waves + 3) / 4);
+      continue;
     }
     ClassKey key{K_XY, xp.op, xp.src.dtype, xp.tt, tier, cols};
     ClassWork& w = classes[0][key];

@@ -336,7 +336,8 @@ int32_t clk_temporaries_held(void* stream, int64_t* stream_bytes, int64_t* pool_
  * "batch_serial" (1 = the classes of a batch stage run one after another on
  * the caller's stream instead of side by side on side streams),
  * "batch_ycols" (batch y/z passes: at most 1, 2 or 4 columns per lane; 0 =
- * as the half-width tier allows),
+ * as the half-width tier allows; default 1), "no_xymm" (1 = batched 16-bit
+ * minimum/maximum filters keep separate x and y passes),
  * "sphere_direct" (ellipsoid filters: 0 = auto, 1 = per-voxel
  * gathers, 2 = LDS row tiles without z sliding), and
  * "autotune": 1 = the first single-axis launch of each signature (pixel

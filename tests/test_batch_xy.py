@@ -1,5 +1,7 @@
 """Fused x+y passes of the batch engine: a 3D filter whose x and y half
-widths are at most 16 runs its x pass inside the y pass (no x temporary).
+widths are at most 16 runs its x pass inside the y pass (no x temporary);
+minimum/maximum of 16-bit integers with half widths up to 8 take the
+register-only form (8-pixel chunks per lane, packed 16-bit min/max).
 Every such job must equal the same batch with fusion off (separate x and y
 passes, already pinned against single calls) and the oracle chain, bit for
 bit, over row lengths that exercise every column width (1, 2, 4 columns per
@@ -81,8 +83,8 @@ def make_jobs(seed):
 @pytest.mark.parametrize("seed", [1, 2, 3])
 def test_xy_fused_equals_separate_and_oracle(gpu, seed):
     jobs = make_jobs(seed)
-    fused = run_batch(jobs, no_xy=0)
-    separate = run_batch(jobs, no_xy=1)
+    fused = run_batch(jobs, no_xy=0, no_xymm=0)
+    separate = run_batch(jobs, no_xy=1, no_xymm=1)
     for i, (job, f, s) in enumerate(zip(jobs, fused, separate)):
         tag = "job %d %s %s %s->%s sigma=%s r=%s" % (i, job[0], job[1].shape, job[1].dtype, np.dtype(job[2]),
                                                   job[3], job[4])
@@ -95,8 +97,8 @@ def test_xy_segment_knob(gpu, seg):
     """The y segment per block (4 = one step, 20 = not a power of two, 1000 =
     whole columns) changes no bit."""
     jobs = make_jobs(4)[:14]
-    want = run_batch(jobs, no_xy=1)
-    got = run_batch(jobs, no_xy=0, batch_xyseg=seg)
+    want = run_batch(jobs, no_xy=1, no_xymm=1)
+    got = run_batch(jobs, no_xy=0, no_xymm=0, batch_xyseg=seg)
     for i, (g, w) in enumerate(zip(got, want)):
         assert same_bits_or_value(g, w), "job %d seg %d" % (i, seg)
 
@@ -120,8 +122,36 @@ def test_xy_float_specials(gpu):
             sigma = (sigma_for_c(c), sigma_for_c(16 - c), 1.0) if kind == "gaussian_blur_3d" else (0.0,) * 3
             radius = (0, 0, 0) if kind == "gaussian_blur_3d" else (c, max(1, 16 - c), 1)
             jobs.append((kind, a, np.float32, sigma, radius))
-    fused = run_batch(jobs, no_xy=0)
-    separate = run_batch(jobs, no_xy=1)
+    fused = run_batch(jobs, no_xy=0, no_xymm=0)
+    separate = run_batch(jobs, no_xy=1, no_xymm=1)
     for i, (job, f, s) in enumerate(zip(jobs, fused, separate)):
         assert same_bits_or_value(f, s), "job %d %s" % (i, job[0])
         assert same_bits_or_value(f, oracle_job(*job)), "job %d %s oracle" % (i, job[0])
+
+
+@pytest.mark.parametrize("seed", [5, 6])
+def test_xymm_register_class(gpu, seed):
+    """Minimum/maximum of uint16/int16 with x and y radii 1..8 (the register
+    class) and 9 (just past it), on rows with partial last chunks, waves
+    spanning many planes, the type extremes (key flip of int16), heights
+    below one step and over several segments: equal to separate passes and
+    the oracle."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i, shape in enumerate(SHAPES + [(3, 90, 17), (2, 130, 9), (70, 4, 2)]):
+        st = (np.uint16, np.int16)[i % 2]
+        info = np.iinfo(st)
+        src = rng.integers(info.min, info.max, size=shape, endpoint=True).astype(st)
+        flat = src.reshape(-1)
+        flat[rng.choice(flat.size, max(1, flat.size // 50), replace=True)] = info.min
+        flat[rng.choice(flat.size, max(1, flat.size // 50), replace=True)] = info.max
+        dt = (st, np.float32, np.uint8)[i % 3]
+        kind = ("minimum_box_3d", "maximum_box_3d")[(i // 2) % 2]
+        radius = (int(rng.integers(1, 10)), int(rng.integers(1, 10)), int(rng.integers(1, 4)))
+        jobs.append((kind, src, dt, (0.0, 0.0, 0.0), radius))
+    fused = run_batch(jobs, no_xymm=0)
+    separate = run_batch(jobs, no_xymm=1, no_xy=1)
+    for i, (job, f, s_) in enumerate(zip(jobs, fused, separate)):
+        tag = "job %d %s %s %s->%s r=%s" % (i, job[0], job[1].shape, job[1].dtype, np.dtype(job[2]), job[4])
+        assert same_bits_or_value(f, s_), tag + " fused != separate"
+        assert same_bits_or_value(f, oracle_job(*job)), tag + " fused != oracle"
