@@ -23,7 +23,15 @@ struct BatchTile {
 struct BatchGrid {
   const uint32_t* off;
   uint32_t njobs;
+  uint32_t pstride;  // bytes between the jobs' parameter entries
 };
+
+// Job j's entry of a class's parameter table (entries pstride bytes apart;
+// an entry may end early where the class never reads further).
+template <class P>
+__device__ __forceinline__ const P& batch_param(const P* ps, const BatchGrid& g, uint32_t job) {
+  return *reinterpret_cast<const P*>(reinterpret_cast<const char*>(ps) + (size_t)job * g.pstride);
+}
 
 // Job and job-local block of this block: a binary search over off (the
 // same for the whole block, so it stays on the scalar unit).

@@ -12,6 +12,7 @@
 // uneven volumes costs a few launches per stage instead of ~768, and the
 // results equal the single calls bit for bit.
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 #include <string.h>
 
@@ -40,7 +41,7 @@ __global__ __launch_bounds__(256) void batch_x_kernel(const SepParams* __restric
                                                       const BatchGrid grid) {
   extern __shared__ uint32_t lds[];
   const BatchTile t = batch_tile(grid);
-  sep_x_body<OP, TS, TD, 1>(ps[t.job], t.vblk, lds);
+  sep_x_body<OP, TS, TD, 1>(batch_param(ps, grid, t.job), t.vblk, lds);
 }
 
 template <int OP, class TS, class TD>
@@ -48,21 +49,21 @@ __global__ __launch_bounds__(256) void batch_xpack_kernel(const SepParams* __res
                                                           const BatchGrid grid) {
   extern __shared__ uint32_t lds[];
   const BatchTile t = batch_tile(grid);
-  sep_xrows_body<OP, TS, TD>(ps[t.job], t.vblk, lds);
+  sep_xrows_body<OP, TS, TD>(batch_param(ps, grid, t.job), t.vblk, lds);
 }
 
 template <int OP, class TS, class TD, int COLS>
 __global__ __launch_bounds__(256) void batch_strided_kernel(const SepParams* __restrict__ ps,
                                                             const BatchGrid grid) {
   const BatchTile t = batch_tile(grid);
-  sep_strided_body<OP, TS, TD, 8, COLS>(ps[t.job], t.vblk);
+  sep_strided_body<OP, TS, TD, 8, COLS>(batch_param(ps, grid, t.job), t.vblk);
 }
 
 template <class TS, class TD, int KIND>
 __global__ __launch_bounds__(256) void batch_median_kernel(const MedParams* __restrict__ ms,
                                                            const BatchGrid grid) {
   const BatchTile t = batch_tile(grid);
-  const MedParams& p = ms[t.job];
+  const MedParams& p = batch_param(ms, grid, t.job);
   const uint32_t bx = t.vblk % p.gx;
   const uint32_t r = t.vblk / p.gx;
   const uint32_t by = r % p.gy, bz = r / p.gy;
@@ -78,7 +79,7 @@ template <class TS, class TD>
 __global__ __launch_bounds__(64) void batch_hist8_kernel(const MedParams* __restrict__ ms,
                                                          const BatchGrid grid) {
   const BatchTile t = batch_tile(grid);
-  const MedParams& p = ms[t.job];
+  const MedParams& p = batch_param(ms, grid, t.job);
   const uint32_t bx = t.vblk % p.gx;
   const uint32_t r = t.vblk / p.gx;
   median_hist8_body<TS, TD>(p, bx, r % p.gy, r / p.gy);
@@ -101,34 +102,57 @@ struct ClassKey {
 };
 
 struct ClassWork {
-  std::vector<SepParams> sep;
-  std::vector<MedParams> med;
-  std::vector<XYParams> xy;
+  std::vector<char> blob;        // parameter entries, `stride` bytes each
+  uint32_t stride = 0;
   std::vector<uint32_t> off{0};  // first block of each job, then the grid size
   uint64_t nblocks = 0;
   size_t lds = 0;
   int64_t nslot = 0;  // K_XY: staged words per row (largest job)
   // device offsets inside the table area
   size_t params_off = 0, off_off = 0;
+  // the first `bytes` of *p (every entry of a class takes the same)
+  void add_params(const void* p, size_t bytes) {
+    if (stride == 0) stride = (uint32_t)((bytes + 15) & ~size_t(15));
+    const size_t at = blob.size();
+    blob.resize(at + stride);
+    memcpy(blob.data() + at, p, bytes < stride ? bytes : stride);
+  }
   void add_job(uint64_t blocks) {
     nblocks += blocks;
     off.push_back((uint32_t)(nblocks < 0xffffffffULL ? nblocks : 0xffffffffULL));
   }
   BatchGrid grid(char* dev) const {
-    return {reinterpret_cast<const uint32_t*>(dev + off_off), (uint32_t)(off.size() - 1)};
+    return {reinterpret_cast<const uint32_t*>(dev + off_off), (uint32_t)(off.size() - 1), stride};
   }
 };
+
+// Bytes of a separable pass's table entry: the tier classes read weights
+// w[0 .. 2*tier] only, so their entries end there; other classes keep the
+// whole structure.
+size_t sep_entry_bytes(int kind, int tier) {
+  if (kind == K_XTIER || kind == K_YDYN) return offsetof(SepParams, w) + (size_t)(2 * tier + 1) * sizeof(float);
+  return sizeof(SepParams);
+}
+
+// Launch tables go from pinned host memory to the device by a copy kernel
+// (tables are whole 256-byte blocks).
+__global__ __launch_bounds__(256) void table_upload_kernel(const u32x4* __restrict__ src, u32x4* __restrict__ dst,
+                                                           const uint32_t n16) {
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n16; i += gridDim.x * 256u) dst[i] = src[i];
+}
 
 // Pinned staging buffers for the launch tables, reused round robin; a slot
 // is reused only after its previous upload completed.
 struct Staging {
   void* host = nullptr;
+  void* dev = nullptr;  // the same memory as the device reads it
   size_t cap = 0;
   hipEvent_t ev = nullptr;
   bool pending = false;
 };
+constexpr int kStagingSlots = 8;  // up to three uploads per batch call
 std::mutex g_stage_mu;
-Staging g_stage[4];
+Staging g_stage[kStagingSlots];
 int g_stage_next = 0;
 
 size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
@@ -398,9 +422,9 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
     }
   }
 
-  // ---- build classes ------------------------------------------------------
+  // ---- temporaries, table area ---------------------------------------------
   const int bseg = tuning(TUNE_BATCH_SEG) > 0 ? tuning(TUNE_BATCH_SEG) : 64;
-  std::vector<std::map<ClassKey, ClassWork>> classes(3);
+  const int xyseg = tuning(TUNE_BATCH_XYSEG) > 0 ? tuning(TUNE_BATCH_XYSEG) : 64;
   std::vector<Single> singles;
   void* temp_base = nullptr;
   CallScratch scratch(st);  // held until the last launch is issued
@@ -413,15 +437,29 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
       if (ps.dst_tmp >= 0) ps.dst.ptr = static_cast<char*>(temp_base) + ps.dst_tmp;
       if (ps.src_tmp >= 0) ps.src.ptr = static_cast<char*>(temp_base) + ps.src_tmp;
     }
-  for (int s = 0; s < 3; ++s) {
+  // Each stage's tables go up in their own region of one device block, so
+  // the host plans stage s + 1 while stage s runs; the block's size bounds
+  // every stage's classes (an entry and a block offset per job, two 256-byte
+  // alignments per class, at most one class per job).
+  size_t tcap = 0;
+  for (int s = 0; s < 3; ++s) tcap += stage_passes[s].size() * (sizeof(SepParams) + 4 + 2 * 256 + 4);
+  tcap += xy_passes.size() * (sizeof(XYParams) + 4 + 2 * 256 + 4);
+  tcap += medians.size() * (sizeof(MedParams) + 4 + 2 * 256 + 4);
+  char* dev_tables = nullptr;
+  if (tcap > 0 && scratch.get(2, tcap, (void**)&dev_tables) != hipSuccess) {
+    *err = "batch tables: allocation failed";
+    return CLK_MEM_OBJECT_ALLOCATION_FAILURE;
+  }
+  size_t tused = 0;
+
+  // ---- classes of one stage ---------------------------------------------------
+  auto classify_stage = [&](const int s, std::map<ClassKey, ClassWork>& classes) -> int32_t {
     for (const Pass& ps : stage_passes[s]) {
       SepParams p;
       int axis_is_x = 0;
       bool nan_fill = false;
       int32_t rc = prepare_separable(ps.op, ps.dst, ps.src, ps.dim, ps.N, ps.s, p, &axis_is_x, &nan_fill, err);
-      if (rc) {
-        return rc;
-      }
+      if (rc) return rc;
       const int64_t V = ps.dst.width * ps.dst.height * ps.dst.depth;
       if (V == 0) continue;
       if (nan_fill || !batch_pair(ps.src.dtype, ps.dst.dtype) || p.L >= (int64_t(1) << 30)) {
@@ -504,180 +542,212 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
         p.nblk = (uint32_t)nblk;
         blocks = (uint32_t)(nblk * nseg);
       }
-      ClassWork& w = classes[s][key];
-      w.sep.push_back(p);
+      ClassWork& w = classes[key];
+      w.add_params(&p, sep_entry_bytes(key.kind, key.tier));
       if (lds > w.lds) w.lds = lds;
       w.add_job(blocks);
     }
-  }
-  const int xyseg = tuning(TUNE_BATCH_XYSEG) > 0 ? tuning(TUNE_BATCH_XYSEG) : 64;
-  for (const XYPass& xp : xy_passes) {
-    clk_image tmp = xp.dst;
-    tmp.dtype = xp.tt;
-    tmp.ptr = static_cast<char*>(temp_base) + xp.dst_tmp;
-    SepParams px, py;
-    int ax = 0;
-    bool nf = false;
-    int32_t rc = prepare_separable(xp.op, tmp, xp.src, 0, xp.Nx, xp.sx, px, &ax, &nf, err);
-    if (rc == 0 && !nf) rc = prepare_separable(xp.op, tmp, tmp, 1, xp.Ny, xp.sy, py, &ax, &nf, err);
-    if (rc) return rc;
-    if (nf) {
-      *err = "internal: fused x+y pass planned for an empty tap window";
-      return CLK_INVALID_VALUE;
+    if (s != 0) return 0;
+    for (const XYPass& xp : xy_passes) {
+      clk_image tmp = xp.dst;
+      tmp.dtype = xp.tt;
+      tmp.ptr = static_cast<char*>(temp_base) + xp.dst_tmp;
+      SepParams px, py;
+      int ax = 0;
+      bool nf = false;
+      int32_t rc = prepare_separable(xp.op, tmp, xp.src, 0, xp.Nx, xp.sx, px, &ax, &nf, err);
+      if (rc == 0 && !nf) rc = prepare_separable(xp.op, tmp, tmp, 1, xp.Ny, xp.sy, py, &ax, &nf, err);
+      if (rc) return rc;
+      if (nf) {
+        *err = "internal: fused x+y pass planned for an empty tap window";
+        return CLK_INVALID_VALUE;
+      }
+      const int cols = xp.cols;
+      const int tier = cols == 0 ? (py.c <= 5 ? 5 : 8) : ydyn_tier(py.c);
+      XYParams q;
+      memset(&q, 0, sizeof(q));
+      q.src = xp.src.ptr;
+      q.dst = tmp.ptr;
+      q.W = (int32_t)xp.dst.width;
+      q.H = (int32_t)xp.dst.height;
+      q.D = (int32_t)xp.dst.depth;
+      q.cx = px.c;
+      q.cy = py.c;
+      q.ng = (int32_t)((q.W + (cols ? cols : 8) - 1) / (cols ? cols : 8));
+      q.ngt = q.D * q.ng;
+      q.rstr = cols ? xy_rstr(q.ng, cols, q.cx) : 0;
+      int seg = (xyseg + kXYK - 1) / kXYK * kXYK;
+      const int hk = (q.H + kXYK - 1) / kXYK * kXYK;
+      if (seg > hk) seg = hk;
+      q.seg = seg;
+      q.nseg = (q.H + seg - 1) / seg;
+      const int ds = clk_dtype_size(xp.tt);
+      q.vec_st = cols && (q.W % cols == 0) && ((uintptr_t)tmp.ptr % (size_t)(cols * ds) == 0);
+      q.divx = px.div;
+      q.rdivx = px.rdiv;
+      q.fastx = px.fastdiv;
+      q.divy = py.div;
+      q.rdivy = py.rdiv;
+      q.fasty = py.fastdiv;
+      if (xp.op == OP_GAUSS) {
+        for (int t = 0; t <= 2 * px.c; ++t) q.wx[t] = px.w[t];
+        for (int t = 0; t <= 2 * py.c; ++t) q.wy[t + tier - py.c] = py.w[t];
+      }
+      if (cols == 0) {
+        ClassWork& w = classes[ClassKey{K_XYMM, xp.op, xp.src.dtype, xp.tt, tier, 0}];
+        w.add_params(&q, sizeof(XYParams));
+        const uint64_t waves = (uint64_t)((q.ngt + 61) / 62) * (uint64_t)q.nseg;  // 62 chunks per wave
+        w.add_job((waves + 3) / 4);
+        continue;
+      }
+      ClassWork& w = classes[ClassKey{K_XY, xp.op, xp.src.dtype, xp.tt, tier, cols}];
+      w.add_params(&q, sizeof(XYParams));
+      const int64_t nsl = (xy_nslot(q.ng, q.D, cols, q.cx) + 3) / 4 * 4;
+      if (nsl > w.nslot) w.nslot = nsl;
+      w.lds = xy_lds_bytes(w.nslot, cols);
+      w.add_job((uint64_t)((q.ngt + 255) / 256) * (uint64_t)q.nseg);
     }
-    const int cols = xp.cols;
-    const int tier = cols == 0 ? (py.c <= 5 ? 5 : 8) : ydyn_tier(py.c);
-    XYParams q;
-    memset(&q, 0, sizeof(q));
-    q.src = xp.src.ptr;
-    q.dst = tmp.ptr;
-    q.W = (int32_t)xp.dst.width;
-    q.H = (int32_t)xp.dst.height;
-    q.D = (int32_t)xp.dst.depth;
-    q.cx = px.c;
-    q.cy = py.c;
-    q.ng = (int32_t)((q.W + (cols ? cols : 8) - 1) / (cols ? cols : 8));
-    q.ngt = q.D * q.ng;
-    q.rstr = cols ? xy_rstr(q.ng, cols, q.cx) : 0;
-    int seg = (xyseg + kXYK - 1) / kXYK * kXYK;
-    const int hk = (q.H + kXYK - 1) / kXYK * kXYK;
-    if (seg > hk) seg = hk;
-    q.seg = seg;
-    q.nseg = (q.H + seg - 1) / seg;
-    const int ds = clk_dtype_size(xp.tt);
-    q.vec_st = cols && (q.W % cols == 0) && ((uintptr_t)tmp.ptr % (size_t)(cols * ds) == 0);
-    q.divx = px.div;
-    q.rdivx = px.rdiv;
-    q.fastx = px.fastdiv;
-    q.divy = py.div;
-    q.rdivy = py.rdiv;
-    q.fasty = py.fastdiv;
-    if (xp.op == OP_GAUSS) {
-      for (int t = 0; t <= 2 * px.c; ++t) q.wx[t] = px.w[t];
-      for (int t = 0; t <= 2 * py.c; ++t) q.wy[t + tier - py.c] = py.w[t];
+    for (auto& mj : medians) {
+      const clk_filter_job& jb = mj.first;
+      MedParams p;
+      memset(&p, 0, sizeof(p));
+      p.src = jb.src.ptr;
+      p.dst = jb.dst.ptr;
+      p.W = jb.dst.width;
+      p.H = jb.dst.height;
+      p.D = jb.dst.depth;
+      p.ex = jb.radius[0];
+      p.ey = jb.radius[1];
+      p.ez = jb.radius[2];
+      p.n = (2 * p.ex + 1) * (2 * p.ey + 1) * (2 * p.ez + 1);
+      if (p.W * p.H * p.D == 0) continue;
+      if (!batch_pair(jb.src.dtype, jb.dst.dtype)) {
+        std::string e2;
+        const int32_t rc = launch_median(st, jb.dst, jb.src, 2 * p.ex + 1, 2 * p.ey + 1, 2 * p.ez + 1, &e2);
+        if (rc) { *err = e2; return rc; }
+        continue;
+      }
+      dim3 grid;
+      int threads;
+      const MedKind mk = plan_median(p, clk_dtype_size(jb.dst.dtype), &grid, &threads);
+      // the 3x3x3 bodies slide along z and the packed 3x3 one along y, as in
+      // single launches
+      if (mk == MED_PK8) {
+        grid.z = (uint32_t)((p.D + kPk8ZRun - 1) / kPk8ZRun);
+      } else if (mk == MED_27 && clk_dtype_size(jb.dst.dtype) == 4) {
+        grid.z = (uint32_t)((p.D + kF27ZRun - 1) / kF27ZRun);
+      } else if (mk == MED_PK9) {
+        grid.y = (uint32_t)((p.H + kPk9YRun - 1) / kPk9YRun);
+        p.gy = grid.y;
+      }
+      ClassKey key{K_MEDIAN, (int)mk, jb.src.dtype, jb.dst.dtype};
+      ClassWork& w = classes[key];
+      w.add_params(&p, sizeof(MedParams));
+      w.add_job((uint64_t)grid.x * grid.y * grid.z);
     }
-    if (cols == 0) {
-      ClassKey key{K_XYMM, xp.op, xp.src.dtype, xp.tt, tier, 0};
-      ClassWork& w = classes[0][key];
-      w.xy.push_back(q);
-      const uint64_t waves = (uint64_t)((q.ngt + 61) / 62) * (uint64_t)q.nseg;  // 62 chunks per wave
-      w.add_job((waves + 3) / 4);
-      continue;
-    }
-    ClassKey key{K_XY, xp.op, xp.src.dtype, xp.tt, tier, cols};
-    ClassWork& w = classes[0][key];
-    w.xy.push_back(q);
-    const int64_t nsl = (xy_nslot(q.ng, q.D, cols, q.cx) + 3) / 4 * 4;
-    if (nsl > w.nslot) w.nslot = nsl;
-    w.lds = xy_lds_bytes(w.nslot, cols);
-    w.add_job((uint64_t)((q.ngt + 255) / 256) * (uint64_t)q.nseg);
-  }
-  for (auto& mj : medians) {
-    const clk_filter_job& jb = mj.first;
-    MedParams p;
-    memset(&p, 0, sizeof(p));
-    p.src = jb.src.ptr;
-    p.dst = jb.dst.ptr;
-    p.W = jb.dst.width;
-    p.H = jb.dst.height;
-    p.D = jb.dst.depth;
-    p.ex = jb.radius[0];
-    p.ey = jb.radius[1];
-    p.ez = jb.radius[2];
-    p.n = (2 * p.ex + 1) * (2 * p.ey + 1) * (2 * p.ez + 1);
-    if (p.W * p.H * p.D == 0) continue;
-    if (!batch_pair(jb.src.dtype, jb.dst.dtype)) {
-      std::string e2;
-      const int32_t rc = launch_median(st, jb.dst, jb.src, 2 * p.ex + 1, 2 * p.ey + 1, 2 * p.ez + 1, &e2);
-      if (rc) { *err = e2; return rc; }
-      continue;
-    }
-    dim3 grid;
-    int threads;
-    const MedKind mk = plan_median(p, clk_dtype_size(jb.dst.dtype), &grid, &threads);
-    // the 3x3x3 bodies slide along z and the packed 3x3 one along y, as in
-    // single launches
-    if (mk == MED_PK8) {
-      grid.z = (uint32_t)((p.D + kPk8ZRun - 1) / kPk8ZRun);
-    } else if (mk == MED_27 && clk_dtype_size(jb.dst.dtype) == 4) {
-      grid.z = (uint32_t)((p.D + kF27ZRun - 1) / kF27ZRun);
-    } else if (mk == MED_PK9) {
-      grid.y = (uint32_t)((p.H + kPk9YRun - 1) / kPk9YRun);
-      p.gy = grid.y;
-    }
-    ClassKey key{K_MEDIAN, (int)mk, jb.src.dtype, jb.dst.dtype};
-    ClassWork& w = classes[0][key];
-    w.med.push_back(p);
-    w.add_job((uint64_t)grid.x * grid.y * grid.z);
-  }
 
-  // ---- upload the launch tables -----------------------------------------------
-  size_t tbytes = 0;
-  for (int s = 0; s < 3; ++s)
-    for (auto& kv : classes[s]) {
+    return 0;
+  };
+
+  // ---- upload one stage's tables (pinned staging ring, stream order) --------
+  auto upload_stage = [&](std::map<ClassKey, ClassWork>& classes) -> int32_t {
+    size_t tb = 0;
+    for (auto& kv : classes) {
       ClassWork& w = kv.second;
-      w.params_off = tbytes;
-      tbytes += align256(w.sep.size() * sizeof(SepParams) + w.med.size() * sizeof(MedParams) +
-                         w.xy.size() * sizeof(XYParams));
-      w.off_off = tbytes;
-      tbytes += align256(w.off.size() * sizeof(uint32_t));
+      w.params_off = tused + tb;
+      tb += align256(w.blob.size());
+      w.off_off = tused + tb;
+      tb += align256(w.off.size() * sizeof(uint32_t));
     }
-  char* dev_tables = nullptr;
-  if (tbytes > 0) {
-    if (scratch.get(2, tbytes, (void**)&dev_tables) != hipSuccess) {
-      *err = "batch tables: allocation failed";
-      return CLK_MEM_OBJECT_ALLOCATION_FAILURE;
+    if (tb == 0) return 0;
+    if (tused + tb > tcap) {
+      *err = "internal: batch tables exceed their bound";
+      return CLK_OUT_OF_RESOURCES;
     }
     std::lock_guard<std::mutex> lk(g_stage_mu);
     Staging& sg = g_stage[g_stage_next];
-    g_stage_next = (g_stage_next + 1) % 4;
+    g_stage_next = (g_stage_next + 1) % kStagingSlots;
     if (sg.pending) {
       (void)hipEventSynchronize(sg.ev);
       sg.pending = false;
     }
     if (!sg.ev && hipEventCreateWithFlags(&sg.ev, hipEventDisableTiming) != hipSuccess) sg.ev = nullptr;
-    if (sg.cap < tbytes) {
+    if (sg.cap < tb) {
+      // grows by powers of two from 1 MB: freeing pinned memory waits for
+      // the whole device, so a slot is replaced rarely
+      size_t cap = size_t(1) << 20;
+      while (cap < tb) cap <<= 1;
       if (sg.host) (void)hipHostFree(sg.host);
-      sg.host = nullptr;
+      sg.host = sg.dev = nullptr;
       sg.cap = 0;
-      if (hipHostMalloc(&sg.host, tbytes, hipHostMallocDefault) != hipSuccess) {
+      if (hipHostMalloc(&sg.host, cap, hipHostMallocMapped) != hipSuccess ||
+          hipHostGetDevicePointer(&sg.dev, sg.host, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        if (sg.host) (void)hipHostFree(sg.host);
+        sg.host = sg.dev = nullptr;
         *err = "batch tables: pinned staging allocation failed";
         return CLK_OUT_OF_RESOURCES;
       }
-      sg.cap = tbytes;
+      sg.cap = cap;
     }
-    char* h = static_cast<char*>(sg.host);
-    for (int s = 0; s < 3; ++s)
-      for (auto& kv : classes[s]) {
-        const ClassWork& w = kv.second;
-        if (!w.sep.empty()) memcpy(h + w.params_off, w.sep.data(), w.sep.size() * sizeof(SepParams));
-        if (!w.med.empty()) memcpy(h + w.params_off, w.med.data(), w.med.size() * sizeof(MedParams));
-        if (!w.xy.empty()) memcpy(h + w.params_off, w.xy.data(), w.xy.size() * sizeof(XYParams));
-        memcpy(h + w.off_off, w.off.data(), w.off.size() * sizeof(uint32_t));
-      }
-    if (hipMemcpyAsync(dev_tables, h, tbytes, hipMemcpyHostToDevice, st) != hipSuccess) {
+    char* const h = static_cast<char*>(sg.host);  // device offset tused + i lands at h + i
+    for (auto& kv : classes) {
+      const ClassWork& w = kv.second;
+      if (!w.blob.empty()) memcpy(h + (w.params_off - tused), w.blob.data(), w.blob.size());
+      memcpy(h + (w.off_off - tused), w.off.data(), w.off.size() * sizeof(uint32_t));
+    }
+    // a copy kernel pulls the tables over the bus: hipMemcpyAsync of a few
+    // hundred KB from pinned memory blocked the host (it copied them itself,
+    // after waiting for the work queued before it, 0.2 to 1.3 ms a stage)
+    const uint32_t n16 = (uint32_t)(tb / 16);
+    const uint32_t nb = (n16 + 255) / 256 < 256 ? (n16 + 255) / 256 : 256;
+    hipLaunchKernelGGL(table_upload_kernel, dim3(nb), dim3(256), 0, st, static_cast<const u32x4*>(sg.dev),
+                       reinterpret_cast<u32x4*>(dev_tables + tused), n16);
+    if (hipGetLastError() != hipSuccess) {
       *err = "batch tables: upload failed";
       return CLK_OUT_OF_RESOURCES;
     }
     if (sg.ev && hipEventRecord(sg.ev, st) == hipSuccess) sg.pending = true;
-  }
+    tused += tb;
+    return 0;
+  };
 
   // ---- launch stage by stage --------------------------------------------------
-  int32_t rc = 0;
-  for (auto& c : copies) {
-    rc = launch_convert_copy(st, c.first, c.second, err);
-    if (rc) break;
-  }
-  // The classes of one stage touch disjoint jobs, so they run side by side
-  // on the stream's side streams (forked from st after the table upload and
-  // the previous stage, joined back into st): a class launch of a few
-  // thousand blocks no longer leaves the GPU idle during its ramp and tail.
-  // The largest classes go first, one per side stream in turn.
+  // Stage s + 1 is planned and its tables uploaded while stage s runs.  The
+  // classes of one stage touch disjoint jobs; with batch_serial 0 they run
+  // side by side on the stream's side streams (forked from st after the
+  // stage's table upload, joined back into st; the largest classes first,
+  // one per side stream in turn), which overlaps the ramp and tail of
+  // launches of a few thousand blocks but costs about 0.5 ms of host time a
+  // call in event calls; on the C4 batch the two measured alike, so the
+  // default runs them in order on st.
   hipStream_t* aux = nullptr;
   hipEvent_t* joins = nullptr;
   hipEvent_t fork = nullptr;
   const int naux = tuning(TUNE_BATCH_SERIAL) ? 0 : scratch.aux(&aux, &joins, &fork);
+  int32_t rc = 0;
+  for (auto& c : copies) {
+    rc = launch_convert_copy(st, c.first, c.second, err);
+    if (rc) return rc;
+  }
+  int used = 0;  // side streams of the previous stage, not yet joined
+  auto join = [&]() {
+    for (int a = 0; a < used; ++a)
+      if (hipEventRecord(joins[a], aux[a]) != hipSuccess || hipStreamWaitEvent(st, joins[a], 0) != hipSuccess) {
+        (void)hipStreamSynchronize(aux[a]);
+        if (rc == 0) {
+          *err = "batch: side stream join failed";
+          rc = CLK_OUT_OF_RESOURCES;
+        }
+      }
+    used = 0;
+  };
+  std::vector<std::map<ClassKey, ClassWork>> classes(3);
   for (int s = 0; s < 3 && rc == 0; ++s) {
+    rc = classify_stage(s, classes[s]);
+    if (rc == 0) rc = upload_stage(classes[s]);
+    join();  // stage s - 1 done before stage s starts (and before its singles)
+    if (rc) break;
     std::vector<std::pair<const ClassKey*, const ClassWork*>> order;
     for (auto& kv : classes[s])
       if (kv.second.nblocks > 0) order.push_back({&kv.first, &kv.second});
@@ -685,7 +755,6 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
       return a.second->nblocks > b.second->nblocks;
     });
     const bool side = naux > 0 && order.size() > 1;
-    int used = 0;
     if (side && hipEventRecord(fork, st) != hipSuccess) {
       *err = "batch: fork event record failed";
       rc = CLK_OUT_OF_RESOURCES;
@@ -720,20 +789,19 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
         break;
       }
     }
-    // join even after a failed launch, so st never runs ahead of side work
-    for (int a = 0; a < used; ++a)
-      if (hipEventRecord(joins[a], aux[a]) != hipSuccess || hipStreamWaitEvent(st, joins[a], 0) != hipSuccess) {
-        (void)hipStreamSynchronize(aux[a]);
-        if (rc == 0) {
-          *err = "batch: side stream join failed";
-          rc = CLK_OUT_OF_RESOURCES;
-        }
+    // passes without a batched class run as single launches on st, after
+    // their stage's classes
+    bool any_single = false;
+    for (const Single& sg : singles) any_single |= sg.stage == s;
+    if (any_single) {
+      join();
+      for (const Single& sg : singles) {
+        if (sg.stage != s || rc) continue;
+        rc = launch_separable(sg.pass.op, st, sg.pass.dst, sg.pass.src, sg.pass.dim, sg.pass.N, sg.pass.s, err);
       }
-    for (const Single& sg : singles) {
-      if (sg.stage != s || rc) continue;
-      rc = launch_separable(sg.pass.op, st, sg.pass.dst, sg.pass.src, sg.pass.dim, sg.pass.N, sg.pass.s, err);
     }
   }
+  join();  // even after a failed launch, so st never runs ahead of side work
   // float -> float medians: the NaN / -0 fix-up per job, one flag buffer
   // reused in stream order
   int64_t flag_bytes = 0;

@@ -31,7 +31,7 @@ template <int OP, class TS, class TD, int CMAX>
 __global__ __launch_bounds__(256) void batch_xtier_kernel(const SepParams* __restrict__ ps,
                                                           const BatchGrid grid) {
   const BatchTile t = batch_tile(grid);
-  sep_xtier_body<OP, TS, TD, CMAX, kXTierUnits>(ps[t.job], t.vblk);
+  sep_xtier_body<OP, TS, TD, CMAX, kXTierUnits>(batch_param(ps, grid, t.job), t.vblk);
 }
 
 template <int OP, class TS, class TD>

@@ -264,7 +264,7 @@ __global__ __launch_bounds__(256) void batch_xy_kernel(const XYParams* __restric
                                                        const int nslot) {
   extern __shared__ uint32_t lds[];
   const BatchTile t = batch_tile(grid);
-  sep_xy_body<OP, TS, TT, TT, CMAX, COLS>(ps[t.job], t.vblk, nslot, lds);
+  sep_xy_body<OP, TS, TT, TT, CMAX, COLS>(batch_param(ps, grid, t.job), t.vblk, nslot, lds);
 }
 
 // The chain's temporary type for a fused job's source: float for the

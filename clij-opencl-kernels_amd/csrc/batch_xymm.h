@@ -219,7 +219,7 @@ __device__ __forceinline__ void sep_xymm_body(const XYParams& p, const uint32_t 
 template <int OP, class TS, int CY>
 __global__ __launch_bounds__(256) void batch_xymm_kernel(const XYParams* __restrict__ ps, const BatchGrid grid) {
   const BatchTile t = batch_tile(grid);
-  sep_xymm_body<OP, TS, CY>(ps[t.job], t.vblk);
+  sep_xymm_body<OP, TS, CY>(batch_param(ps, grid, t.job), t.vblk);
 }
 
 }  // namespace clk

@@ -22,7 +22,7 @@ template <int OP, class TS, class TD, int CMAX, int COLS, int K>
 __global__ __launch_bounds__(256) void batch_ydyn_kernel(const SepParams* __restrict__ ps,
                                                          const BatchGrid grid) {
   const BatchTile t = batch_tile(grid);
-  sep_ydyn_body<OP, TS, TD, CMAX, COLS, K>(ps[t.job], t.vblk);
+  sep_ydyn_body<OP, TS, TD, CMAX, COLS, K>(batch_param(ps, grid, t.job), t.vblk);
 }
 
 

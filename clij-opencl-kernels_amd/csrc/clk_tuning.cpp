@@ -18,7 +18,7 @@ const char* kNames[TUNE_COUNT] = {"strided_rounds", "x_sub", "batch_seg",
                                   "autotune", "xreg_units", "x_dynamic", "x_tasks", "sphere_direct",
                                   "no_xpack", "yring", "no_xtier", "no_xy", "batch_xyseg",
                                   "batch_serial", "batch_ycols", "no_xymm"};
-const int32_t kDefaults[TUNE_COUNT] = {16, 0, 64, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 1, 0};
+const int32_t kDefaults[TUNE_COUNT] = {16, 0, 64, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 1, 1, 0};
 std::atomic<int32_t> g_val[TUNE_COUNT];
 std::once_flag g_once;
 

@@ -26,7 +26,8 @@ enum TuneKey : int {
   TUNE_NO_XTIER = 16,        // 1 = batched x passes never take the register-window tier classes
   TUNE_NO_XY = 17,           // 1 = batched 3D filters never fuse their x and y passes
   TUNE_BATCH_XYSEG = 18,     // fused x+y batch pass: y outputs per block (0 = 64)
-  TUNE_BATCH_SERIAL = 19,    // 1 = batch classes of a stage run one after another on the caller's stream
+  TUNE_BATCH_SERIAL = 19,    // 1 (default) = batch classes of a stage run one after another on the
+                             // caller's stream; 0 = side by side on side streams
   TUNE_BATCH_YCOLS = 20,     // batch y/z tier classes: at most this many columns per lane (0 = tier's own; default 1)
   TUNE_NO_XYMM = 21,         // 1 = batched 16-bit minimum/maximum keep separate x and y passes
   TUNE_COUNT = 22

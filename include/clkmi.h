@@ -333,8 +333,8 @@ int32_t clk_temporaries_held(void* stream, int64_t* stream_bytes, int64_t* pool_
  * 1 = never, 2 = always), "no_xtier" (1 = batched x passes keep the LDS row
  * classes), "no_xy" (1 = batched 3D filters keep separate x and y passes),
  * "batch_xyseg" (fused x+y batch pass: y outputs per block, 0 = 64),
- * "batch_serial" (1 = the classes of a batch stage run one after another on
- * the caller's stream instead of side by side on side streams),
+ * "batch_serial" (1, the default: the classes of a batch stage run one
+ * after another on the caller's stream; 0: side by side on side streams),
  * "batch_ycols" (batch y/z passes: at most 1, 2 or 4 columns per lane; 0 =
  * as the half-width tier allows; default 1), "no_xymm" (1 = batched 16-bit
  * minimum/maximum filters keep separate x and y passes),
