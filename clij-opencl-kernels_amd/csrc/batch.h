@@ -102,6 +102,12 @@ constexpr int kXYMMMaxC = 8;
 int32_t launch_xymm_class(hipStream_t st, int op, int ts, int tier, const XYParams* ps, const BatchGrid g,
                           uint32_t nblocks, std::string* err);
 
+// Fused x+y Gaussian/mean with float temporaries, x in registers and y in
+// registers, rows handed over through LDS (batch_xyg.h): half widths up to
+// kXYMaxC; x tiers 8, 16, y tiers 5, 8, 16; ng = chunks of 8 pixels per row.
+int32_t launch_xyg_class(hipStream_t st, int op, int ts, int xtier, int ytier, const XYParams* ps,
+                         const BatchGrid g, uint32_t nblocks, std::string* err);
+
 // y/z passes with a half width known at run time (sep_ydyn.h): the tier
 // (largest half width compiled: 5, 8 or 16; 0 = none fits), the columns
 // per lane the tier allows, and the outputs per step.

@@ -16,6 +16,8 @@ bool launch_xtier_mean(hipStream_t, int, int, int, const SepParams*, const Batch
 bool launch_xtier_min(hipStream_t, int, int, int, const SepParams*, const BatchGrid, uint32_t);
 bool launch_xtier_max(hipStream_t, int, int, int, const SepParams*, const BatchGrid, uint32_t);
 bool launch_xy_gauss(hipStream_t, int, int, int, const XYParams*, const BatchGrid, uint32_t, size_t, int);
+bool launch_xyg_gauss(hipStream_t, int, int, int, const XYParams*, const BatchGrid, uint32_t);
+bool launch_xyg_mean(hipStream_t, int, int, int, const XYParams*, const BatchGrid, uint32_t);
 bool launch_xy_mean(hipStream_t, int, int, int, const XYParams*, const BatchGrid, uint32_t, size_t, int);
 bool launch_xy_min(hipStream_t, int, int, int, const XYParams*, const BatchGrid, uint32_t, size_t, int);
 bool launch_xy_max(hipStream_t, int, int, int, const XYParams*, const BatchGrid, uint32_t, size_t, int);
@@ -63,6 +65,18 @@ int32_t launch_xy_class(hipStream_t st, int op, int ts, int tier, int cols, cons
   }
   if (!ok) {
     *err = "internal: no fused x+y class for this job";
+    return CLK_INVALID_VALUE;
+  }
+  return 0;
+}
+
+int32_t launch_xyg_class(hipStream_t st, int op, int ts, int xtier, int ytier, const XYParams* ps,
+                         const BatchGrid grid, uint32_t nblocks, std::string* err) {
+  bool ok = false;
+  if (op == OP_GAUSS) ok = launch_xyg_gauss(st, ts, xtier, ytier, ps, grid, nblocks);
+  else if (op == OP_MEAN) ok = launch_xyg_mean(st, ts, xtier, ytier, ps, grid, nblocks);
+  if (!ok) {
+    *err = "internal: no fused x+y Gaussian/mean class for this job";
     return CLK_INVALID_VALUE;
   }
   return 0;

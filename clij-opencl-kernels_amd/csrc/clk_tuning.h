@@ -30,7 +30,8 @@ enum TuneKey : int {
                              // caller's stream; 0 = side by side on side streams
   TUNE_BATCH_YCOLS = 20,     // batch y/z tier classes: at most this many columns per lane (0 = tier's own; default 1)
   TUNE_NO_XYMM = 21,         // 1 = batched 16-bit minimum/maximum keep separate x and y passes
-  TUNE_COUNT = 22
+  TUNE_NO_XYG = 22,          // 1 (default) = batched Gaussian/mean keep separate x and y passes
+  TUNE_COUNT = 23
 };
 
 int32_t tuning(TuneKey k);

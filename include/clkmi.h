@@ -337,7 +337,9 @@ int32_t clk_temporaries_held(void* stream, int64_t* stream_bytes, int64_t* pool_
  * after another on the caller's stream; 0: side by side on side streams),
  * "batch_ycols" (batch y/z passes: at most 1, 2 or 4 columns per lane; 0 =
  * as the half-width tier allows; default 1), "no_xymm" (1 = batched 16-bit
- * minimum/maximum filters keep separate x and y passes),
+ * minimum/maximum filters keep separate x and y passes), "no_xyg" (1, the
+ * default: batched Gaussian and mean filters keep separate x and y passes;
+ * 0: their register fused x+y class),
  * "sphere_direct" (ellipsoid filters: 0 = auto, 1 = per-voxel
  * gathers, 2 = LDS row tiles without z sliding), and
  * "autotune": 1 = the first single-axis launch of each signature (pixel

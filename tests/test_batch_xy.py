@@ -1,7 +1,9 @@
 """Fused x+y passes of the batch engine: a 3D filter whose x and y half
-widths are at most 16 runs its x pass inside the y pass (no x temporary);
-minimum/maximum of 16-bit integers with half widths up to 8 take the
-register-only form (8-pixel chunks per lane, packed 16-bit min/max).
+widths are at most 16 runs its x pass inside the y pass (no x temporary):
+the LDS-staged form (any op; off by default), the register form of
+minimum/maximum of 16-bit integers with half widths up to 8 (8-pixel chunks
+per lane, packed 16-bit min/max) and the register form of the Gaussian and
+the mean (x by DPP chunks, rows handed to the y ring through LDS).
 Every such job must equal the same batch with fusion off (separate x and y
 passes, already pinned against single calls) and the oracle chain, bit for
 bit, over row lengths that exercise every column width (1, 2, 4 columns per
@@ -83,13 +85,16 @@ def make_jobs(seed):
 @pytest.mark.parametrize("seed", [1, 2, 3])
 def test_xy_fused_equals_separate_and_oracle(gpu, seed):
     jobs = make_jobs(seed)
-    fused = run_batch(jobs, no_xy=0, no_xymm=0)
-    separate = run_batch(jobs, no_xy=1, no_xymm=1)
+    fused = run_batch(jobs, no_xy=0, no_xymm=0, no_xyg=1)
+    separate = run_batch(jobs, no_xy=1, no_xymm=1, no_xyg=1)
+    reg = run_batch(jobs, no_xy=1, no_xymm=0, no_xyg=0)
     for i, (job, f, s) in enumerate(zip(jobs, fused, separate)):
         tag = "job %d %s %s %s->%s sigma=%s r=%s" % (i, job[0], job[1].shape, job[1].dtype, np.dtype(job[2]),
                                                   job[3], job[4])
         assert same_bits_or_value(f, s), tag + " fused != separate"
         assert same_bits_or_value(f, oracle_job(*job)), tag + " fused != oracle"
+    for i, (job, g, s_) in enumerate(zip(jobs, reg, separate)):
+        assert same_bits_or_value(g, s_), "job %d %s %s register fused != separate" % (i, job[0], job[1].shape)
 
 
 @pytest.mark.parametrize("seg", [4, 8, 20, 1000])
@@ -97,10 +102,12 @@ def test_xy_segment_knob(gpu, seg):
     """The y segment per block (4 = one step, 20 = not a power of two, 1000 =
     whole columns) changes no bit."""
     jobs = make_jobs(4)[:14]
-    want = run_batch(jobs, no_xy=1, no_xymm=1)
-    got = run_batch(jobs, no_xy=0, no_xymm=0, batch_xyseg=seg)
-    for i, (g, w) in enumerate(zip(got, want)):
+    want = run_batch(jobs, no_xy=1, no_xymm=1, no_xyg=1)
+    got = run_batch(jobs, no_xy=0, no_xymm=0, no_xyg=1, batch_xyseg=seg)
+    reg = run_batch(jobs, no_xy=1, no_xymm=0, no_xyg=0, batch_xyseg=seg)
+    for i, (g, r, w) in enumerate(zip(got, reg, want)):
         assert same_bits_or_value(g, w), "job %d seg %d" % (i, seg)
+        assert same_bits_or_value(r, w), "job %d seg %d register fused" % (i, seg)
 
 
 def test_xy_float_specials(gpu):
@@ -122,10 +129,12 @@ def test_xy_float_specials(gpu):
             sigma = (sigma_for_c(c), sigma_for_c(16 - c), 1.0) if kind == "gaussian_blur_3d" else (0.0,) * 3
             radius = (0, 0, 0) if kind == "gaussian_blur_3d" else (c, max(1, 16 - c), 1)
             jobs.append((kind, a, np.float32, sigma, radius))
-    fused = run_batch(jobs, no_xy=0, no_xymm=0)
-    separate = run_batch(jobs, no_xy=1, no_xymm=1)
-    for i, (job, f, s) in enumerate(zip(jobs, fused, separate)):
+    fused = run_batch(jobs, no_xy=0, no_xymm=0, no_xyg=1)
+    separate = run_batch(jobs, no_xy=1, no_xymm=1, no_xyg=1)
+    reg = run_batch(jobs, no_xy=1, no_xymm=0, no_xyg=0)
+    for i, (job, f, s, r) in enumerate(zip(jobs, fused, separate, reg)):
         assert same_bits_or_value(f, s), "job %d %s" % (i, job[0])
+        assert same_bits_or_value(r, s), "job %d %s register fused" % (i, job[0])
         assert same_bits_or_value(f, oracle_job(*job)), "job %d %s oracle" % (i, job[0])
 
 
@@ -150,7 +159,7 @@ def test_xymm_register_class(gpu, seed):
         radius = (int(rng.integers(1, 10)), int(rng.integers(1, 10)), int(rng.integers(1, 4)))
         jobs.append((kind, src, dt, (0.0, 0.0, 0.0), radius))
     fused = run_batch(jobs, no_xymm=0)
-    separate = run_batch(jobs, no_xymm=1, no_xy=1)
+    separate = run_batch(jobs, no_xymm=1, no_xy=1, no_xyg=1)
     for i, (job, f, s_) in enumerate(zip(jobs, fused, separate)):
         tag = "job %d %s %s %s->%s r=%s" % (i, job[0], job[1].shape, job[1].dtype, np.dtype(job[2]), job[4])
         assert same_bits_or_value(f, s_), tag + " fused != separate"
