@@ -31,7 +31,8 @@ enum TuneKey : int {
   TUNE_BATCH_YCOLS = 20,     // batch y/z tier classes: at most this many columns per lane (0 = tier's own; default 1)
   TUNE_NO_XYMM = 21,         // 1 = batched 16-bit minimum/maximum keep separate x and y passes
   TUNE_NO_XYG = 22,          // 1 (default) = batched Gaussian/mean keep separate x and y passes
-  TUNE_COUNT = 23
+  TUNE_HIST8_ROWS = 23,      // 1 = 8-bit median histograms per output lane (not per column)
+  TUNE_COUNT = 24
 };
 
 int32_t tuning(TuneKey k);

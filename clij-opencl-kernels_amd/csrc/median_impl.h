@@ -802,6 +802,155 @@ __device__ __forceinline__ void median_hist8_fixed_body(const MedParams& p, uint
   }
 }
 
+// Order LDS accesses of the lanes of one wave (the block of the column
+// histogram kernel): a wave's LDS instructions execute in issue order, so
+// only the compiler must not move them across this point; a workgroup
+// barrier would also wait for the loads in flight for the next step.
+__device__ __forceinline__ void wave_lds_order() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+// Column histograms: the 5x5x5 (and any EX <= 3, EY <= 3) window at x is the
+// union of the 2EX+1 "columns" at x-EX .. x+EX, each column being the
+// (2EY+1) x (2ez+1) window of the y/z plane at one x.  Lane l keeps the
+// histogram of its own column (x = bx*(64-2EX) + l - EX, clamped into the
+// row, so lanes past the row ends hold the edge columns the clamp reads), so
+// a z step changes 2(2EY+1) counters per lane instead of 2(2EX+1)(2EY+1);
+// lanes EX .. 63-EX store outputs.  A column keeps 256 byte counters (64
+// dwords, dword d of lane l at d*64 + l) and 16 coarse byte counters of 16
+// bins each (4 dwords after them).  The window's counts are the byte-wise
+// sums of 2EX+1 lanes' dwords (no byte overflows: a count is at most
+// n <= 255), and its median is found without a walk: the coarse bin holding
+// rank n/2 from the 4 coarse window dwords (dword sums by v_sad_u8, then
+// bytes, compared branch-free), then the bin inside it from the 4 fine
+// dwords it spans.  Every lane does the same work each step.
+template <class TS, class TD, int EX, int EY>
+__device__ __forceinline__ void median_hist8_cols_body(const MedParams& p, uint32_t bx, uint32_t by, uint32_t bz) {
+  constexpr int NY = 2 * EY + 1;
+  constexpr int OUTW = 64 - 2 * EX;
+  using Raw = typename std::conditional<std::is_integral<TS>::value, int32_t, TS>::type;
+  __shared__ uint32_t hist[68 * 64];
+  const int lane = threadIdx.x;
+  const int W = (int)p.W;
+  const int64_t xo = (int64_t)bx * OUTW + lane - EX;
+  const int xi = xo < 0 ? 0 : (xo >= W ? W - 1 : (int)xo);
+  const bool out = lane >= EX && lane < 64 - EX && xo < W;
+  const int64_t y = by;
+  const int64_t z0 = (int64_t)bz * kHistZSeg;
+  const int64_t z1 = (z0 + kHistZSeg < p.D) ? z0 + kHistZSeg : p.D;
+#pragma unroll 4
+  for (int i = 0; i < 68; ++i) hist[i * 64 + lane] = 0u;
+  auto key_of = [&](Raw r) -> int {
+    if constexpr (std::is_same<TS, TD>::value && std::is_integral<TS>::value)
+      return (r - static_cast<int>(Pix<TD>::lo)) << 6;
+    else return hist8_bin<TS, TD>(static_cast<TS>(r), p.wrap) << 6;
+  };
+  // a key b << 6 (b the bin): fine counter dword at byte (key & 0x3f00) from
+  // the lane's column, byte shift (key >> 3) & 31; coarse bin b >> 4 in
+  // coarse dword key >> 12, byte shift (key >> 7) & 24
+  char* const col = reinterpret_cast<char*>(hist + lane);
+  auto update = [&](int key, bool inc) {
+    const uint32_t one = 1u << ((key >> 3) & 31);
+    const uint32_t cone = 1u << ((key >> 7) & 24);
+    __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(col + (key & 0x3f00)), inc ? one : 0u - one,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(col + 64 * 64 * 4 + ((key >> 12) << 8)),
+                           inc ? cone : 0u - cone, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  };
+  const TS* __restrict__ src = static_cast<const TS*>(p.src);
+  const int ez = p.ez;
+  int64_t rows[NY];
+#pragma unroll
+  for (int j = 0; j < NY; ++j) rows[j] = clamp64(y + j - EY, 0, p.H - 1) * p.W + xi;
+  auto load = [&](int64_t zz, int (&kk)[NY]) {
+    const TS* pl = src + clamp64(zz, 0, p.D - 1) * p.H * p.W;
+    Raw v[NY];
+#pragma unroll
+    for (int j = 0; j < NY; ++j) v[j] = static_cast<Raw>(pl[rows[j]]);
+#pragma unroll
+    for (int j = 0; j < NY; ++j) kk[j] = key_of(v[j]);
+  };
+  // the window's packed counts of dword i (fine 0..63, coarse 64..67):
+  // lanes l-EX .. l+EX (halo lanes read neighbours' words that exist, and
+  // their results are not stored)
+  const int wl = lane < EX ? EX : (lane > 63 - EX ? 63 - EX : lane);
+  const uint32_t* const hw = hist + wl - EX;
+  auto kword = [&](int i) -> uint32_t {
+    uint32_t s = 0;
+#pragma unroll
+    for (int dx = 0; dx <= 2 * EX; ++dx) s += hw[i * 64 + dx];
+    return s;
+  };
+  // among 16 packed byte counts (4 dwords), the byte holding rank r and the
+  // count below it
+  auto select16 = [](const uint32_t (&w)[4], int r, int& below) -> int {
+    const int s0 = (int)__builtin_amdgcn_sad_u8(w[0], 0u, 0u);
+    const int s1 = s0 + (int)__builtin_amdgcn_sad_u8(w[1], 0u, 0u);
+    const int s2 = s1 + (int)__builtin_amdgcn_sad_u8(w[2], 0u, 0u);
+    const int d = (r >= s0) + (r >= s1) + (r >= s2);
+    const int base = d == 0 ? 0 : (d == 1 ? s0 : (d == 2 ? s1 : s2));
+    const uint32_t x = d == 0 ? w[0] : (d == 1 ? w[1] : (d == 2 ? w[2] : w[3]));
+    const int b0 = (int)(x & 255u), b1 = b0 + (int)((x >> 8) & 255u), b2 = b1 + (int)((x >> 16) & 255u);
+    const int rr = r - base;
+    const int b = (rr >= b0) + (rr >= b1) + (rr >= b2);
+    below = base + (b == 0 ? 0 : (b == 1 ? b0 : (b == 2 ? b1 : b2)));
+    return 4 * d + b;
+  };
+  const int k = p.n / 2;
+  TD* __restrict__ dst = static_cast<TD*>(p.dst);
+  const int64_t plane = p.H * p.W;
+  auto median_out = [&](int64_t z) {
+    uint32_t cw[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) cw[q] = kword(64 + q);
+    int lt = 0;
+    const int c = select16(cw, k, lt);
+    uint32_t fw[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) fw[q] = kword(4 * c + q);
+    int lt2 = 0;
+    const int f = select16(fw, k - lt, lt2);
+    if (out) dst[z * plane + y * p.W + xi] = static_cast<TD>(16 * c + f + static_cast<int>(Pix<TD>::lo));
+  };
+  for (int dz = -ez; dz <= ez; ++dz) {
+    int kk[NY];
+    load(z0 + dz, kk);
+#pragma unroll
+    for (int j = 0; j < NY; ++j) update(kk[j], true);
+  }
+  wave_lds_order();  // every lane's counter updates before the reads
+  median_out(z0);
+  // the planes of steps z + 1 and z + 2 are in flight during step z
+  // (clamped past the volume): one step of work did not cover the latency
+  int ko[NY], ki[NY], no[NY], ni[NY], mo[NY], mi[NY];
+  load(z0 - ez, ko);
+  load(z0 + 1 + ez, ki);
+  load(z0 + 1 - ez, no);
+  load(z0 + 2 + ez, ni);
+  for (int64_t z = z0 + 1; z < z1; ++z) {
+    load(z + 1 - ez, mo);
+    load(z + 2 + ez, mi);
+    wave_lds_order();  // the previous step's reads before these updates
+#pragma unroll
+    for (int j = 0; j < NY; ++j) {
+      update(ko[j], false);
+      update(ki[j], true);
+    }
+    wave_lds_order();  // counter updates before the reads
+    median_out(z);
+#pragma unroll
+    for (int j = 0; j < NY; ++j) {
+      ko[j] = no[j];
+      ki[j] = ni[j];
+      no[j] = mo[j];
+      ni[j] = mi[j];
+    }
+  }
+}
+
+template <class TS, class TD, int EX, int EY>
+__global__ __launch_bounds__(64) void median_hist8_cols_kernel(const MedParams p) {
+  median_hist8_cols_body<TS, TD, EX, EY>(p, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
 #define CLK_MEDIAN_KERNEL(NAME, THREADS)                                              \
   template <class TS, class TD>                                                      \
   __global__ __launch_bounds__(THREADS) void NAME##_kernel(const MedParams p) {      \

@@ -32,7 +32,12 @@ static int32_t median_typed(hipStream_t st, MedParams& p, std::string* err) {
       if constexpr (sizeof(TD) == 1) {
 #define CLK_HIST8_FIXED(EX, EY)                                                                    \
   if (p.ex == EX && p.ey == EY) {                                                                  \
-    hipLaunchKernelGGL((median_hist8_fixed_kernel<TS, TD, EX, EY>), grid, dim3(threads), 0, st, p); \
+    if (tuning(TUNE_HIST8_ROWS) == 0) {                                                            \
+      const dim3 g((uint32_t)((p.W + (64 - 2 * EX) - 1) / (64 - 2 * EX)), grid.y, grid.z);          \
+      hipLaunchKernelGGL((median_hist8_cols_kernel<TS, TD, EX, EY>), g, dim3(threads), 0, st, p);   \
+    } else {                                                                                       \
+      hipLaunchKernelGGL((median_hist8_fixed_kernel<TS, TD, EX, EY>), grid, dim3(threads), 0, st, p); \
+    }                                                                                              \
     break;                                                                                         \
   }
       CLK_HIST8_FIXED(1, 1) CLK_HIST8_FIXED(1, 2) CLK_HIST8_FIXED(1, 3)

@@ -339,7 +339,8 @@ int32_t clk_temporaries_held(void* stream, int64_t* stream_bytes, int64_t* pool_
  * as the half-width tier allows; default 1), "no_xymm" (1 = batched 16-bit
  * minimum/maximum filters keep separate x and y passes), "no_xyg" (1, the
  * default: batched Gaussian and mean filters keep separate x and y passes;
- * 0: their register fused x+y class),
+ * 0: their register fused x+y class), "hist8_rows" (1 = 8-bit median
+ * histograms kept per output instead of per window column),
  * "sphere_direct" (ellipsoid filters: 0 = auto, 1 = per-voxel
  * gathers, 2 = LDS row tiles without z sliding), and
  * "autotune": 1 = the first single-axis launch of each signature (pixel

@@ -155,6 +155,42 @@ def test_median_hist8_wide_rows(gpu, src_t):
             assert_parity(got, want, "median %s -> %s %s" % (DTYPE_NAMES[src_t], DTYPE_NAMES[dst_t], Ns))
 
 
+@pytest.mark.parametrize("src_t,dst_t", [(np.uint8, np.uint8), (np.int8, np.int8), (np.uint16, np.uint8),
+                                         (np.float32, np.int8)], ids=lambda t: DTYPE_NAMES[t])
+def test_median_hist8_column_histograms(gpu, src_t, dst_t):
+    """8-bit dst medians with x and y half widths up to 3 keep one histogram
+    per window column (64 - 2 ex outputs per 64-lane block): rows narrower
+    than a block, one and two pixels wide, spanning several blocks; depths
+    of one plane and over several 64-plane segments; data that is constant
+    at either end of the range (the median walk's bounds) and salt and
+    pepper.  Equal to the per-output histograms and to the oracle."""
+    import clkmi
+    rng = np.random.default_rng(41)
+    cases = []
+    for shape in ((70, 5, 150), (3, 4, 1), (2, 3, 2), (1, 9, 61), (66, 3, 59), (5, 7, 121)):
+        cases.append(random_volume(shape, src_t, seed=int(rng.integers(1 << 30))))
+    lo, hi = (0, 255) if np.dtype(src_t).kind == "u" else (-128, 127)
+    cases.append(np.full((9, 6, 70), lo, dtype=src_t))
+    cases.append(np.full((9, 6, 70), hi, dtype=src_t))
+    sp = rng.integers(40, 200, size=(20, 8, 90)).astype(src_t)
+    m = rng.random(sp.shape) < 0.3
+    sp[m] = np.where(rng.random(int(m.sum())) < 0.5, lo, hi).astype(src_t)
+    cases.append(sp)
+    old = clkmi.get_tuning("hist8_rows")
+    try:
+        for src in cases:
+            for Ns in ((5, 5, 5), (3, 3, 5), (7, 7, 3), (5, 3, 7), (3, 7, 5), (7, 5, 1)):
+                clkmi.set_tuning("hist8_rows", 0)
+                got = gpu_median(src, dst_t, *Ns)
+                clkmi.set_tuning("hist8_rows", 1)
+                rows = gpu_median(src, dst_t, *Ns)
+                tag = "median %s %s %s->%s" % (src.shape, Ns, DTYPE_NAMES[src_t], DTYPE_NAMES[dst_t])
+                assert_parity(got, rows, tag + " columns vs rows")
+                assert_parity(got, O.median_box_3d(src, dst_t, *Ns), tag)
+    finally:
+        clkmi.set_tuning("hist8_rows", old)
+
+
 @pytest.mark.parametrize("src_t,dst_t", [(np.uint8, np.uint8), (np.int16, np.int16), (np.float32, np.uint16),
                                          (np.uint16, np.int8)], ids=lambda t: DTYPE_NAMES[t])
 def test_median_3x3x3_packed_z_runs(gpu, src_t, dst_t):
