@@ -68,6 +68,8 @@ struct XYParams {
   int32_t fastx, fasty;
   float wx[2 * kXYMaxC + 1];  // x weights in tap order t = 0..2cx
   float wy[2 * kXYMaxC + 1];  // y weights at CMAX - cy .. CMAX + cy (tier order)
+  int64_t sy, sz;  // K_XYMM: elements between rows of the walk, and between
+                   // its outer slices (x+y: W and H*W; a z pass: H*W and W)
 };
 
 // Staged words per plane row: the padded row rounded up to whole COLS-word
@@ -107,6 +109,25 @@ int32_t launch_xymm_class(hipStream_t st, int op, int ts, int tier, const XYPara
 // kXYMaxC; x tiers 8, 16, y tiers 5, 8, 16; ng = chunks of 8 pixels per row.
 int32_t launch_xyg_class(hipStream_t st, int op, int ts, int xtier, int ytier, const XYParams* ps,
                          const BatchGrid g, uint32_t nblocks, std::string* err);
+
+// A whole 3D minimum/maximum of 16-bit integer pixels in one launch
+// (batch_xyzmm.h): half widths up to kXYMMMaxC along x and y and kXYZMaxCZ
+// along z (one z tier).  A block (kXYZThreads threads) owns
+// R = kXYZThreads / ng whole rows (ng = chunks of 8 pixels per row),
+// TY = R - 2 cy of them outputs, and a segment of zseg planes; blocks of a
+// job = nyt tiles x nzseg segments.
+constexpr int kXYZThreads = 768;
+constexpr int kXYZMaxCZ = 5;
+struct MMParams {
+  const void* src;
+  void* dst;
+  int32_t W, H, D;
+  int32_t cx, cy, cz;
+  int32_t ng, R, TY, nyt;
+  int32_t zseg, nzseg;
+};
+int32_t launch_xyzmm_class(hipStream_t st, int op, int ts, int tier, const MMParams* ps, const BatchGrid g,
+                           uint32_t nblocks, std::string* err);
 
 // y/z passes with a half width known at run time (sep_ydyn.h): the tier
 // (largest half width compiled: 5, 8 or 16; 0 = none fits), the columns

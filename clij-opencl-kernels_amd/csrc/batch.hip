@@ -17,6 +17,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <map>
 #include <mutex>
 #include <string>
@@ -88,7 +89,7 @@ __global__ __launch_bounds__(64) void batch_hist8_kernel(const MedParams* __rest
 namespace {
 
 enum ClassKind : int { K_X = 0, K_STRIDED4 = 1, K_STRIDED1 = 2, K_MEDIAN = 3, K_XPACK = 4, K_YDYN = 5, K_XTIER = 6, K_XY = 7,
-                       K_XYMM = 8, K_XYG = 9 };
+                       K_XYMM = 8, K_XYG = 9, K_XYZMM = 10 };
 
 // Pixel-type pairs with batched kernels: same type, to float, from float.
 bool batch_pair(int ts, int td) { return ts == td || td == CLK_FLOAT || ts == CLK_FLOAT; }
@@ -234,6 +235,9 @@ void med_dst(hipStream_t st, const ClassKey& k, const ClassWork& w, char* dev) {
 }
 
 int32_t launch_class(hipStream_t st, const ClassKey& k, const ClassWork& w, char* dev, std::string* err) {
+  if (k.kind == K_XYZMM)
+    return launch_xyzmm_class(st, k.op, k.ts, k.tier, reinterpret_cast<const MMParams*>(dev + w.params_off),
+                              w.grid(dev), (uint32_t)w.nblocks, err);
   if (k.kind == K_XYG)
     return launch_xyg_class(st, k.op, k.ts, k.cols, k.tier, reinterpret_cast<const XYParams*>(dev + w.params_off),
                             w.grid(dev), (uint32_t)w.nblocks, err);
@@ -291,6 +295,27 @@ struct XYPass {
   int64_t dst_tmp;
   int cols;  // K_XY: columns per thread; 0: K_XYMM; -1: K_XYG (registers and LDS, 8-pixel chunks)
 };
+
+// Minimum/maximum of 16-bit integer pixels with half widths up to
+// kXYMMMaxC along x and y and kXYZMaxCZ along z, and rows short enough for
+// a tile of at least 4 output rows (the threads of a block hold
+// R = kXYZThreads / ng rows of ng chunks), can run whole in one pass
+// (batch_xyzmm.h).  Off by default: one block per CU and two barriers a
+// plane left it at about 1.2 TB/s on the uneven batch, 1.5 ms against 1.2 ms
+// for the fused x+y pass and the z pass of the same jobs.
+bool xyzmm_ok(int op, const clk_filter_job& jb, const int (&Ns)[3]) {
+  if (tuning(TUNE_NO_XYZMM) != 0) return false;
+  if (op != OP_MIN && op != OP_MAX) return false;
+  if (jb.src.dtype != CLK_USHORT && jb.src.dtype != CLK_SHORT) return false;
+  if (jb.dst.dtype != jb.src.dtype) return false;  // the chain's last pass converts; this class writes the source type
+  for (int a = 0; a < 3; ++a)
+    if (Ns[a] < 1 || (Ns[a] - 1) / 2 > (a == 2 ? kXYZMaxCZ : kXYMMMaxC)) return false;
+  const int64_t W = jb.dst.width, H = jb.dst.height, D = jb.dst.depth;
+  if (W <= 0 || H <= 0 || D <= 0 || W * H * D >= (int64_t(1) << 31) - 1) return false;
+  const int64_t ng = (W + 7) / 8;
+  const int cy = (Ns[1] - 1) / 2;
+  return ng <= kXYZThreads && kXYZThreads / ng - 2 * cy >= 4;
+}
 
 // Gaussian/mean jobs whose x and y half widths are at most kXYMaxC can take
 // the register fused class (batch_xyg.h) for 8-bit unsigned, 16-bit and
@@ -367,6 +392,7 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
   // ---- plan: passes per stage, temporaries --------------------------------
   std::vector<std::vector<Pass>> stage_passes(3);
   std::vector<XYPass> xy_passes;  // stage 0
+  std::vector<std::pair<clk_filter_job, std::array<int, 3>>> mm3;  // whole 3D min/max, stage 0
   std::vector<std::pair<clk_filter_job, int>> medians;  // job, unused
   std::vector<std::pair<clk_image, clk_image>> copies;  // no active axis: converting copy
 
@@ -398,6 +424,10 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
     // whose values are exact in any type that holds the source
     const int tdt = (op == OP_MIN || op == OP_MAX) ? jb.src.dtype : CLK_FLOAT;
     const int64_t V = jb.dst.width * jb.dst.height * jb.dst.depth;
+    if (nact == 3 && xyzmm_ok(op, jb, Ns)) {
+      mm3.push_back({jb, {Ns[0], Ns[1], Ns[2]}});
+      continue;
+    }
     if (nact == 3) {
       const bool mm = xymm_ok(op, jb, Ns);
       const bool xg = !mm && xyg_ok(op, jb, Ns);
@@ -465,6 +495,7 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
   size_t tcap = 0;
   for (int s = 0; s < 3; ++s) tcap += stage_passes[s].size() * (sizeof(SepParams) + 4 + 2 * 256 + 4);
   tcap += xy_passes.size() * (sizeof(XYParams) + 4 + 2 * 256 + 4);
+  tcap += mm3.size() * (sizeof(MMParams) + 4 + 2 * 256 + 4);
   tcap += medians.size() * (sizeof(MedParams) + 4 + 2 * 256 + 4);
   char* dev_tables = nullptr;
   if (tcap > 0 && scratch.get(2, tcap, (void**)&dev_tables) != hipSuccess) {
@@ -485,6 +516,37 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
       if (V == 0) continue;
       if (nan_fill || !batch_pair(ps.src.dtype, ps.dst.dtype) || p.L >= (int64_t(1) << 30)) {
         singles.push_back({s, ps});
+        continue;
+      }
+      if (ps.dim == 2 && (ps.op == OP_MIN || ps.op == OP_MAX) && ps.src.dtype == ps.dst.dtype &&
+          (ps.src.dtype == CLK_USHORT || ps.src.dtype == CLK_SHORT) && p.c >= 1 && p.c <= kXYMMMaxC &&
+          V < (int64_t(1) << 31) - 1 && tuning(TUNE_NO_ZMM) == 0) {
+        // 16-bit minimum/maximum along z: the register min/max class walking
+        // planes (8-pixel chunks, packed pairs).  Off by default: on the
+        // uneven batch it made the call 3.27 -> 3.48 ms against the y/z ring
+        // classes.
+        XYParams q;
+        memset(&q, 0, sizeof(q));
+        q.src = ps.src.ptr;
+        q.dst = ps.dst.ptr;
+        q.W = (int32_t)ps.dst.width;
+        q.H = (int32_t)ps.dst.depth;   // the walk
+        q.D = (int32_t)ps.dst.height;  // its outer slices
+        q.cx = 0;
+        q.cy = p.c;
+        q.ng = (q.W + 7) / 8;
+        q.ngt = q.D * q.ng;
+        int zseg = (xyseg + kXYK - 1) / kXYK * kXYK;
+        const int hk = (q.H + kXYK - 1) / kXYK * kXYK;
+        if (zseg > hk) zseg = hk;
+        q.seg = zseg;
+        q.nseg = (q.H + zseg - 1) / zseg;
+        q.sy = ps.dst.width * ps.dst.height;
+        q.sz = ps.dst.width;
+        ClassWork& w = classes[ClassKey{K_XYMM, ps.op, ps.src.dtype, ps.dst.dtype, p.c <= 5 ? 5 : 8, 0}];
+        w.add_params(&q, sizeof(XYParams));
+        const uint64_t waves = (uint64_t)((q.ngt + 61) / 62) * (uint64_t)q.nseg;  // 62 chunks per wave
+        w.add_job((waves + 3) / 4);
         continue;
       }
       ClassKey key{0, ps.op, ps.src.dtype, ps.dst.dtype};
@@ -569,6 +631,29 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
       w.add_job(blocks);
     }
     if (s != 0) return 0;
+    const int zseg = tuning(TUNE_BATCH_ZSEG) > 0 ? tuning(TUNE_BATCH_ZSEG) : 64;
+    for (const auto& m : mm3) {
+      const clk_filter_job& jb = m.first;
+      MMParams q;
+      memset(&q, 0, sizeof(q));
+      q.src = jb.src.ptr;
+      q.dst = jb.dst.ptr;
+      q.W = (int32_t)jb.dst.width;
+      q.H = (int32_t)jb.dst.height;
+      q.D = (int32_t)jb.dst.depth;
+      q.cx = (m.second[0] - 1) / 2;
+      q.cy = (m.second[1] - 1) / 2;
+      q.cz = (m.second[2] - 1) / 2;
+      q.ng = (q.W + 7) / 8;
+      q.R = kXYZThreads / q.ng;
+      q.TY = q.R - 2 * q.cy;
+      q.nyt = (q.H + q.TY - 1) / q.TY;
+      q.zseg = zseg;
+      q.nzseg = (q.D + zseg - 1) / zseg;
+      ClassWork& w = classes[ClassKey{K_XYZMM, jb.kind, jb.src.dtype, jb.dst.dtype, kXYZMaxCZ, 0}];
+      w.add_params(&q, sizeof(MMParams));
+      w.add_job((uint64_t)q.nyt * (uint64_t)q.nzseg);
+    }
     for (const XYPass& xp : xy_passes) {
       clk_image tmp = xp.dst;
       tmp.dtype = xp.tt;
@@ -603,6 +688,8 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
       if (seg > hk) seg = hk;
       q.seg = seg;
       q.nseg = (q.H + seg - 1) / seg;
+      q.sy = q.W;
+      q.sz = (int64_t)q.W * q.H;
       const int ds = clk_dtype_size(xp.tt);
       q.vec_st = cols > 0 && (q.W % cols == 0) && ((uintptr_t)tmp.ptr % (size_t)(cols * ds) == 0);
       q.divx = px.div;

@@ -20,6 +20,10 @@
 // integers are exact in any order, so the results are those of the x and y
 // single launches bit for bit.
 //
+// The same kernel runs a z pass (half width 0 along x, no x filter): the
+// walk then goes over planes (row stride H*W) and its outer slices are the
+// rows of a plane (stride W).
+//
 // Rows are loaded two steps ahead of their x filter: at step n the rows of
 // step n + 1 (loaded at step n - 2) are filtered along x into the ring and
 // the rows of step n + 3 are loaded, so two steps of compute cover their
@@ -68,8 +72,9 @@ __device__ __forceinline__ void sep_xymm_body(const XYParams& p, const uint32_t 
   const int z = gc / ng;
   const int j = gc - z * ng;
   const int x0 = 8 * j;
-  const TS* const zs = static_cast<const TS*>(p.src) + (int64_t)z * H * W;
-  TS* const zd = static_cast<TS*>(p.dst) + (int64_t)z * H * W + x0;
+  const TS* const zs = static_cast<const TS*>(p.src) + (int64_t)z * p.sz;
+  TS* const zd = static_cast<TS*>(p.dst) + (int64_t)z * p.sz + x0;
+  const int64_t sy = p.sy;
   const int cx = __builtin_amdgcn_readfirstlane(p.cx);
   const int cy = __builtin_amdgcn_readfirstlane(p.cy);
   const int tlo = CY - cy, thi = CY + cy;
@@ -80,7 +85,7 @@ __device__ __forceinline__ void sep_xymm_body(const XYParams& p, const uint32_t 
   // the lane's chunk of row y (clamped) as raw 16-bit pixel pairs
   auto load_row = [&](int y, uint32_t (&c)[4]) {
     y = y < 0 ? 0 : (y > H - 1 ? H - 1 : y);
-    const TS* const rp = zs + (int64_t)y * W;
+    const TS* const rp = zs + (int64_t)y * sy;
     if (full) {
       load_nt_words_ua<16>(rp + x0, c);
     } else {
@@ -93,6 +98,11 @@ __device__ __forceinline__ void sep_xymm_body(const XYParams& p, const uint32_t 
   };
   // the 8 x results of a loaded row as 4 key pairs
   auto xrow = [&](const uint32_t (&r)[4], uint32_t (&o)[4]) {
+    if (cx == 0) {  // a z pass: rows as they are
+#pragma unroll
+      for (int i = 0; i < 4; ++i) o[i] = r[i] ^ FLIP;
+      return;
+    }
     uint32_t e[12];  // window pixels x0 - 8 .. x0 + 15 as pairs (2i, 2i + 1)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -146,7 +156,7 @@ __device__ __forceinline__ void sep_xymm_body(const XYParams& p, const uint32_t 
 
   auto store_row = [&](const int y, const uint32_t (&o)[4]) {
     if (!active) return;
-    TS* const dp = zd + (int64_t)y * W;
+    TS* const dp = zd + (int64_t)y * sy;
     uint32_t v[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) v[i] = o[i] ^ FLIP;

@@ -32,7 +32,10 @@ enum TuneKey : int {
   TUNE_NO_XYMM = 21,         // 1 = batched 16-bit minimum/maximum keep separate x and y passes
   TUNE_NO_XYG = 22,          // 1 (default) = batched Gaussian/mean keep separate x and y passes
   TUNE_HIST8_ROWS = 23,      // 1 = 8-bit median histograms per output lane (not per column)
-  TUNE_COUNT = 24
+  TUNE_NO_XYZMM = 24,        // 1 (default) = batched 16-bit minimum/maximum never run as one 3D pass
+  TUNE_BATCH_ZSEG = 25,      // whole-3D batch pass: planes per block (0 = 64)
+  TUNE_NO_ZMM = 26,          // 1 (default) = batched 16-bit minimum/maximum z passes keep the y/z ring classes
+  TUNE_COUNT = 27
 };
 
 int32_t tuning(TuneKey k);

@@ -340,7 +340,12 @@ int32_t clk_temporaries_held(void* stream, int64_t* stream_bytes, int64_t* pool_
  * minimum/maximum filters keep separate x and y passes), "no_xyg" (1, the
  * default: batched Gaussian and mean filters keep separate x and y passes;
  * 0: their register fused x+y class), "hist8_rows" (1 = 8-bit median
- * histograms kept per output instead of per window column),
+ * histograms kept per output instead of per window column), "no_xyzmm" (1,
+ * the default: batched 16-bit minimum/maximum never run as one whole-3D
+ * pass; 0: they do where it applies),
+ * "batch_zseg" (whole-3D batch pass: planes per block, 0 = 64), "no_zmm"
+ * (1, the default: batched 16-bit minimum/maximum z passes keep the y/z
+ * ring classes; 0: they run the register min/max kernel over planes),
  * "sphere_direct" (ellipsoid filters: 0 = auto, 1 = per-voxel
  * gathers, 2 = LDS row tiles without z sliding), and
  * "autotune": 1 = the first single-axis launch of each signature (pixel

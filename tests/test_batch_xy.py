@@ -3,7 +3,9 @@ widths are at most 16 runs its x pass inside the y pass (no x temporary):
 the LDS-staged form (any op; off by default), the register form of
 minimum/maximum of 16-bit integers with half widths up to 8 (8-pixel chunks
 per lane, packed 16-bit min/max) and the register form of the Gaussian and
-the mean (x by DPP chunks, rows handed to the y ring through LDS).
+the mean (x by DPP chunks, rows handed to the y ring through LDS); and the
+whole 3D minimum/maximum of 16-bit integers in one pass (x and y through
+LDS, z in a register ring).
 Every such job must equal the same batch with fusion off (separate x and y
 passes, already pinned against single calls) and the oracle chain, bit for
 bit, over row lengths that exercise every column width (1, 2, 4 columns per
@@ -85,9 +87,9 @@ def make_jobs(seed):
 @pytest.mark.parametrize("seed", [1, 2, 3])
 def test_xy_fused_equals_separate_and_oracle(gpu, seed):
     jobs = make_jobs(seed)
-    fused = run_batch(jobs, no_xy=0, no_xymm=0, no_xyg=1)
-    separate = run_batch(jobs, no_xy=1, no_xymm=1, no_xyg=1)
-    reg = run_batch(jobs, no_xy=1, no_xymm=0, no_xyg=0)
+    fused = run_batch(jobs, no_xy=0, no_xymm=0, no_xyg=1, no_xyzmm=1)
+    separate = run_batch(jobs, no_xy=1, no_xymm=1, no_xyg=1, no_xyzmm=1, no_zmm=1)
+    reg = run_batch(jobs, no_xy=1, no_xymm=0, no_xyg=0, no_xyzmm=0)
     for i, (job, f, s) in enumerate(zip(jobs, fused, separate)):
         tag = "job %d %s %s %s->%s sigma=%s r=%s" % (i, job[0], job[1].shape, job[1].dtype, np.dtype(job[2]),
                                                   job[3], job[4])
@@ -102,9 +104,9 @@ def test_xy_segment_knob(gpu, seg):
     """The y segment per block (4 = one step, 20 = not a power of two, 1000 =
     whole columns) changes no bit."""
     jobs = make_jobs(4)[:14]
-    want = run_batch(jobs, no_xy=1, no_xymm=1, no_xyg=1)
-    got = run_batch(jobs, no_xy=0, no_xymm=0, no_xyg=1, batch_xyseg=seg)
-    reg = run_batch(jobs, no_xy=1, no_xymm=0, no_xyg=0, batch_xyseg=seg)
+    want = run_batch(jobs, no_xy=1, no_xymm=1, no_xyg=1, no_xyzmm=1, no_zmm=1)
+    got = run_batch(jobs, no_xy=0, no_xymm=0, no_xyg=1, no_xyzmm=1, batch_xyseg=seg)
+    reg = run_batch(jobs, no_xy=1, no_xymm=0, no_xyg=0, no_xyzmm=0, batch_xyseg=seg, batch_zseg=seg)
     for i, (g, r, w) in enumerate(zip(got, reg, want)):
         assert same_bits_or_value(g, w), "job %d seg %d" % (i, seg)
         assert same_bits_or_value(r, w), "job %d seg %d register fused" % (i, seg)
@@ -141,7 +143,8 @@ def test_xy_float_specials(gpu):
 @pytest.mark.parametrize("seed", [5, 6])
 def test_xymm_register_class(gpu, seed):
     """Minimum/maximum of uint16/int16 with x and y radii 1..8 (the register
-    class) and 9 (just past it), on rows with partial last chunks, waves
+    class; with the source type out its z pass runs the same kernel over
+    planes) and 9 (just past it), on rows with partial last chunks, waves
     spanning many planes, the type extremes (key flip of int16), heights
     below one step and over several segments: equal to separate passes and
     the oracle."""
@@ -158,9 +161,41 @@ def test_xymm_register_class(gpu, seed):
         kind = ("minimum_box_3d", "maximum_box_3d")[(i // 2) % 2]
         radius = (int(rng.integers(1, 10)), int(rng.integers(1, 10)), int(rng.integers(1, 4)))
         jobs.append((kind, src, dt, (0.0, 0.0, 0.0), radius))
-    fused = run_batch(jobs, no_xymm=0)
-    separate = run_batch(jobs, no_xymm=1, no_xy=1, no_xyg=1)
+    fused = run_batch(jobs, no_xymm=0, no_xyzmm=1)
+    separate = run_batch(jobs, no_xymm=1, no_xy=1, no_xyg=1, no_xyzmm=1, no_zmm=1)
     for i, (job, f, s_) in enumerate(zip(jobs, fused, separate)):
         tag = "job %d %s %s %s->%s r=%s" % (i, job[0], job[1].shape, job[1].dtype, np.dtype(job[2]), job[4])
         assert same_bits_or_value(f, s_), tag + " fused != separate"
         assert same_bits_or_value(f, oracle_job(*job)), tag + " fused != oracle"
+
+
+@pytest.mark.parametrize("seed", [7, 8])
+def test_xyzmm_whole_3d_class(gpu, seed):
+    """Whole-3D minimum/maximum of uint16/int16 (x, y radii 1..8, z radius
+    1..5; the source type out): tiles of whole rows over heights of one to
+    several tiles, rows of 1..300 pixels (partial last chunks, up to 38
+    chunks a row), depths of one plane to several z segments (64 and the
+    knob's 7), the type extremes.  Equal to separate passes and the
+    oracle; radii just past the class take the other paths and must match
+    too."""
+    rng = np.random.default_rng(seed)
+    shapes = [(3, 5, 1), (2, 7, 3), (1, 1, 1), (70, 9, 17), (5, 130, 40), (3, 41, 300), (9, 33, 64),
+              (130, 6, 9), (4, 90, 144), (2, 3, 257)]
+    jobs = []
+    for i, shape in enumerate(shapes):
+        st = (np.uint16, np.int16)[i % 2]
+        info = np.iinfo(st)
+        src = rng.integers(info.min, info.max, size=shape, endpoint=True).astype(st)
+        flat = src.reshape(-1)
+        flat[rng.choice(flat.size, max(1, flat.size // 40), replace=True)] = info.min
+        flat[rng.choice(flat.size, max(1, flat.size // 40), replace=True)] = info.max
+        kind = ("minimum_box_3d", "maximum_box_3d")[(i // 2) % 2]
+        radius = (int(rng.integers(1, 9)), int(rng.integers(1, 9)), int(rng.integers(1, 7)))
+        jobs.append((kind, src, st, (0.0, 0.0, 0.0), radius))
+    separate = run_batch(jobs, no_xymm=1, no_xy=1, no_xyg=1, no_xyzmm=1, no_zmm=1)
+    for knobs in ({}, {"batch_zseg": 7}):
+        whole = run_batch(jobs, no_xyzmm=0, **knobs)
+        for i, (job, w, s_) in enumerate(zip(jobs, whole, separate)):
+            tag = "job %d %s %s %s r=%s %s" % (i, job[0], job[1].shape, job[1].dtype, job[4], knobs)
+            assert same_bits_or_value(w, s_), tag + " whole-3D != separate"
+            assert same_bits_or_value(w, oracle_job(*job)), tag + " whole-3D != oracle"
