@@ -166,6 +166,22 @@ __device__ __forceinline__ void load_nt_words(const void* __restrict__ p, uint32
 typedef u32x4 u32x4_ua __attribute__((aligned(1)));
 typedef u32x2 u32x2_ua __attribute__((aligned(1)));
 
+// A buffer resource over the n bytes at p (n < 2^31), for raw buffer loads
+// whose address is a uniform base (scalar registers) plus a 32-bit lane
+// offset: no per-lane 64-bit address arithmetic.  Word 3 is the gfx9
+// family's raw-buffer setting.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t byte_rsrc(const void* p, int64_t n) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)n, 0x00020000);
+}
+
+// Pixel at byte offset off of a buffer resource (offsets inside it only).
+template <class T>
+__device__ __forceinline__ T rsrc_px(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  if constexpr (sizeof(T) == 1) return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b8(r, off, 0, 0));
+  else if constexpr (sizeof(T) == 2) return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0));
+  else return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+
 template <int B>
 __device__ __forceinline__ void load_nt_words_ua(const void* __restrict__ p, uint32_t (&w)[B / 4]) {
   static_assert(B == 8 || B % 16 == 0, "8 bytes or whole 16-byte vectors");

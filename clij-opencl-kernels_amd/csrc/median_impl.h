@@ -834,8 +834,11 @@ __device__ __forceinline__ void median_hist8_cols_body(const MedParams& p, uint3
   const int xi = xo < 0 ? 0 : (xo >= W ? W - 1 : (int)xo);
   const bool out = lane >= EX && lane < 64 - EX && xo < W;
   const int64_t y = by;
-  const int64_t z0 = (int64_t)bz * kHistZSeg;
-  const int64_t z1 = (z0 + kHistZSeg < p.D) ? z0 + kHistZSeg : p.D;
+  // z in 32 bits (D < 2^31, checked by the host): uniform 32-bit compares
+  // stay on the scalar unit
+  const int D = (int)p.D;
+  const int z0 = (int)bz * kHistZSeg;
+  const int z1 = (z0 + kHistZSeg < D) ? z0 + kHistZSeg : D;
 #pragma unroll 4
   for (int i = 0; i < 68; ++i) hist[i * 64 + lane] = 0u;
   auto key_of = [&](Raw r) -> int {
@@ -857,16 +860,21 @@ __device__ __forceinline__ void median_hist8_cols_body(const MedParams& p, uint3
   };
   const TS* __restrict__ src = static_cast<const TS*>(p.src);
   const int ez = p.ez;
-  int64_t rows[NY];
+  // byte offsets of the lane's pixels in a plane (a plane is under 2^31
+  // bytes, checked by the host): each load is a buffer load from the
+  // plane's resource at a 32-bit lane offset
+  uint32_t roff[NY];
 #pragma unroll
-  for (int j = 0; j < NY; ++j) rows[j] = clamp64(y + j - EY, 0, p.H - 1) * p.W + xi;
-  auto load = [&](int64_t zz, int (&kk)[NY]) {
-    const TS* pl = src + clamp64(zz, 0, p.D - 1) * p.H * p.W;
-    Raw v[NY];
+  for (int j = 0; j < NY; ++j)
+    roff[j] = (uint32_t)((clamp64(y + j - EY, 0, p.H - 1) * p.W + xi) * (int64_t)sizeof(TS));
+  const int64_t pbytes = p.H * p.W * (int64_t)sizeof(TS);
+  // the raw pixels of plane zz (clamped) at the lane's NY rows; their keys
+  // are formed where they are used, so a load in flight is not waited for
+  auto load = [&](int zz, Raw (&v)[NY]) {
+    zz = zz < 0 ? 0 : (zz > D - 1 ? D - 1 : zz);
+    const __amdgpu_buffer_rsrc_t r = byte_rsrc(reinterpret_cast<const char*>(src) + (int64_t)zz * pbytes, pbytes);
 #pragma unroll
-    for (int j = 0; j < NY; ++j) v[j] = static_cast<Raw>(pl[rows[j]]);
-#pragma unroll
-    for (int j = 0; j < NY; ++j) kk[j] = key_of(v[j]);
+    for (int j = 0; j < NY; ++j) v[j] = static_cast<Raw>(rsrc_px<TS>(r, roff[j]));
   };
   // the window's packed counts of dword i (fine 0..63, coarse 64..67):
   // lanes l-EX .. l+EX (halo lanes read neighbours' words that exist, and
@@ -881,67 +889,85 @@ __device__ __forceinline__ void median_hist8_cols_body(const MedParams& p, uint3
   };
   // among 16 packed byte counts (4 dwords), the byte holding rank r and the
   // count below it
+  // (each choice is a chain of selects on the monotone prefix tests: a
+  // chain on the index d itself is turned into a private array indexed at
+  // run time, i.e. a scratch store and load per output)
   auto select16 = [](const uint32_t (&w)[4], int r, int& below) -> int {
     const int s0 = (int)__builtin_amdgcn_sad_u8(w[0], 0u, 0u);
     const int s1 = s0 + (int)__builtin_amdgcn_sad_u8(w[1], 0u, 0u);
     const int s2 = s1 + (int)__builtin_amdgcn_sad_u8(w[2], 0u, 0u);
-    const int d = (r >= s0) + (r >= s1) + (r >= s2);
-    const int base = d == 0 ? 0 : (d == 1 ? s0 : (d == 2 ? s1 : s2));
-    const uint32_t x = d == 0 ? w[0] : (d == 1 ? w[1] : (d == 2 ? w[2] : w[3]));
+    const bool g0 = r >= s0, g1 = r >= s1, g2 = r >= s2;  // g2 => g1 => g0
+    int base = 0, d = 0;
+    uint32_t x = w[0];
+    if (g0) { base = s0; x = w[1]; d = 1; }
+    if (g1) { base = s1; x = w[2]; d = 2; }
+    if (g2) { base = s2; x = w[3]; d = 3; }
     const int b0 = (int)(x & 255u), b1 = b0 + (int)((x >> 8) & 255u), b2 = b1 + (int)((x >> 16) & 255u);
     const int rr = r - base;
-    const int b = (rr >= b0) + (rr >= b1) + (rr >= b2);
-    below = base + (b == 0 ? 0 : (b == 1 ? b0 : (b == 2 ? b1 : b2)));
+    const bool h0 = rr >= b0, h1 = rr >= b1, h2 = rr >= b2;
+    int lo = 0, b = 0;
+    if (h0) { lo = b0; b = 1; }
+    if (h1) { lo = b1; b = 2; }
+    if (h2) { lo = b2; b = 3; }
+    below = base + lo;
     return 4 * d + b;
   };
   const int k = p.n / 2;
   TD* __restrict__ dst = static_cast<TD*>(p.dst);
   const int64_t plane = p.H * p.W;
-  auto median_out = [&](int64_t z) {
-    uint32_t cw[4];
+  auto coarse_reads = [&](uint32_t (&cw)[4]) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) cw[q] = kword(64 + q);
-    int lt = 0;
-    const int c = select16(cw, k, lt);
-    uint32_t fw[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) fw[q] = kword(4 * c + q);
-    int lt2 = 0;
-    const int f = select16(fw, k - lt, lt2);
-    if (out) dst[z * plane + y * p.W + xi] = static_cast<TD>(16 * c + f + static_cast<int>(Pix<TD>::lo));
   };
   for (int dz = -ez; dz <= ez; ++dz) {
-    int kk[NY];
-    load(z0 + dz, kk);
+    Raw v[NY];
+    load(z0 + dz, v);
 #pragma unroll
-    for (int j = 0; j < NY; ++j) update(kk[j], true);
+    for (int j = 0; j < NY; ++j) update(key_of(v[j]), true);
   }
   wave_lds_order();  // every lane's counter updates before the reads
-  median_out(z0);
-  // the planes of steps z + 1 and z + 2 are in flight during step z
-  // (clamped past the volume): one step of work did not cover the latency
-  int ko[NY], ki[NY], no[NY], ni[NY], mo[NY], mi[NY];
-  load(z0 - ez, ko);
-  load(z0 + 1 + ez, ki);
-  load(z0 + 1 - ez, no);
-  load(z0 + 2 + ez, ni);
-  for (int64_t z = z0 + 1; z < z1; ++z) {
-    load(z + 1 - ez, mo);
-    load(z + 2 + ez, mi);
-    wave_lds_order();  // the previous step's reads before these updates
+  uint32_t cw[4];
+  coarse_reads(cw);
+  // The planes that step s (the window moving from s - 1 to s) removes and
+  // adds, in buffer (s - z0 - 1) % 3: three steps are in flight, so their
+  // loads are covered, and the buffers' roles are static (no copies).
+  Raw pb[3][2][NY];
 #pragma unroll
-    for (int j = 0; j < NY; ++j) {
-      update(ko[j], false);
-      update(ki[j], true);
-    }
-    wave_lds_order();  // counter updates before the reads
-    median_out(z);
+  for (int u = 0; u < 3; ++u) {
+    load(z0 + u - ez, pb[u][0]);
+    load(z0 + u + 1 + ez, pb[u][1]);
+  }
+  // Step z: the coarse bin from the coarse reads issued in step z - 1, the
+  // fine reads of z, then step z + 1's counter updates and coarse reads
+  // (after the last output too, so the body has no branch), and only then
+  // the fine bin of z.  A wave's LDS instructions run in issue order, so
+  // the fine reads see the counts of step z, and their latency is covered
+  // by z + 1's updates and reads.
+  for (int zb = z0;; zb += 3) {
 #pragma unroll
-    for (int j = 0; j < NY; ++j) {
-      ko[j] = no[j];
-      ki[j] = ni[j];
-      no[j] = mo[j];
-      ni[j] = mi[j];
+    for (int u = 0; u < 3; ++u) {
+      const int z = zb + u;
+      if (z >= z1) return;  // wave-uniform
+      int lt = 0;
+      const int c = select16(cw, k, lt);
+      uint32_t fw[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) fw[q] = kword(4 * c + q);
+      wave_lds_order();  // step z's reads before these updates
+#pragma unroll
+      for (int j = 0; j < NY; ++j) {
+        update(key_of(pb[u][0][j]), false);
+        update(key_of(pb[u][1][j]), true);
+      }
+      // step z + 4's planes into the buffer just used
+      load(z + 3 - ez, pb[u][0]);
+      load(z + 4 + ez, pb[u][1]);
+      wave_lds_order();  // counter updates before the reads
+      coarse_reads(cw);
+      int lt2 = 0;
+      const int f = select16(fw, k - lt, lt2);
+      TD* const drow = dst + (int64_t)z * plane + y * p.W;  // uniform
+      if (out) drow[(uint32_t)xi] = static_cast<TD>(16 * c + f + static_cast<int>(Pix<TD>::lo));
     }
   }
 }

@@ -32,7 +32,8 @@ static int32_t median_typed(hipStream_t st, MedParams& p, std::string* err) {
       if constexpr (sizeof(TD) == 1) {
 #define CLK_HIST8_FIXED(EX, EY)                                                                    \
   if (p.ex == EX && p.ey == EY) {                                                                  \
-    if (tuning(TUNE_HIST8_ROWS) == 0) {                                                            \
+    if (tuning(TUNE_HIST8_ROWS) == 0 && p.H * p.W * (int64_t)sizeof(TS) < (int64_t(1) << 31) &&    \
+        p.D < (int64_t(1) << 31)) {                                                                \
       const dim3 g((uint32_t)((p.W + (64 - 2 * EX) - 1) / (64 - 2 * EX)), grid.y, grid.z);          \
       hipLaunchKernelGGL((median_hist8_cols_kernel<TS, TD, EX, EY>), g, dim3(threads), 0, st, p);   \
     } else {                                                                                       \
