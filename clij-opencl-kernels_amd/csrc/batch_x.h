@@ -37,8 +37,8 @@ __device__ __forceinline__ void sep_xrows_body(const SepParams& p, const uint32_
   const int64_t row0 = (int64_t)task * p.nseg;
   const int k = (int)(p.outer - row0 < p.nseg ? p.outer - row0 : p.nseg);
   const int n = k * W;
-  const TS* __restrict__ s0 = static_cast<const TS*>(p.src) + row0 * W;
-  TD* __restrict__ d0 = static_cast<TD*>(p.dst) + row0 * W;
+  const CLK_GLOBAL TS* __restrict__ s0 = gmem(static_cast<const TS*>(p.src)) + row0 * W;
+  CLK_GLOBAL TD* __restrict__ d0 = gmem(static_cast<TD*>(p.dst)) + row0 * W;
   const float rW = 1.0f / (float)W;
   // (row, x) of span element f < 512 + W: a float quotient, then corrected
   auto rowx = [&](int f, int& r, int& x) {
@@ -61,7 +61,7 @@ __device__ __forceinline__ void sep_xrows_body(const SepParams& p, const uint32_
   }
   for (int ci = lane; ci < nch; ci += 64) {
     const int f0 = head + ci * EI;
-    const PixVec<TS, EI> v = load_nt<TS, EI>(s0 + f0);
+    const PixVec<TS, EI> v = load_nt<TS, EI>(gen(s0 + f0));
     int r, x;
     rowx(f0, r, x);
 #pragma unroll
@@ -119,7 +119,7 @@ __device__ __forceinline__ void sep_xrows_body(const SepParams& p, const uint32_
     TD o[EO];
 #pragma unroll
     for (int j = 0; j < EO; ++j) o[j] = obuf[f0 + j];
-    store_nt<TD, EO>(d0 + f0, o);
+    store_nt<TD, EO>(gen(d0 + f0), o);
   }
   for (int f = headd + nco * EO + lane; f < n; f += 64) d0[f] = obuf[f];
 }

@@ -113,7 +113,7 @@ __device__ __forceinline__ void sep_xy_body(const XYParams& p, const uint32_t vb
       const TS* const rp = src + (int64_t)y * W;
 #pragma unroll
       for (int m = 0; m < kXYSlots; ++m)
-        if (own[m]) v[k][m] = static_cast<VX>(rp[off[m]]);
+        if (own[m]) v[k][m] = static_cast<VX>(gmem(rp)[off[m]]);
     }
   };
   auto stage_write = [&](const int nrows) {
@@ -223,7 +223,7 @@ __device__ __forceinline__ void sep_xy_body(const XYParams& p, const uint32_t vb
     } else {
 #pragma unroll
       for (int j = 0; j < COLS; ++j)
-        if (x0 + j < W) dp[j] = o[j];
+        if (x0 + j < W) gmem(dp)[j] = o[j];
     }
   };
 

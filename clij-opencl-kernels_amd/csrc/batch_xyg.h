@@ -80,7 +80,7 @@ __device__ __forceinline__ void sep_xyg_body(const XYParams& p, const uint32_t v
     } else {
       TS v[8];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) v[i] = rp[(xs + i < W) ? xs + i : W - 1];
+      for (int i = 0; i < 8; ++i) v[i] = gmem(rp)[(xs + i < W) ? xs + i : W - 1];
       if constexpr (sizeof(TS) == 4) {
 #pragma unroll
         for (int i = 0; i < 8; ++i) c.w[i] = __builtin_bit_cast(uint32_t, v[i]);
@@ -233,9 +233,9 @@ __device__ __forceinline__ void sep_xyg_body(const XYParams& p, const uint32_t v
     float* const dp = zd + (int64_t)y * W;
     if (ystore2) {
       const u32x2 t = {__builtin_bit_cast(uint32_t, o[0]), __builtin_bit_cast(uint32_t, o[1])};
-      __builtin_nontemporal_store(t, reinterpret_cast<u32x2_ua*>(dp));
+      __builtin_nontemporal_store(t, reinterpret_cast<CLK_GLOBAL u32x2_ua*>(gmem(dp)));
     } else if (xy < W) {
-      dp[0] = o[0];
+      gmem(dp)[0] = o[0];
     }
   };
 

@@ -91,7 +91,7 @@ __device__ __forceinline__ void sep_xymm_body(const XYParams& p, const uint32_t 
     } else {
       uint16_t v[8];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) v[i] = (uint16_t)rp[(x0 + i < W) ? x0 + i : W - 1];
+      for (int i = 0; i < 8; ++i) v[i] = (uint16_t)gmem(rp)[(x0 + i < W) ? x0 + i : W - 1];
 #pragma unroll
       for (int i = 0; i < 4; ++i) c[i] = (uint32_t)v[2 * i] | ((uint32_t)v[2 * i + 1] << 16);
     }
@@ -162,11 +162,11 @@ __device__ __forceinline__ void sep_xymm_body(const XYParams& p, const uint32_t 
     for (int i = 0; i < 4; ++i) v[i] = o[i] ^ FLIP;
     if (full) {
       const u32x4 t = {v[0], v[1], v[2], v[3]};
-      __builtin_nontemporal_store(t, reinterpret_cast<u32x4_ua*>(dp));
+      __builtin_nontemporal_store(t, reinterpret_cast<CLK_GLOBAL u32x4_ua*>(gmem(dp)));
     } else {
 #pragma unroll
       for (int i = 0; i < 8; ++i)
-        if (x0 + i < W) dp[i] = (TS)(uint16_t)(v[i / 2] >> (16 * (i % 2)));
+        if (x0 + i < W) gmem(dp)[i] = (TS)(uint16_t)(v[i / 2] >> (16 * (i % 2)));
     }
   };
 

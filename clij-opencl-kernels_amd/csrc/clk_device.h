@@ -56,16 +56,65 @@ __device__ __forceinline__ float div_uniform_fast(float a, float b, float r) {
   return __builtin_fmaf(e, r, q);
 }
 
+// Image pixels live in global memory.  The batch engine reads its image
+// pointers from a table in memory, where the compiler cannot tell them from
+// LDS or scratch pointers and emits flat accesses, which are waited for at
+// once (a flat access may complete out of order), so nothing overlaps a
+// load's latency.  Every image access goes through a global-space pointer.
+#define CLK_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ const CLK_GLOBAL T* gmem(const T* p) { return (const CLK_GLOBAL T*)p; }
+template <class T>
+__device__ __forceinline__ CLK_GLOBAL T* gmem(T* p) { return (CLK_GLOBAL T*)p; }
+// Back to a generic pointer, for the helpers below that take one (they
+// access through gmem again).
+template <class T>
+__device__ __forceinline__ const T* gen(const CLK_GLOBAL T* p) { return (const T*)p; }
+template <class T>
+__device__ __forceinline__ T* gen(CLK_GLOBAL T* p) { return (T*)p; }
+
 // Load COLS consecutive pixels of type T, widened to float.  `p` must be
 // aligned to sizeof(T)*COLS when COLS > 1.
 template <class T, int COLS> struct alignas(sizeof(T) * COLS) PixVec { T v[COLS]; };
 
+// N pixels from / to global memory as one access (a vector of builtin type:
+// a class object cannot be copied through a global-space reference); N that
+// is not a power of two goes pixel by pixel.
+template <class T, int N>
+__device__ __forceinline__ PixVec<T, N> gload_vec(const T* p) {
+  PixVec<T, N> v;
+  if constexpr (N > 1 && (N & (N - 1)) == 0) {
+    typedef T V __attribute__((ext_vector_type(N)));
+    const V x = *reinterpret_cast<const CLK_GLOBAL V*>(gmem(p));
+#pragma unroll
+    for (int i = 0; i < N; ++i) v.v[i] = x[i];
+  } else {
+#pragma unroll
+    for (int i = 0; i < N; ++i) v.v[i] = gmem(p)[i];
+  }
+  return v;
+}
+
+template <class T, int N>
+__device__ __forceinline__ void gstore_vec(T* p, const PixVec<T, N>& v) {
+  if constexpr (N > 1 && (N & (N - 1)) == 0) {
+    typedef T V __attribute__((ext_vector_type(N)));
+    V x;
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = v.v[i];
+    *reinterpret_cast<CLK_GLOBAL V*>(gmem(p)) = x;
+  } else {
+#pragma unroll
+    for (int i = 0; i < N; ++i) gmem(p)[i] = v.v[i];
+  }
+}
+
 template <class T, int COLS>
 __device__ __forceinline__ void load_widen(const T* __restrict__ p, float (&out)[COLS]) {
   if constexpr (COLS == 1) {
-    out[0] = static_cast<float>(p[0]);
+    out[0] = static_cast<float>(gmem(p)[0]);
   } else {
-    const PixVec<T, COLS> v = *reinterpret_cast<const PixVec<T, COLS>*>(p);
+    const PixVec<T, COLS> v = gload_vec<T, COLS>(p);
 #pragma unroll
     for (int i = 0; i < COLS; ++i) out[i] = static_cast<float>(v.v[i]);
   }
@@ -74,12 +123,12 @@ __device__ __forceinline__ void load_widen(const T* __restrict__ p, float (&out)
 template <class T, int COLS>
 __device__ __forceinline__ void store_vec(T* __restrict__ p, const T (&in)[COLS]) {
   if constexpr (COLS == 1) {
-    p[0] = in[0];
+    gmem(p)[0] = in[0];
   } else {
     PixVec<T, COLS> v;
 #pragma unroll
     for (int i = 0; i < COLS; ++i) v.v[i] = in[i];
-    *reinterpret_cast<PixVec<T, COLS>*>(p) = v;
+    gstore_vec<T, COLS>(p, v);
   }
 }
 
@@ -101,15 +150,15 @@ __device__ __forceinline__ void store_nt(T* __restrict__ p, const T (&in)[N]) {
     uint32_t w[B / 4];
     __builtin_memcpy(w, &v, B);
     if constexpr (B == 4) {
-      __builtin_nontemporal_store(w[0], reinterpret_cast<uint32_t*>(p));
+      __builtin_nontemporal_store(w[0], reinterpret_cast<CLK_GLOBAL uint32_t*>(gmem(p)));
     } else if constexpr (B == 8) {
       const u32x2 t = {w[0], w[1]};
-      __builtin_nontemporal_store(t, reinterpret_cast<u32x2*>(p));
+      __builtin_nontemporal_store(t, reinterpret_cast<CLK_GLOBAL u32x2*>(gmem(p)));
     } else {
 #pragma unroll
       for (int k = 0; k < B / 16; ++k) {
         const u32x4 t = {w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]};
-        __builtin_nontemporal_store(t, reinterpret_cast<u32x4*>(p) + k);
+        __builtin_nontemporal_store(t, reinterpret_cast<CLK_GLOBAL u32x4*>(gmem(p)) + k);
       }
     }
   }
@@ -121,18 +170,18 @@ __device__ __forceinline__ PixVec<T, N> load_nt(const T* __restrict__ p) {
   constexpr int B = N * (int)sizeof(T);
   PixVec<T, N> v;
   if constexpr (B % 4 != 0 || (B > 8 && B % 16 != 0)) {
-    v = *reinterpret_cast<const PixVec<T, N>*>(p);
+    v = gload_vec<T, N>(p);
   } else {
     uint32_t w[B / 4];
     if constexpr (B == 4) {
-      w[0] = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(p));
+      w[0] = __builtin_nontemporal_load(reinterpret_cast<const CLK_GLOBAL uint32_t*>(gmem(p)));
     } else if constexpr (B == 8) {
-      const u32x2 t = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(p));
+      const u32x2 t = __builtin_nontemporal_load(reinterpret_cast<const CLK_GLOBAL u32x2*>(gmem(p)));
       w[0] = t.x; w[1] = t.y;
     } else {
 #pragma unroll
       for (int k = 0; k < B / 16; ++k) {
-        const u32x4 t = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p) + k);
+        const u32x4 t = __builtin_nontemporal_load(reinterpret_cast<const CLK_GLOBAL u32x4*>(gmem(p)) + k);
         w[4 * k] = t.x; w[4 * k + 1] = t.y; w[4 * k + 2] = t.z; w[4 * k + 3] = t.w;
       }
     }
@@ -147,12 +196,12 @@ template <int B>
 __device__ __forceinline__ void load_nt_words(const void* __restrict__ p, uint32_t (&w)[B / 4]) {
   static_assert(B == 8 || B % 16 == 0, "8 bytes or whole 16-byte vectors");
   if constexpr (B == 8) {
-    const u32x2 t = __builtin_nontemporal_load(static_cast<const u32x2*>(p));
+    const u32x2 t = __builtin_nontemporal_load(static_cast<const CLK_GLOBAL u32x2*>(gmem(p)));
     w[0] = t.x; w[1] = t.y;
   } else {
 #pragma unroll
     for (int k = 0; k < B / 16; ++k) {
-      const u32x4 t = __builtin_nontemporal_load(static_cast<const u32x4*>(p) + k);
+      const u32x4 t = __builtin_nontemporal_load(static_cast<const CLK_GLOBAL u32x4*>(gmem(p)) + k);
       w[4 * k] = t.x; w[4 * k + 1] = t.y; w[4 * k + 2] = t.z; w[4 * k + 3] = t.w;
     }
   }
@@ -186,12 +235,12 @@ template <int B>
 __device__ __forceinline__ void load_nt_words_ua(const void* __restrict__ p, uint32_t (&w)[B / 4]) {
   static_assert(B == 8 || B % 16 == 0, "8 bytes or whole 16-byte vectors");
   if constexpr (B == 8) {
-    const u32x2 t = __builtin_nontemporal_load(static_cast<const u32x2_ua*>(p));
+    const u32x2 t = __builtin_nontemporal_load(static_cast<const CLK_GLOBAL u32x2_ua*>(gmem(p)));
     w[0] = t.x; w[1] = t.y;
   } else {
 #pragma unroll
     for (int k = 0; k < B / 16; ++k) {
-      const u32x4 t = __builtin_nontemporal_load(static_cast<const u32x4_ua*>(p) + k);
+      const u32x4 t = __builtin_nontemporal_load(static_cast<const CLK_GLOBAL u32x4_ua*>(gmem(p)) + k);
       w[4 * k] = t.x; w[4 * k + 1] = t.y; w[4 * k + 2] = t.z; w[4 * k + 3] = t.w;
     }
   }
@@ -210,12 +259,12 @@ __device__ __forceinline__ void store_nt_ua(T* __restrict__ p, const T (&in)[N])
   __builtin_memcpy(w, &v, B);
   if constexpr (B == 8) {
     const u32x2 t = {w[0], w[1]};
-    __builtin_nontemporal_store(t, reinterpret_cast<u32x2_ua*>(p));
+    __builtin_nontemporal_store(t, reinterpret_cast<CLK_GLOBAL u32x2_ua*>(gmem(p)));
   } else {
 #pragma unroll
     for (int k = 0; k < B / 16; ++k) {
       const u32x4 t = {w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]};
-      __builtin_nontemporal_store(t, reinterpret_cast<u32x4_ua*>(p) + k);
+      __builtin_nontemporal_store(t, reinterpret_cast<CLK_GLOBAL u32x4_ua*>(gmem(p)) + k);
     }
   }
 }

@@ -92,14 +92,14 @@ __device__ __forceinline__ void median_generic_body(const MedParams& p, uint32_t
   const int64_t y = by;
   const int64_t z = bz;
   if (x >= p.W) return;
-  const TS* __restrict__ src = static_cast<const TS*>(p.src);
+  const CLK_GLOBAL TS* __restrict__ src = gmem(static_cast<const TS*>(p.src));
   // f(key) for every value of the window
   auto each = [&](auto&& f) {
     for (int dz = -p.ez; dz <= p.ez; ++dz) {
       const int64_t zz = clamp64(z + dz, 0, p.D - 1);
       for (int dy = -p.ey; dy <= p.ey; ++dy) {
         const int64_t yy = clamp64(y + dy, 0, p.H - 1);
-        const TS* rowp = src + (zz * p.H + yy) * p.W;
+        const CLK_GLOBAL TS* rowp = src + (zz * p.H + yy) * p.W;
         for (int dx = -p.ex; dx <= p.ex; ++dx) f(value_key<TD>(gather_value<TS, TD>(rowp[clamp64(x + dx, 0, p.W - 1)], p.wrap)));
       }
     }
@@ -126,7 +126,7 @@ __device__ __forceinline__ void median_generic_body(const MedParams& p, uint32_t
   }
   uint32_t r = 0;
   each([&](uint32_t v) { r = (v <= hi && v > r) ? v : r; });
-  static_cast<TD*>(p.dst)[(z * p.H + y) * p.W + x] = static_cast<TD>(key_value<TD>(r));
+  gmem(static_cast<TD*>(p.dst))[(z * p.H + y) * p.W + x] = static_cast<TD>(key_value<TD>(r));
 }
 
 // Median (rank N/2) of N = 2k+1 values held in registers by forgetful
@@ -233,7 +233,7 @@ __device__ __forceinline__ void median_regs_body(const MedParams& p, uint32_t bx
   const int64_t y = by;
   const int64_t z = bz;
   if (x >= p.W) return;
-  const TS* __restrict__ src = static_cast<const TS*>(p.src);
+  const CLK_GLOBAL TS* __restrict__ src = gmem(static_cast<const TS*>(p.src));
   int64_t xs[NX];
 #pragma unroll
   for (int dx = -MX; dx <= MX; ++dx) xs[dx + MX] = clamp64(x + dx, 0, p.W - 1);
@@ -245,7 +245,7 @@ __device__ __forceinline__ void median_regs_body(const MedParams& p, uint32_t bx
 #pragma unroll
     for (int dy = -MY; dy <= MY; ++dy) {
       const int64_t yy = clamp64(y + dy, 0, p.H - 1);
-      const TS* rowp = src + (zz * p.H + yy) * p.W;
+      const CLK_GLOBAL TS* rowp = src + (zz * p.H + yy) * p.W;
       const bool row_in = dz >= -p.ez && dz <= p.ez && dy >= -p.ey && dy <= p.ey;  // wave-uniform
 #pragma unroll
       for (int dx = -MX; dx <= MX; ++dx) {
@@ -259,7 +259,7 @@ __device__ __forceinline__ void median_regs_body(const MedParams& p, uint32_t bx
       }
     }
   }
-  TD* const out = static_cast<TD*>(p.dst) + (z * p.H + y) * p.W + x;
+  CLK_GLOBAL TD* const out = gmem(static_cast<TD*>(p.dst)) + (z * p.H + y) * p.W + x;
   // A box that fills the tier has its rank at a compile-time position, so
   // the pruned network applies; it is taken when the bisection would need
   // more instructions (about two per slot for each bit of the key range:
@@ -315,7 +315,7 @@ __device__ __forceinline__ void median_27_pk8_slide_body(const MedParams& p, uin
   const int64_t z0 = (int64_t)bz * kPk8ZRun;
   const int64_t z1 = (z0 + kPk8ZRun < p.D) ? z0 + kPk8ZRun : p.D;
   const int W = (int)p.W;
-  const TS* __restrict__ src = static_cast<const TS*>(p.src);
+  const CLK_GLOBAL TS* __restrict__ src = gmem(static_cast<const TS*>(p.src));
   constexpr int kBias = -static_cast<int>(Pix<TD>::lo);  // signed values shift to 0..2^bits-1
   // this thread's columns x = xt - 1 + 2t + i, i = 0..3 (i = 3 for t = 255 only), clamped
   int64_t xc[4];
@@ -330,7 +330,7 @@ __device__ __forceinline__ void median_27_pk8_slide_body(const MedParams& p, uin
   const int64_t PS = p.H * p.W;
   TS raw[3][4];
   auto fetch = [&](int64_t zz) {
-    const TS* pl = src + clamp64(zz, 0, p.D - 1) * PS;
+    const CLK_GLOBAL TS* pl = src + clamp64(zz, 0, p.D - 1) * PS;
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
 #pragma unroll
@@ -357,7 +357,7 @@ __device__ __forceinline__ void median_27_pk8_slide_body(const MedParams& p, uin
   put(2);
   __syncthreads();
   const int64_t x = xt + 2 * t;
-  TD* __restrict__ dst = static_cast<TD*>(p.dst);
+  CLK_GLOBAL TD* __restrict__ dst = gmem(static_cast<TD*>(p.dst));
   for (int64_t z = z0; z < z1; ++z) {
     const int zi = (int)(z - z0);
     const bool more = z + 1 < z1;
@@ -375,7 +375,7 @@ __device__ __forceinline__ void median_27_pk8_slide_body(const MedParams& p, uin
       }
     }
     const us2 m = batcher_median<27>(a);
-    TD* const row = dst + (z * p.H + y) * p.W;
+    CLK_GLOBAL TD* const row = dst + (z * p.H + y) * p.W;
     if (x < W) row[x] = static_cast<TD>((int)m.x - kBias);
     if (x + 1 < W) row[x + 1] = static_cast<TD>((int)m.y - kBias);
     if (more) put((zi + 3) & 3);
@@ -400,7 +400,7 @@ __device__ __forceinline__ void median_9_pk8_slide_body(const MedParams& p, uint
   const int64_t y1 = (y0 + kPk9YRun < p.H) ? y0 + kPk9YRun : p.H;
   const int64_t z = bz;
   const int W = (int)p.W;
-  const TS* __restrict__ plane = static_cast<const TS*>(p.src) + z * p.H * p.W;
+  const CLK_GLOBAL TS* __restrict__ plane = gmem(static_cast<const TS*>(p.src)) + z * p.H * p.W;
   constexpr int kBias = -static_cast<int>(Pix<TD>::lo);
   int64_t xc[4];  // x = xt - 1 + 2t + i, clamped (i = 3 for t = 255 only)
 #pragma unroll
@@ -410,7 +410,7 @@ __device__ __forceinline__ void median_9_pk8_slide_body(const MedParams& p, uint
   }
   TS raw[4];
   auto fetch = [&](int64_t yy) {
-    const TS* row = plane + clamp64(yy, 0, p.H - 1) * p.W;
+    const CLK_GLOBAL TS* row = plane + clamp64(yy, 0, p.H - 1) * p.W;
 #pragma unroll
     for (int i = 0; i < 3; ++i) raw[i] = row[xc[i]];
     raw[3] = t == 255 ? row[xc[3]] : TS(0);
@@ -431,7 +431,7 @@ __device__ __forceinline__ void median_9_pk8_slide_body(const MedParams& p, uint
   put(2);
   __syncthreads();
   const int64_t x = xt + 2 * t;
-  TD* __restrict__ dst = static_cast<TD*>(p.dst) + z * p.H * p.W;
+  CLK_GLOBAL TD* __restrict__ dst = gmem(static_cast<TD*>(p.dst)) + z * p.H * p.W;
   for (int64_t y = y0; y < y1; ++y) {
     const int yi = (int)(y - y0);
     const bool more = y + 1 < y1;
@@ -446,7 +446,7 @@ __device__ __forceinline__ void median_9_pk8_slide_body(const MedParams& p, uint
       __builtin_memcpy(&a[3 * dr + 2], &w2, 4);
     }
     const us2 m = forgetful_median<9>(a);
-    TD* const row = dst + y * p.W;
+    CLK_GLOBAL TD* const row = dst + y * p.W;
     if (x < W) row[x] = static_cast<TD>((int)m.x - kBias);
     if (x + 1 < W) row[x + 1] = static_cast<TD>((int)m.y - kBias);
     if (more) put((yi + 3) & 3);
@@ -460,7 +460,7 @@ __device__ __forceinline__ void median_27_body(const MedParams& p, uint32_t bx, 
   const int64_t y = by;
   const int64_t z = bz;
   if (x >= p.W) return;
-  const TS* __restrict__ src = static_cast<const TS*>(p.src);
+  const CLK_GLOBAL TS* __restrict__ src = gmem(static_cast<const TS*>(p.src));
   // order-preserving keys (-0 and +0 are one key), one voxel per instruction
   uint32_t a[27];
   int i = 0;
@@ -470,7 +470,7 @@ __device__ __forceinline__ void median_27_body(const MedParams& p, uint32_t bx, 
 #pragma unroll
     for (int dy = -1; dy <= 1; ++dy) {
       const int64_t yy = clamp64(y + dy, 0, p.H - 1);
-      const TS* rowp = src + (zz * p.H + yy) * p.W;
+      const CLK_GLOBAL TS* rowp = src + (zz * p.H + yy) * p.W;
 #pragma unroll
       for (int dx = -1; dx <= 1; ++dx) {
         const int64_t xx = clamp64(x + dx, 0, p.W - 1);
@@ -479,7 +479,7 @@ __device__ __forceinline__ void median_27_body(const MedParams& p, uint32_t bx, 
     }
   }
   const uint32_t m = batcher_median<27, uint32_t>(a, 0xffffffffu);
-  static_cast<TD*>(p.dst)[(z * p.H + y) * p.W + x] = static_cast<TD>(key_value<TD>(m));
+  gmem(static_cast<TD*>(p.dst))[(z * p.H + y) * p.W + x] = static_cast<TD>(key_value<TD>(m));
 }
 
 // 3x3x3 median with a float dst in single launches: a block filters 256
@@ -498,7 +498,7 @@ __device__ __forceinline__ void median_27_key_slide_body(const MedParams& p, uin
   const int64_t y = by;
   const int64_t z0 = (int64_t)bz * kF27ZRun;
   const int64_t z1 = (z0 + kF27ZRun < p.D) ? z0 + kF27ZRun : p.D;
-  const TS* __restrict__ src = static_cast<const TS*>(p.src);
+  const CLK_GLOBAL TS* __restrict__ src = gmem(static_cast<const TS*>(p.src));
   // staged columns x = xt - 1 + c: c = t, and c = 256 + t for t < 2; clamped
   const int64_t xa = clamp64(xt - 1 + t, 0, p.W - 1);
   const int64_t xb = clamp64(xt + 255 + t, 0, p.W - 1);
@@ -509,7 +509,7 @@ __device__ __forceinline__ void median_27_key_slide_body(const MedParams& p, uin
   const int64_t PS = p.H * p.W;
   TS va[3], vb[3];
   auto fetch = [&](int64_t zz) {
-    const TS* pl = src + clamp64(zz, 0, p.D - 1) * PS;
+    const CLK_GLOBAL TS* pl = src + clamp64(zz, 0, p.D - 1) * PS;
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
       va[r] = pl[rows[r] + xa];
@@ -532,7 +532,7 @@ __device__ __forceinline__ void median_27_key_slide_body(const MedParams& p, uin
   put(2);
   __syncthreads();
   const int64_t x = xt + t;
-  TD* __restrict__ dst = static_cast<TD*>(p.dst);
+  CLK_GLOBAL TD* __restrict__ dst = gmem(static_cast<TD*>(p.dst));
   for (int64_t z = z0; z < z1; ++z) {
     const int zi = (int)(z - z0);
     const bool more = z + 1 < z1;
@@ -635,19 +635,19 @@ __device__ __forceinline__ void median_hist8_body(const MedParams& p, uint32_t b
   if (x >= p.W) return;  // lanes never share data: no barrier follows
   const Hist8Lane hl{hist + lane};
   hl.clear();
-  const TS* __restrict__ src = static_cast<const TS*>(p.src);
+  const CLK_GLOBAL TS* __restrict__ src = gmem(static_cast<const TS*>(p.src));
   const int ex = p.ex, ey = p.ey, ez = p.ez;
   const int W = (int)p.W;
   const int xi = (int)x;
   // the whole block's x window inside the row: no clamp, offsets from x
   const bool interior = (int64_t)bx * 64 - ex >= 0 && (int64_t)bx * 64 + 63 + ex < p.W;
   // row pointer of a (z, y) row: uniform across the wave
-  auto rowp = [&](int64_t zz, int64_t yy) -> const TS* {
+  auto rowp = [&](int64_t zz, int64_t yy) -> const CLK_GLOBAL TS* {
     zz = clamp64(zz, 0, p.D - 1);
     yy = clamp64(yy, 0, p.H - 1);
     return src + (zz * p.H + yy) * p.W;
   };
-  auto bin_at = [&](const TS* row, int dx) -> int {
+  auto bin_at = [&](const CLK_GLOBAL TS* row, int dx) -> int {
     int xx = xi + dx;
     if (!interior) xx = xx < 0 ? 0 : (xx >= W ? W - 1 : xx);
     return hist8_bin<TS, TD>(row[xx], p.wrap);
@@ -655,19 +655,19 @@ __device__ __forceinline__ void median_hist8_body(const MedParams& p, uint32_t b
   const int k = p.n / 2;
   for (int dz = -ez; dz <= ez; ++dz)
     for (int dy = -ey; dy <= ey; ++dy) {
-      const TS* row = rowp(z0 + dz, y + dy);
+      const CLK_GLOBAL TS* row = rowp(z0 + dz, y + dy);
       for (int dx = -ex; dx <= ex; ++dx) hl.add(bin_at(row, dx), true);
     }
   int med = 0, lt = 0;
   hl.walk_from_zero(med, lt, k);
-  TD* __restrict__ dst = static_cast<TD*>(p.dst);
+  CLK_GLOBAL TD* __restrict__ dst = gmem(static_cast<TD*>(p.dst));
   const int64_t plane = p.H * p.W;
   dst[z0 * plane + y * p.W + x] = static_cast<TD>(med + static_cast<int>(Pix<TD>::lo));
   for (int64_t z = z0 + 1; z < z1; ++z) {
     const int64_t zo = z - 1 - ez, zi = z + ez;
     for (int dy = -ey; dy <= ey; ++dy) {
-      const TS* ro = rowp(zo, y + dy);
-      const TS* ri = rowp(zi, y + dy);
+      const CLK_GLOBAL TS* ro = rowp(zo, y + dy);
+      const CLK_GLOBAL TS* ri = rowp(zi, y + dy);
       for (int dx = -ex; dx <= ex; ++dx) {
         const int a = bin_at(ro, dx);
         const int b = bin_at(ri, dx);
@@ -711,7 +711,7 @@ __device__ __forceinline__ void median_hist8_fixed_body(const MedParams& p, uint
       return (r - static_cast<int>(Pix<TD>::lo)) << 6;
     else return hist8_bin<TS, TD>(static_cast<TS>(r), p.wrap) << 6;
   };
-  const TS* __restrict__ src = static_cast<const TS*>(p.src);
+  const CLK_GLOBAL TS* __restrict__ src = gmem(static_cast<const TS*>(p.src));
   const int W = (int)p.W;
   const int ez = p.ez;
   const int xi = valid ? (int)x : W - 1;
@@ -730,7 +730,7 @@ __device__ __forceinline__ void median_hist8_fixed_body(const MedParams& p, uint
     Raw halo[NY][EX];
   };
   auto load = [&](int64_t zz, Plane& r) {
-    const TS* pl = src + clamp64(zz, 0, p.D - 1) * p.H * p.W;
+    const CLK_GLOBAL TS* pl = src + clamp64(zz, 0, p.D - 1) * p.H * p.W;
 #pragma unroll
     for (int j = 0; j < NY; ++j) r.own[j] = static_cast<Raw>(pl[rows[j] + xi]);
 #pragma unroll
@@ -768,7 +768,7 @@ __device__ __forceinline__ void median_hist8_fixed_body(const MedParams& p, uint
   }
   int med = 0, lt = 0;
   hl.walk_from_zero(med, lt, k);
-  TD* __restrict__ dst = static_cast<TD*>(p.dst);
+  CLK_GLOBAL TD* __restrict__ dst = gmem(static_cast<TD*>(p.dst));
   const int64_t plane = p.H * p.W;
   if (valid) dst[z0 * plane + y * p.W + x] = static_cast<TD>(med + static_cast<int>(Pix<TD>::lo));
   // step z removes plane z-1-ez and adds plane z+ez, and loads the planes of
@@ -858,7 +858,7 @@ __device__ __forceinline__ void median_hist8_cols_body(const MedParams& p, uint3
     __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(col + 64 * 64 * 4 + ((key >> 12) << 8)),
                            inc ? cone : 0u - cone, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   };
-  const TS* __restrict__ src = static_cast<const TS*>(p.src);
+  const CLK_GLOBAL TS* __restrict__ src = gmem(static_cast<const TS*>(p.src));
   const int ez = p.ez;
   // byte offsets of the lane's pixels in a plane (a plane is under 2^31
   // bytes, checked by the host): each load is a buffer load from the
@@ -872,7 +872,7 @@ __device__ __forceinline__ void median_hist8_cols_body(const MedParams& p, uint3
   // are formed where they are used, so a load in flight is not waited for
   auto load = [&](int zz, Raw (&v)[NY]) {
     zz = zz < 0 ? 0 : (zz > D - 1 ? D - 1 : zz);
-    const __amdgpu_buffer_rsrc_t r = byte_rsrc(reinterpret_cast<const char*>(src) + (int64_t)zz * pbytes, pbytes);
+    const __amdgpu_buffer_rsrc_t r = byte_rsrc(reinterpret_cast<const char*>(gen(src)) + (int64_t)zz * pbytes, pbytes);
 #pragma unroll
     for (int j = 0; j < NY; ++j) v[j] = static_cast<Raw>(rsrc_px<TS>(r, roff[j]));
   };
@@ -913,7 +913,7 @@ __device__ __forceinline__ void median_hist8_cols_body(const MedParams& p, uint3
     return 4 * d + b;
   };
   const int k = p.n / 2;
-  TD* __restrict__ dst = static_cast<TD*>(p.dst);
+  CLK_GLOBAL TD* __restrict__ dst = gmem(static_cast<TD*>(p.dst));
   const int64_t plane = p.H * p.W;
   auto coarse_reads = [&](uint32_t (&cw)[4]) {
 #pragma unroll
@@ -966,7 +966,7 @@ __device__ __forceinline__ void median_hist8_cols_body(const MedParams& p, uint3
       coarse_reads(cw);
       int lt2 = 0;
       const int f = select16(fw, k - lt, lt2);
-      TD* const drow = dst + (int64_t)z * plane + y * p.W;  // uniform
+      CLK_GLOBAL TD* const drow = dst + (int64_t)z * plane + y * p.W;  // uniform
       if (out) drow[(uint32_t)xi] = static_cast<TD>(16 * c + f + static_cast<int>(Pix<TD>::lo));
     }
   }

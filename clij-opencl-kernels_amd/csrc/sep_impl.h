@@ -64,9 +64,9 @@ template <int I> struct IntC { static constexpr int value = I; };
 template <class TS, class VT, int COLS>
 __device__ __forceinline__ void load_vt(const TS* __restrict__ p, VT (&out)[COLS]) {
   if constexpr (COLS == 1) {
-    out[0] = static_cast<VT>(p[0]);
+    out[0] = static_cast<VT>(gmem(p)[0]);
   } else {
-    const PixVec<TS, COLS> v = *reinterpret_cast<const PixVec<TS, COLS>*>(p);
+    const PixVec<TS, COLS> v = gload_vec<TS, COLS>(p);
 #pragma unroll
     for (int i = 0; i < COLS; ++i) out[i] = static_cast<VT>(v.v[i]);
   }
@@ -330,7 +330,7 @@ __device__ __forceinline__ void sep_x_body(const SepParams& p, const uint32_t vb
   const bool valid = task < (uint32_t)p.ntasks;
   const uint32_t row = div_nseg(p, task);
   const int64_t x0 = (int64_t)(task - row * (uint32_t)p.nseg) * TSEG;
-  const TS* __restrict__ rp = static_cast<const TS*>(p.src) + (int64_t)row * W;
+  const CLK_GLOBAL TS* __restrict__ rp = gmem(static_cast<const TS*>(p.src)) + (int64_t)row * W;
 
   if (valid) {
     PixVec<TS, K> v[SUB];
@@ -338,7 +338,7 @@ __device__ __forceinline__ void sep_x_body(const SepParams& p, const uint32_t vb
     for (int m = 0; m < SUB; ++m) {
       const int64_t xi = x0 + kXSeg * m + K * lane;
       if (p.vec_ld && xi + K <= W) {
-        v[m] = load_nt<TS, K>(rp + xi);
+        v[m] = load_nt<TS, K>(gen(rp + xi));
       } else {
 #pragma unroll
         for (int j = 0; j < K; ++j) v[m].v[j] = rp[xi + j < W ? xi + j : W - 1];
@@ -365,12 +365,12 @@ __device__ __forceinline__ void sep_x_body(const SepParams& p, const uint32_t vb
     x_ring<OP, TS, VT>(lb, c, ntaps, p, acc);
     const int64_t xs = x0 + kXSeg * m + (int64_t)K * lane;
     if (xs < W) {
-      TD* __restrict__ dp = static_cast<TD*>(p.dst) + (int64_t)row * W + xs;
+      CLK_GLOBAL TD* __restrict__ dp = gmem(static_cast<TD*>(p.dst)) + (int64_t)row * W + xs;
       TD out[K];
 #pragma unroll
       for (int q = 0; q < K; ++q) out[q] = finish<OP, TS, TD, VT>(acc[q], p);
       if (p.vec_st && xs + K <= W) {
-        store_nt<TD, K>(dp, out);
+        store_nt<TD, K>(gen(dp), out);
       } else {
 #pragma unroll
         for (int q = 0; q < K; ++q)
@@ -406,9 +406,9 @@ __device__ __forceinline__ void sep_xpack_body(const SepParams& p, const uint32_
   const int PW = W + 2 * c;
   const int64_t row0 = (int64_t)task * p.nseg;
   const int k = (int)(p.outer - row0 < p.nseg ? p.outer - row0 : p.nseg);
-  const TS* __restrict__ rp0 = static_cast<const TS*>(p.src) + row0 * W;
+  const CLK_GLOBAL TS* __restrict__ rp0 = gmem(static_cast<const TS*>(p.src)) + row0 * W;
   for (int r = 0; r < k; ++r) {
-    const TS* __restrict__ rp = rp0 + (int64_t)r * W;
+    const CLK_GLOBAL TS* __restrict__ rp = rp0 + (int64_t)r * W;
     for (int e = lane; e < PW; e += 64) {
       int x = e - c;
       x = x < 0 ? 0 : (x >= W ? W - 1 : x);
@@ -424,7 +424,7 @@ __device__ __forceinline__ void sep_xpack_body(const SepParams& p, const uint32_
   const int i0 = c + K * lane;
   int r = i0 / PW;
   int x = i0 - r * PW - c;
-  TD* __restrict__ dp = static_cast<TD*>(p.dst) + row0 * W;
+  CLK_GLOBAL TD* __restrict__ dp = gmem(static_cast<TD*>(p.dst)) + row0 * W;
 #pragma unroll
   for (int q = 0; q < K; ++q) {
     if (x >= W + c) {

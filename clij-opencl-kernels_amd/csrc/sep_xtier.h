@@ -36,7 +36,7 @@ __device__ __forceinline__ void xt_load_chunk(const TS* __restrict__ rp, int x0,
   }
   TS v[8];
 #pragma unroll
-  for (int i = 0; i < 8; ++i) v[i] = rp[x0 + i < W ? x0 + i : W - 1];
+  for (int i = 0; i < 8; ++i) v[i] = gmem(rp)[x0 + i < W ? x0 + i : W - 1];
   if constexpr (sizeof(TS) == 4) {
 #pragma unroll
     for (int i = 0; i < 8; ++i) c.w[i] = __builtin_bit_cast(uint32_t, v[i]);
@@ -177,7 +177,7 @@ __device__ __forceinline__ void xt_compute(const SepParams& p, const XtUnit<TS, 
   } else {
 #pragma unroll
     for (int q = 0; q < 8; ++q)
-      if (x0 + q < W) dp[q] = out[q];
+      if (x0 + q < W) gmem(dp)[q] = out[q];
   }
 }
 
