@@ -159,7 +159,7 @@ int32_t launch_xtier_class(hipStream_t st, int op, int ts, int td, int tier, con
 // CLK_INVALID_VALUE when no kernel exists for the pair (the planner only
 // forms classes that have one: ydyn_pair, and float sources for the
 // Gaussian and the mean).
-int32_t launch_ydyn_class(hipStream_t st, int op, int ts, int td, int tier, int cols, const SepParams* ps,
+int32_t launch_ydyn_class(hipStream_t st, int op, int ts, int td, int tier, int cols, bool fin, const SepParams* ps,
                           const BatchGrid g, uint32_t nblocks, std::string* err);
 
 }  // namespace clk

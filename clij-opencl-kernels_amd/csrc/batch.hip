@@ -97,8 +97,9 @@ bool batch_pair(int ts, int td) { return ts == td || td == CLK_FLOAT || ts == CL
 struct ClassKey {
   int kind, op, ts, td;  // op: separable op, or MedKind for medians
   int tier = 0, cols = 0;  // K_YDYN, K_XTIER, K_XY: half-width tier (K_XY: of y); K_YDYN, K_XY: columns per lane
+  int fin = 0;             // K_YDYN: the branch-free body (Gaussian and mean of finite data)
   bool operator<(const ClassKey& o) const {
-    return std::tie(kind, op, ts, td, tier, cols) < std::tie(o.kind, o.op, o.ts, o.td, o.tier, o.cols);
+    return std::tie(kind, op, ts, td, tier, cols, fin) < std::tie(o.kind, o.op, o.ts, o.td, o.tier, o.cols, o.fin);
   }
 };
 
@@ -251,7 +252,7 @@ int32_t launch_class(hipStream_t st, const ClassKey& k, const ClassWork& w, char
     return launch_xtier_class(st, k.op, k.ts, k.td, k.tier, reinterpret_cast<const SepParams*>(dev + w.params_off),
                               w.grid(dev), (uint32_t)w.nblocks, err);
   if (k.kind == K_YDYN)
-    return launch_ydyn_class(st, k.op, k.ts, k.td, k.tier, k.cols,
+    return launch_ydyn_class(st, k.op, k.ts, k.td, k.tier, k.cols, k.fin != 0,
                              reinterpret_cast<const SepParams*>(dev + w.params_off),
                              w.grid(dev), (uint32_t)w.nblocks, err);
   if (k.kind == K_MEDIAN) {
@@ -282,6 +283,7 @@ struct Pass {
   int dim, N;
   float s;
   int64_t src_tmp, dst_tmp;
+  bool fin;  // the job's source is an integer image: every value finite
 };
 
 // The fused x+y pass of a job (batch_xy.h): src -> temporary of type tt at
@@ -442,7 +444,7 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
         clk_image tmp = jb.dst;
         tmp.dtype = tdt;
         tmp.ptr = nullptr;
-        stage_passes[1].push_back({op, jb.dst, tmp, 2, Ns[2], g ? jb.sigma[2] : 0.f, toff, -1});
+        stage_passes[1].push_back({op, jb.dst, tmp, 2, Ns[2], g ? jb.sigma[2] : 0.f, toff, -1, jb.src.dtype != CLK_FLOAT});
         continue;
       }
     }
@@ -467,14 +469,14 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
         ti ^= 1;
       }
       stage_passes[stage++].push_back({op, out, cur, a, Ns[a], op == OP_GAUSS ? jb.sigma[a] : 0.f,
-                                       cur_tmp, out_tmp});
+                                       cur_tmp, out_tmp, jb.src.dtype != CLK_FLOAT});
       cur = out;
       cur_tmp = out_tmp;
     }
   }
 
   // ---- temporaries, table area ---------------------------------------------
-  const int bseg = tuning(TUNE_BATCH_SEG) > 0 ? tuning(TUNE_BATCH_SEG) : 64;
+  const int bseg = tuning(TUNE_BATCH_SEG) > 0 ? tuning(TUNE_BATCH_SEG) : 32;
   const int xyseg = tuning(TUNE_BATCH_XYSEG) > 0 ? tuning(TUNE_BATCH_XYSEG) : 64;
   std::vector<Single> singles;
   void* temp_base = nullptr;
@@ -597,7 +599,9 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
         const int ycap = tuning(TUNE_BATCH_YCOLS);
         key.cols = ydyn_cols(key.tier, v4 && (ycap == 0 || ycap >= 4), v2 && (ycap == 0 || ycap >= 2));
         const int K = ydyn_k(key.tier, key.cols);
-        if (ps.op == OP_GAUSS) set_tier_weights(p, key.tier);
+        // finite data (chains from integer images): the branch-free body
+        key.fin = ps.fin && (ps.op == OP_GAUSS || ps.op == OP_MEAN) && tuning(TUNE_YDYN_SKIP) == 0;
+        if (ps.op == OP_GAUSS || key.fin) set_ydyn_weights(p, ps.op, key.tier, key.fin != 0);
         p.ncolgroups = p.inner / key.cols;
         p.ncols_total = p.outer * p.ncolgroups;
         const int64_t nblk = (p.ncols_total + 255) / 256;

@@ -7,10 +7,10 @@
 
 namespace clk {
 
-bool launch_ydyn_gauss(hipStream_t, int, int, int, int, const SepParams*, const BatchGrid, uint32_t);
-bool launch_ydyn_mean(hipStream_t, int, int, int, int, const SepParams*, const BatchGrid, uint32_t);
-bool launch_ydyn_min(hipStream_t, int, int, int, int, const SepParams*, const BatchGrid, uint32_t);
-bool launch_ydyn_max(hipStream_t, int, int, int, int, const SepParams*, const BatchGrid, uint32_t);
+bool launch_ydyn_gauss(hipStream_t, bool, int, int, int, int, const SepParams*, const BatchGrid, uint32_t);
+bool launch_ydyn_mean(hipStream_t, bool, int, int, int, int, const SepParams*, const BatchGrid, uint32_t);
+bool launch_ydyn_min(hipStream_t, bool, int, int, int, int, const SepParams*, const BatchGrid, uint32_t);
+bool launch_ydyn_max(hipStream_t, bool, int, int, int, int, const SepParams*, const BatchGrid, uint32_t);
 bool launch_xtier_gauss(hipStream_t, int, int, int, const SepParams*, const BatchGrid, uint32_t);
 bool launch_xtier_mean(hipStream_t, int, int, int, const SepParams*, const BatchGrid, uint32_t);
 bool launch_xtier_min(hipStream_t, int, int, int, const SepParams*, const BatchGrid, uint32_t);
@@ -38,14 +38,14 @@ int32_t launch_xtier_class(hipStream_t st, int op, int ts, int td, int tier, con
   return 0;
 }
 
-int32_t launch_ydyn_class(hipStream_t st, int op, int ts, int td, int tier, int cols, const SepParams* ps,
+int32_t launch_ydyn_class(hipStream_t st, int op, int ts, int td, int tier, int cols, bool fin, const SepParams* ps,
                           const BatchGrid grid, uint32_t nblocks, std::string* err) {
   bool ok = false;
   switch (op) {
-    case OP_GAUSS: ok = launch_ydyn_gauss(st, ts, td, tier, cols, ps, grid, nblocks); break;
-    case OP_MEAN: ok = launch_ydyn_mean(st, ts, td, tier, cols, ps, grid, nblocks); break;
-    case OP_MIN: ok = launch_ydyn_min(st, ts, td, tier, cols, ps, grid, nblocks); break;
-    case OP_MAX: ok = launch_ydyn_max(st, ts, td, tier, cols, ps, grid, nblocks); break;
+    case OP_GAUSS: ok = launch_ydyn_gauss(st, fin, ts, td, tier, cols, ps, grid, nblocks); break;
+    case OP_MEAN: ok = launch_ydyn_mean(st, fin, ts, td, tier, cols, ps, grid, nblocks); break;
+    case OP_MIN: ok = launch_ydyn_min(st, fin, ts, td, tier, cols, ps, grid, nblocks); break;
+    case OP_MAX: ok = launch_ydyn_max(st, fin, ts, td, tier, cols, ps, grid, nblocks); break;
   }
   if (!ok) {
     *err = "internal: no y/z tier class for this pass";

@@ -4,9 +4,9 @@
 
 namespace clk {
 
-bool launch_ydyn_max(hipStream_t st, int ts, int td, int tier, int cols, const SepParams* ps, const BatchGrid grid,
+bool launch_ydyn_max(hipStream_t st, bool fin, int ts, int td, int tier, int cols, const SepParams* ps, const BatchGrid grid,
                     uint32_t n) {
-  return launch_ydyn_op<OP_MAX>(st, ts, td, tier, cols, ps, grid, n);
+  return launch_ydyn_op<OP_MAX>(st, fin, ts, td, tier, cols, ps, grid, n);
 }
 
 }  // namespace clk

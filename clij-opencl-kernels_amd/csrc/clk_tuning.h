@@ -35,7 +35,8 @@ enum TuneKey : int {
   TUNE_NO_XYZMM = 24,        // 1 (default) = batched 16-bit minimum/maximum never run as one 3D pass
   TUNE_BATCH_ZSEG = 25,      // whole-3D batch pass: planes per block (0 = 64)
   TUNE_NO_ZMM = 26,          // 1 (default) = batched 16-bit minimum/maximum z passes keep the y/z ring classes
-  TUNE_COUNT = 27
+  TUNE_YDYN_SKIP = 27,       // 1 = batched Gaussian/mean y/z passes of finite data keep the tap-skipping body
+  TUNE_COUNT = 28
 };
 
 int32_t tuning(TuneKey k);

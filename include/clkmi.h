@@ -346,6 +346,10 @@ int32_t clk_temporaries_held(void* stream, int64_t* stream_bytes, int64_t* pool_
  * "batch_zseg" (whole-3D batch pass: planes per block, 0 = 64), "no_zmm"
  * (1, the default: batched 16-bit minimum/maximum z passes keep the y/z
  * ring classes; 0: they run the register min/max kernel over planes),
+ * "ydyn_skip" (1 = batched Gaussian and mean y/z passes of chains from
+ * integer images skip the taps outside each job's half width by branches,
+ * as float-image jobs do; 0, the default: they run every tap of the tier
+ * with weight 0 outside it, same results, no branches),
  * "sphere_direct" (ellipsoid filters: 0 = auto, 1 = per-voxel
  * gathers, 2 = LDS row tiles without z sliding), and
  * "autotune": 1 = the first single-axis launch of each signature (pixel
