@@ -313,7 +313,8 @@ bool xyzmm_ok(int op, const clk_filter_job& jb, const int (&Ns)[3]) {
   for (int a = 0; a < 3; ++a)
     if (Ns[a] < 1 || (Ns[a] - 1) / 2 > (a == 2 ? kXYZMaxCZ : kXYMMMaxC)) return false;
   const int64_t W = jb.dst.width, H = jb.dst.height, D = jb.dst.depth;
-  if (W <= 0 || H <= 0 || D <= 0 || W * H * D >= (int64_t(1) << 31) - 1) return false;
+  // rows of at least one 8-pixel chunk (chunk16_start)
+  if (W < 8 || H <= 0 || D <= 0 || W * H * D >= (int64_t(1) << 31) - 1) return false;
   const int64_t ng = (W + 7) / 8;
   const int cy = (Ns[1] - 1) / 2;
   return ng <= kXYZThreads && kXYZThreads / ng - 2 * cy >= 4;
@@ -345,7 +346,9 @@ bool xymm_ok(int op, const clk_filter_job& jb, const int (&Ns)[3]) {
   const int cx = (Ns[0] - 1) / 2, cy = (Ns[1] - 1) / 2;
   if (Ns[0] < 1 || Ns[1] < 1 || cx > kXYMMMaxC || cy > kXYMMMaxC) return false;
   const int64_t W = jb.dst.width, H = jb.dst.height, D = jb.dst.depth;
-  return W > 0 && H > 0 && D > 0 && W * H * D < (int64_t(1) << 31) - 1;
+  // rows of at least one 8-pixel chunk (chunk16_start); images under 2^30
+  // bytes (the class's dropped stores use offsets from 2^31)
+  return W >= 8 && H > 0 && D > 0 && W * H * D < (int64_t(1) << 29);
 }
 
 // Columns per thread of a fused x+y pass for a job, or 0 when the job takes
@@ -522,7 +525,7 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
       }
       if (ps.dim == 2 && (ps.op == OP_MIN || ps.op == OP_MAX) && ps.src.dtype == ps.dst.dtype &&
           (ps.src.dtype == CLK_USHORT || ps.src.dtype == CLK_SHORT) && p.c >= 1 && p.c <= kXYMMMaxC &&
-          V < (int64_t(1) << 31) - 1 && tuning(TUNE_NO_ZMM) == 0) {
+          V < (int64_t(1) << 29) && ps.dst.width >= 8 && tuning(TUNE_NO_ZMM) == 0) {
         // 16-bit minimum/maximum along z: the register min/max class walking
         // planes (8-pixel chunks, packed pairs).  Off by default: on the
         // uneven batch it made the call 3.27 -> 3.48 ms against the y/z ring
@@ -547,7 +550,7 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
         q.sz = ps.dst.width;
         ClassWork& w = classes[ClassKey{K_XYMM, ps.op, ps.src.dtype, ps.dst.dtype, p.c <= 5 ? 5 : 8, 0}];
         w.add_params(&q, sizeof(XYParams));
-        const uint64_t waves = (uint64_t)((q.ngt + 61) / 62) * (uint64_t)q.nseg;  // 62 chunks per wave
+        const uint64_t waves = (uint64_t)((q.ngt + 60) / 61) * (uint64_t)q.nseg;  // 61 chunks per wave
         w.add_job((waves + 3) / 4);
         continue;
       }
@@ -717,7 +720,7 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
       if (cols == 0) {
         ClassWork& w = classes[ClassKey{K_XYMM, xp.op, xp.src.dtype, xp.tt, tier, 0}];
         w.add_params(&q, sizeof(XYParams));
-        const uint64_t waves = (uint64_t)((q.ngt + 61) / 62) * (uint64_t)q.nseg;  // 62 chunks per wave
+        const uint64_t waves = (uint64_t)((q.ngt + 60) / 61) * (uint64_t)q.nseg;  // 61 chunks per wave
         w.add_job((waves + 3) / 4);
         continue;
       }

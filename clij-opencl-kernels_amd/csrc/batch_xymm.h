@@ -6,10 +6,10 @@
 //
 // A lane owns one 8-pixel chunk of a plane row (chunks numbered per row,
 // ceil(W/8) per row, plane after plane, as the x tier pass numbers them) and
-// a wave walks one segment of y with 62 chunks; its lanes 0 and 63 hold
+// a wave walks one segment of y with 61 chunks; its lanes 0, 1 and 63 hold
 // the chunks on either side and store nothing.  For every row a lane loads
-// its chunk (one 16-byte vector for a whole chunk, aligned or not; pixel by
-// pixel, clamped, for a row's partial last chunk), takes the neighbouring
+// its chunk (one 16-byte vector, aligned or not; chunk16_start and chunk16_fix), takes the
+// neighbouring
 // chunks from lanes l-1 and l+1 by DPP wave shifts (chunks outside the row
 // are replaced by the row's edge pixel), and computes the 8 x results two at a time on 16-bit pairs
 // (v_pk_min_u16 / v_pk_max_u16; pairs that start at an odd pixel are formed
@@ -50,6 +50,31 @@ __device__ __forceinline__ uint32_t mm_op(uint32_t a, uint32_t b) {
   else return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(x, y));
 }
 
+// The 8 pixels x0 .. x0 + 7 of a row of W >= 8 16-bit pixels, clamped to
+// the row (pixels past its end repeat its last one), as 4 pairs.  Every
+// lane loads one 16-byte vector from chunk16_start(x0, W) = min(x0, W - 8),
+// inside the row; a row's partial last chunk then moves its pixels down by
+// d = x0 - (W - 8) and fills the top with the last pixel (chunk16_fix, in
+// registers, where the row is used).  No load sits in a lane-dependent
+// branch and no loaded value is used at once, so the compiler counts the
+// loads in flight (a branch around or right after loads makes it wait for
+// all of them).
+__device__ __forceinline__ int chunk16_start(int x0, int W) { return x0 + 8 <= W ? x0 : W - 8; }
+
+// sh = 16 * (x0 - chunk16_start(x0, W)) bits, 16 .. 112 for a partial chunk
+__device__ __forceinline__ void chunk16_fix(uint32_t (&c)[4], int sh) {
+  // the chunk as one 128-bit value, shifted down with the last pixel
+  // shifted in (shifts, no indexed registers: a select among words by index
+  // is turned into a private array in scratch memory)
+  typedef unsigned __int128 u128;
+  const uint32_t f = (c[3] >> 16) * 0x10001u;
+  const u128 x = (u128)c[0] | ((u128)c[1] << 32) | ((u128)c[2] << 64) | ((u128)c[3] << 96);
+  const u128 fill = (u128)f * ((((u128)1) << 96) | (((u128)1) << 64) | (((u128)1) << 32) | 1u);
+  const u128 y = (x >> sh) | (fill << (128 - sh));
+#pragma unroll
+  for (int k = 0; k < 4; ++k) c[k] = (uint32_t)(y >> (32 * k));
+}
+
 template <int OP, class TS, int CY>
 __device__ __forceinline__ void sep_xymm_body(const XYParams& p, const uint32_t vblk) {
   static_assert(sizeof(TS) == 2, "16-bit pixels");
@@ -58,46 +83,42 @@ __device__ __forceinline__ void sep_xymm_body(const XYParams& p, const uint32_t 
   constexpr int RS = U * K;
   constexpr uint32_t FLIP = std::is_signed<TS>::value ? 0x80008000u : 0u;
   const int lane = threadIdx.x & 63;
-  const uint32_t nunit = ((uint32_t)p.ngt + 61u) / 62u;
+  const uint32_t nunit = ((uint32_t)p.ngt + 60u) / 61u;
   const uint32_t w = __builtin_amdgcn_readfirstlane(vblk * 4u + (threadIdx.x >> 6));
   if (w >= nunit * (uint32_t)p.nseg) return;  // whole waves (no barriers in this kernel)
   const uint32_t sid = w / nunit;
   const uint32_t unit = w - sid * nunit;
   const int ng = p.ng, W = p.W, H = p.H;
-  // lanes 1..62 own chunks unit*62 + lane - 1; lanes 0 and 63 hold the
-  // chunks on either side (their neighbours' halo) and store nothing
-  const int g = (int)unit * 62 + lane - 1;
-  const bool active = lane >= 1 && lane <= 62 && g < p.ngt;
+  // lanes 2..62 own chunks unit*61 + lane - 2; lanes 0, 1 and 63 hold the
+  // chunks on either side and store nothing (lane 1's results are valid, as
+  // its neighbours' chunks are in the wave: a partial chunk at lane 2 takes
+  // its overlap from them)
+  const int g = (int)unit * 61 + lane - 2;
+  const bool active = lane >= 2 && lane <= 62 && g < p.ngt;
   const int gc = g < 0 ? 0 : (g < p.ngt ? g : p.ngt - 1);
   const int z = gc / ng;
   const int j = gc - z * ng;
   const int x0 = 8 * j;
   const TS* const zs = static_cast<const TS*>(p.src) + (int64_t)z * p.sz;
-  TS* const zd = static_cast<TS*>(p.dst) + (int64_t)z * p.sz + x0;
   const int64_t sy = p.sy;
   const int cx = __builtin_amdgcn_readfirstlane(p.cx);
   const int cy = __builtin_amdgcn_readfirstlane(p.cy);
   const int tlo = CY - cy, thi = CY + cy;
   const int a0 = (int)sid * p.seg;
   const int a1 = (a0 + p.seg < H) ? a0 + p.seg : H;
-  const bool full = x0 + 8 <= W;
 
-  // the lane's chunk of row y (clamped) as raw 16-bit pixel pairs
+  // the lane's chunk of row y (clamped) as raw 16-bit pixel pairs, from xs
+  const int xs = chunk16_start(x0, W);
+  const bool part = xs != x0;
+  const int psh = 16 * (x0 - xs);
   auto load_row = [&](int y, uint32_t (&c)[4]) {
     y = y < 0 ? 0 : (y > H - 1 ? H - 1 : y);
-    const TS* const rp = zs + (int64_t)y * sy;
-    if (full) {
-      load_nt_words_ua<16>(rp + x0, c);
-    } else {
-      uint16_t v[8];
-#pragma unroll
-      for (int i = 0; i < 8; ++i) v[i] = (uint16_t)gmem(rp)[(x0 + i < W) ? x0 + i : W - 1];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) c[i] = (uint32_t)v[2 * i] | ((uint32_t)v[2 * i + 1] << 16);
-    }
+    load_nt_words_ua<16>(zs + (int64_t)y * sy + xs, c);
   };
   // the 8 x results of a loaded row as 4 key pairs
-  auto xrow = [&](const uint32_t (&r)[4], uint32_t (&o)[4]) {
+  auto xrow = [&](const uint32_t (&rl)[4], uint32_t (&o)[4]) {
+    uint32_t r[4] = {rl[0], rl[1], rl[2], rl[3]};
+    if (part) chunk16_fix(r, psh);
     if (cx == 0) {  // a z pass: rows as they are
 #pragma unroll
       for (int i = 0; i < 4; ++i) o[i] = r[i] ^ FLIP;
@@ -154,63 +175,74 @@ __device__ __forceinline__ void sep_xymm_body(const XYParams& p, const uint32_t 
       xrow(r, ring[i]);
     }
 
-  auto store_row = [&](const int y, const uint32_t (&o)[4]) {
-    if (!active) return;
-    TS* const dp = zd + (int64_t)y * sy;
-    uint32_t v[4];
+  // Stores: every lane issues one 16-byte buffer store per output row, at
+  // pixel xs of its chunk, whatever it holds, so no store sits in a branch
+  // (the compiler cannot count stores issued in a lane-dependent branch and
+  // then waits for every load and store in flight).  Offsets with bit 31 set
+  // lie past the image (under 2^30 bytes, checked by the planner) and the
+  // hardware drops those stores: lanes that own no chunk, rows past the
+  // segment.  A row's partial last chunk stores pixels W - 8 .. W - 1: its
+  // own results shifted up by d, below them the left lane's last d results
+  // (lane 1 or an owner), which that lane also stores, with the same values.
+  const __amdgpu_buffer_rsrc_t dr = byte_rsrc(p.dst, 2 * (int64_t)W * H * p.D);
+  const uint32_t lane_off = (uint32_t)(2 * ((int64_t)z * p.sz + xs)) | (active ? 0u : 0x80000000u);
+  auto store_row = [&](const int y, const bool live, const uint32_t (&o)[4]) {
+    uint32_t v[4], l[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) v[i] = o[i] ^ FLIP;
-    if (full) {
-      const u32x4 t = {v[0], v[1], v[2], v[3]};
-      __builtin_nontemporal_store(t, reinterpret_cast<CLK_GLOBAL u32x4_ua*>(gmem(dp)));
-    } else {
-#pragma unroll
-      for (int i = 0; i < 8; ++i)
-        if (x0 + i < W) gmem(dp)[i] = (TS)(uint16_t)(v[i / 2] >> (16 * (i % 2)));
+    for (int i = 0; i < 4; ++i) {
+      v[i] = o[i] ^ FLIP;
+      l[i] = dpp_from_left(v[i], v[i]);
     }
+    if (part) {
+      typedef unsigned __int128 u128;
+      const u128 vv = (u128)v[0] | ((u128)v[1] << 32) | ((u128)v[2] << 64) | ((u128)v[3] << 96);
+      const u128 ll = (u128)l[0] | ((u128)l[1] << 32) | ((u128)l[2] << 64) | ((u128)l[3] << 96);
+      const u128 m = (ll >> (128 - psh)) | (vv << psh);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) v[k] = (uint32_t)(m >> (32 * k));
+    }
+    const uint32_t off = (lane_off + (uint32_t)(2 * (int64_t)y * sy)) | (live ? 0u : 0x80000000u);
+    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+    __builtin_amdgcn_raw_buffer_store_b128(v4u{v[0], v[1], v[2], v[3]}, dr, off, 0, 0);
   };
 
   // Raw rows in flight, two steps of K: at step n buffer n % 2 holds step
   // n + 1's rows (ring rows n*K + K + thi + k, loaded at step n - 2); after
-  // their x filter it takes step n + 3's rows.
+  // their x filter it takes step n + 3's rows.  Every step loads and filters
+  // (rows past the segment are clamped rows, never used), so no load sits in
+  // a branch.
   uint32_t nx[2][K][4];
-  if (a0 + K < a1) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) load_row(a0 - CY + K + thi + k, nx[0][k]);
-  }
-  if (a0 + 2 * K < a1) {
+  for (int k = 0; k < K; ++k) load_row(a0 - CY + K + thi + k, nx[0][k]);
 #pragma unroll
-    for (int k = 0; k < K; ++k) load_row(a0 - CY + 2 * K + thi + k, nx[1][k]);
-  }
+  for (int k = 0; k < K; ++k) load_row(a0 - CY + 2 * K + thi + k, nx[1][k]);
   for (int s = 0;; s += U) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int n = s + u;
       const int p0 = a0 + n * K;
       if (p0 >= a1) return;  // wave-uniform
-      if (p0 + K < a1) {
-        uint32_t xk[K][4];
+      uint32_t xk[K][4];
 #pragma unroll
-        for (int k = 0; k < K; ++k) xrow(nx[u % 2][k], xk[k]);
-        // step n + 3's rows into the buffer just used
-        if (p0 + 3 * K < a1) {
+      for (int k = 0; k < K; ++k) xrow(nx[u % 2][k], xk[k]);
+      // step n + 3's rows into the buffer just used
 #pragma unroll
-          for (int k = 0; k < K; ++k) load_row(a0 - CY + n * K + 3 * K + thi + k, nx[u % 2][k]);
-        }
+      for (int k = 0; k < K; ++k) load_row(a0 - CY + n * K + 3 * K + thi + k, nx[u % 2][k]);
+      // step n + 1's new ring rows (their slots depend on cy: uniform
+      // branches over register moves); the slots they take hold rows older
+      // than any row step n reads (the ring spans 2K + 2cy rows)
 #pragma unroll
-        for (int jj = CY; jj < 2 * CY + K; ++jj) {
+      for (int jj = CY; jj < 2 * CY + K; ++jj) {
 #pragma unroll
-          for (int k = 0; k < K; ++k)
-            if (jj - thi == k) {
+        for (int k = 0; k < K; ++k)
+          if (jj - thi == k) {
 #pragma unroll
-              for (int i = 0; i < 4; ++i) ring[(u * K + K + jj) % RS][i] = xk[k][i];
-            }
-        }
+            for (int i = 0; i < 4; ++i) ring[(u * K + K + jj) % RS][i] = xk[k][i];
+          }
       }
       // y results of rows p0 .. p0 + K - 1
 #pragma unroll
       for (int q = 0; q < K; ++q) {
-        if (p0 + q >= a1) break;
         uint32_t o[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) o[i] = ring[(u * K + q + CY) % RS][i];
@@ -220,7 +252,7 @@ __device__ __forceinline__ void sep_xymm_body(const XYParams& p, const uint32_t 
 #pragma unroll
           for (int i = 0; i < 4; ++i) o[i] = mm_op<OP>(o[i], ring[(u * K + q + t) % RS][i]);
         }
-        store_row(p0 + q, o);
+        store_row(p0 + q, p0 + q < a1, o);
       }
     }
   }

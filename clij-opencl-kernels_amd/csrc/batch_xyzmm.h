@@ -7,8 +7,8 @@
 // over one segment.  Thread t holds chunk j = t % ng (8 pixels) of tile row
 // r = t / ng, R = kXYZThreads / ng rows a tile (its TY = R - 2cy middle rows
 // are outputs, the cy rows on either side are their y halo).  Per plane:
-//   1. every thread loads its chunk (one 16-byte vector, aligned or not; a
-//      row's partial last chunk pixel by pixel, clamped) and writes it to
+//   1. every thread loads its chunk (one 16-byte vector, aligned or not,
+//      from chunk16_start; chunk16_fix where it is used) and writes it to
 //      LDS;
 //   2. it reads the chunks on either side (or repeats the row's edge pixel)
 //      and takes the 8 x results two at a time on 16-bit pairs
@@ -62,6 +62,9 @@ __device__ __forceinline__ void sep_xyzmm_body(const MMParams& p, const uint32_t
   const bool outrow = live && r >= cy && r < R - cy && yr < H;
   const int x0 = 8 * j;
   const bool full = x0 + 8 <= W;
+  const int xs = chunk16_start(x0, W);  // the thread's loads start here
+  const bool part = xs != x0;
+  const int psh = 16 * (x0 - xs);
   const TS* const rs = static_cast<const TS*>(p.src) + (int64_t)yc * W;
   TS* const rd = static_cast<TS*>(p.dst) + (int64_t)yc * W + x0;
   const int64_t plane = (int64_t)H * W;
@@ -75,20 +78,15 @@ __device__ __forceinline__ void sep_xyzmm_body(const MMParams& p, const uint32_t
     if (!live) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) c[i] = 0u;
-    } else if (full) {
-      load_nt_words_ua<16>(rp + x0, c);
     } else {
-      uint16_t v[8];
-#pragma unroll
-      for (int i = 0; i < 8; ++i) v[i] = (uint16_t)gmem(rp)[(x0 + i < W) ? x0 + i : W - 1];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) c[i] = (uint32_t)v[2 * i] | ((uint32_t)v[2 * i + 1] << 16);
+      load_nt_words_ua<16>(rp + xs, c);
     }
   };
   // xy results of the plane whose raw chunk is c[0]; c[1], c[2] hold the
   // next two planes' and plane zload is loaded behind them (two barriers;
   // every thread of the block calls it)
   auto xyplane = [&](uint32_t (&c)[3][4], const int zload, uint32_t (&o)[4]) {
+    if (part) chunk16_fix(c[0], psh);
     raw[t] = u32x4{c[0][0] ^ FLIP, c[0][1] ^ FLIP, c[0][2] ^ FLIP, c[0][3] ^ FLIP};
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
