@@ -199,3 +199,28 @@ def test_xyzmm_whole_3d_class(gpu, seed):
             tag = "job %d %s %s %s r=%s %s" % (i, job[0], job[1].shape, job[1].dtype, job[4], knobs)
             assert same_bits_or_value(w, s_), tag + " whole-3D != separate"
             assert same_bits_or_value(w, oracle_job(*job)), tag + " whole-3D != oracle"
+
+
+def test_xymm_partial_chunks_and_lane_layout(gpu):
+    """The register min/max class's partial last chunks: rows of 491 pixels
+    (62 chunks, so every 61st row ends at the first owning lane of a wave
+    and takes its overlap from the halo lane before it), rows of 8..15
+    pixels (one whole and one partial chunk), exactly 8 pixels, and rows
+    under 8 pixels (separate passes); segments shorter than the volume so
+    that rows past a segment are dropped stores.  Same bits as the separate
+    passes and the oracle."""
+    jobs = []
+    shapes = [(2, 70, 491), (3, 130, 491), (4, 9, 9), (2, 17, 15), (5, 6, 8), (3, 11, 5), (2, 64, 497)]
+    for i, shape in enumerate(shapes):
+        for kind in ("minimum_box_3d", "maximum_box_3d"):
+            st = (np.uint16, np.int16)[i % 2]
+            src = random_volume(shape, st, seed=700 + 2 * i + (kind == "maximum_box_3d"))
+            radius = (1 + i % 8, 1 + (3 * i) % 8, 1 + i % 5)
+            jobs.append((kind, src, st, (0.0, 0.0, 0.0), radius))
+    separate = run_batch(jobs, no_xy=1, no_xymm=1, no_xyg=1, no_xyzmm=1, no_zmm=1)
+    for knobs in ({"batch_xyseg": 8}, {"batch_xyseg": 1000}):
+        reg = run_batch(jobs, no_xy=1, no_xymm=0, no_xyg=1, no_xyzmm=1, **knobs)
+        for i, (job, r, s_) in enumerate(zip(jobs, reg, separate)):
+            tag = "job %d %s %s r=%s %s" % (i, job[0], job[1].shape, job[4], knobs)
+            assert same_bits_or_value(r, s_), tag + " register != separate"
+            assert same_bits_or_value(r, oracle_job(*job)), tag + " != oracle"
