@@ -606,6 +606,13 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
         key.fin = ps.fin && (ps.op == OP_GAUSS || ps.op == OP_MEAN) && tuning(TUNE_YDYN_SKIP) == 0;
         if (ps.op == OP_GAUSS || key.fin) set_ydyn_weights(p, ps.op, key.tier, key.fin != 0);
         p.ncolgroups = p.inner / key.cols;
+        p.ntasks = 0;
+        if (tuning(TUNE_BATCH_YPAD) != 0 && p.outer > 1 && p.ncolgroups % 64 != 0) {
+          // y passes: each row's column groups padded to whole waves, so no
+          // wave spans the end of one plane's row and the start of the next
+          p.ntasks = p.ncolgroups;
+          p.ncolgroups = (p.ncolgroups + 63) / 64 * 64;
+        }
         p.ncols_total = p.outer * p.ncolgroups;
         const int64_t nblk = (p.ncols_total + 255) / 256;
         int64_t seg = ((int64_t)bseg + K - 1) / K * K;

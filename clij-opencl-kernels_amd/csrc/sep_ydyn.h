@@ -67,6 +67,9 @@ __device__ __forceinline__ void sep_ydyn_body(const SepParams& p, const uint32_t
   if (g >= p.ncols_total) return;  // no barriers below
   const int64_t o = g / p.ncolgroups;
   const int64_t cg = g - o * p.ncolgroups;
+  // rows padded to whole waves (p.ntasks = the real column groups): the
+  // padding lanes have nothing to do (no barriers below)
+  if (p.ntasks > 0 && cg >= p.ntasks) return;
   const int64_t eoff = o * (p.L * p.inner) + cg * COLS;
   const char* const ls = static_cast<const char*>(p.src) + eoff * (int64_t)sizeof(TS);
   char* const ld = static_cast<char*>(p.dst) + eoff * (int64_t)sizeof(TD);

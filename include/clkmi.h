@@ -349,7 +349,10 @@ int32_t clk_temporaries_held(void* stream, int64_t* stream_bytes, int64_t* pool_
  * "ydyn_skip" (1 = batched Gaussian and mean y/z passes of chains from
  * integer images skip the taps outside each job's half width by branches,
  * as float-image jobs do; 0, the default: they run every tap of the tier
- * with weight 0 outside it, same results, no branches),
+ * with weight 0 outside it, same results, no branches), "batch_ypad"
+ * (1, the default: batched y passes pad each row's columns to whole
+ * waves, so no wave spans the end of one plane's row and the start of the
+ * next; 0: columns packed),
  * "sphere_direct" (ellipsoid filters: 0 = auto, 1 = per-voxel
  * gathers, 2 = LDS row tiles without z sliding), and
  * "autotune": 1 = the first single-axis launch of each signature (pixel
