@@ -296,6 +296,7 @@ struct XYPass {
   int tt;
   int64_t dst_tmp;
   int cols;  // K_XY: columns per thread; 0: K_XYMM; -1: K_XYG (registers and LDS, 8-pixel chunks)
+  bool zaxis = false;  // K_XYMM: the fused axis is z (Ny is z's; the job's y pass follows)
 };
 
 // Minimum/maximum of 16-bit integer pixels with half widths up to
@@ -337,14 +338,17 @@ bool xyg_ok(int op, const clk_filter_job& jb, const int (&Ns)[3]) {
   return W > 0 && H > 0 && D > 0 && W * H * D < (int64_t(1) << 31) - 1;
 }
 
-// Minimum/maximum of 16-bit integer pixels with both half widths up to
-// kXYMMMaxC take the register-only fused class (batch_xymm.h).
-bool xymm_ok(int op, const clk_filter_job& jb, const int (&Ns)[3]) {
+// Minimum/maximum of 16-bit integer pixels with the x half width and that
+// of the fused axis (y, or z with xymm_z) up to kXYMMMaxC take the
+// register-only fused class (batch_xymm.h).  Minimum and maximum of
+// integers are exact in any order, so x with z, then y, gives the bits of
+// x, y, z.
+bool xymm_ok(int op, const clk_filter_job& jb, const int (&Ns)[3], int axis) {
   if (tuning(TUNE_NO_XYMM) != 0) return false;
   if (op != OP_MIN && op != OP_MAX) return false;
   if (jb.src.dtype != CLK_USHORT && jb.src.dtype != CLK_SHORT) return false;
-  const int cx = (Ns[0] - 1) / 2, cy = (Ns[1] - 1) / 2;
-  if (Ns[0] < 1 || Ns[1] < 1 || cx > kXYMMMaxC || cy > kXYMMMaxC) return false;
+  const int cx = (Ns[0] - 1) / 2, cy = (Ns[axis] - 1) / 2;
+  if (Ns[0] < 1 || Ns[axis] < 1 || cx > kXYMMMaxC || cy > kXYMMMaxC) return false;
   const int64_t W = jb.dst.width, H = jb.dst.height, D = jb.dst.depth;
   // rows of at least one 8-pixel chunk (chunk16_start); images under 2^30
   // bytes (the class's dropped stores use offsets from 2^31)
@@ -434,20 +438,25 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
       continue;
     }
     if (nact == 3) {
-      const bool mm = xymm_ok(op, jb, Ns);
+      // 16-bit min/max: x with z (a wave's chunks in consecutive rows of one
+      // plane, contiguous) when xymm_z asks for it, else x with y
+      const bool mmz = tuning(TUNE_XYMM_Z) != 0 && xymm_ok(op, jb, Ns, 2);
+      const bool mm = mmz || xymm_ok(op, jb, Ns, 1);
       const bool xg = !mm && xyg_ok(op, jb, Ns);
       const int cols = mm ? 0 : (xg ? -1 : xy_cols(op, jb, Ns));
       if (mm || cols != 0) {
-        // x and y in one pass into one temporary, then the z pass
+        // x and y (or z) in one pass into one temporary, then the last pass
         const int64_t toff = temp_bytes;
         temp_bytes += (int64_t)align256((size_t)(V * clk_dtype_size(tdt)));
         const bool g = op == OP_GAUSS;
-        xy_passes.push_back({op, jb.dst, jb.src, Ns[0], Ns[1], g ? jb.sigma[0] : 0.f, g ? jb.sigma[1] : 0.f,
-                             tdt, toff, cols});
+        const int fa = mmz ? 2 : 1, la = mmz ? 1 : 2;  // fused axis, last axis
+        XYPass xp{op, jb.dst, jb.src, Ns[0], Ns[fa], g ? jb.sigma[0] : 0.f, g ? jb.sigma[fa] : 0.f, tdt, toff, cols};
+        xp.zaxis = mmz;
+        xy_passes.push_back(xp);
         clk_image tmp = jb.dst;
         tmp.dtype = tdt;
         tmp.ptr = nullptr;
-        stage_passes[1].push_back({op, jb.dst, tmp, 2, Ns[2], g ? jb.sigma[2] : 0.f, toff, -1, jb.src.dtype != CLK_FLOAT});
+        stage_passes[1].push_back({op, jb.dst, tmp, la, Ns[la], g ? jb.sigma[la] : 0.f, toff, -1, jb.src.dtype != CLK_FLOAT});
         continue;
       }
     }
@@ -676,7 +685,7 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
       int ax = 0;
       bool nf = false;
       int32_t rc = prepare_separable(xp.op, tmp, xp.src, 0, xp.Nx, xp.sx, px, &ax, &nf, err);
-      if (rc == 0 && !nf) rc = prepare_separable(xp.op, tmp, tmp, 1, xp.Ny, xp.sy, py, &ax, &nf, err);
+      if (rc == 0 && !nf) rc = prepare_separable(xp.op, tmp, tmp, xp.zaxis ? 2 : 1, xp.Ny, xp.sy, py, &ax, &nf, err);
       if (rc) return rc;
       if (nf) {
         *err = "internal: fused x+y pass planned for an empty tap window";
@@ -690,8 +699,8 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
       q.src = xp.src.ptr;
       q.dst = tmp.ptr;
       q.W = (int32_t)xp.dst.width;
-      q.H = (int32_t)xp.dst.height;
-      q.D = (int32_t)xp.dst.depth;
+      q.H = (int32_t)(xp.zaxis ? xp.dst.depth : xp.dst.height);  // the walk
+      q.D = (int32_t)(xp.zaxis ? xp.dst.height : xp.dst.depth);  // its outer slices
       q.cx = px.c;
       q.cy = py.c;
       q.ng = (int32_t)((q.W + (cols > 0 ? cols : 8) - 1) / (cols > 0 ? cols : 8));
@@ -702,8 +711,8 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
       if (seg > hk) seg = hk;
       q.seg = seg;
       q.nseg = (q.H + seg - 1) / seg;
-      q.sy = q.W;
-      q.sz = (int64_t)q.W * q.H;
+      q.sy = xp.zaxis ? (int64_t)q.W * q.D : q.W;  // elements between rows of the walk
+      q.sz = xp.zaxis ? q.W : (int64_t)q.W * q.H;  // and between its outer slices
       const int ds = clk_dtype_size(xp.tt);
       q.vec_st = cols > 0 && (q.W % cols == 0) && ((uintptr_t)tmp.ptr % (size_t)(cols * ds) == 0);
       q.divx = px.div;

@@ -352,7 +352,9 @@ int32_t clk_temporaries_held(void* stream, int64_t* stream_bytes, int64_t* pool_
  * with weight 0 outside it, same results, no branches), "batch_ypad"
  * (1, the default: batched y passes pad each row's columns to whole
  * waves, so no wave spans the end of one plane's row and the start of the
- * next; 0: columns packed),
+ * next; 0: columns packed), "xymm_z" (1, the default: batched 16-bit
+ * minimum/maximum fuse x with z, a wave's chunks in consecutive rows of
+ * one plane, then run y; 0: x with y, then z; same results),
  * "sphere_direct" (ellipsoid filters: 0 = auto, 1 = per-voxel
  * gathers, 2 = LDS row tiles without z sliding), and
  * "autotune": 1 = the first single-axis launch of each signature (pixel

@@ -143,8 +143,8 @@ def test_xy_float_specials(gpu):
 @pytest.mark.parametrize("seed", [5, 6])
 def test_xymm_register_class(gpu, seed):
     """Minimum/maximum of uint16/int16 with x and y radii 1..8 (the register
-    class; with the source type out its z pass runs the same kernel over
-    planes) and 9 (just past it), on rows with partial last chunks, waves
+    class, fusing x with y or, by default, with z; with the source type out
+    its z pass runs the same kernel over planes) and 9 (just past it), on rows with partial last chunks, waves
     spanning many planes, the type extremes (key flip of int16), heights
     below one step and over several segments: equal to separate passes and
     the oracle."""
@@ -161,12 +161,14 @@ def test_xymm_register_class(gpu, seed):
         kind = ("minimum_box_3d", "maximum_box_3d")[(i // 2) % 2]
         radius = (int(rng.integers(1, 10)), int(rng.integers(1, 10)), int(rng.integers(1, 4)))
         jobs.append((kind, src, dt, (0.0, 0.0, 0.0), radius))
-    fused = run_batch(jobs, no_xymm=0, no_xyzmm=1)
     separate = run_batch(jobs, no_xymm=1, no_xy=1, no_xyg=1, no_xyzmm=1, no_zmm=1)
-    for i, (job, f, s_) in enumerate(zip(jobs, fused, separate)):
-        tag = "job %d %s %s %s->%s r=%s" % (i, job[0], job[1].shape, job[1].dtype, np.dtype(job[2]), job[4])
-        assert same_bits_or_value(f, s_), tag + " fused != separate"
-        assert same_bits_or_value(f, oracle_job(*job)), tag + " fused != oracle"
+    for order in (0, 1):  # x with y, then z; x with z, then y
+        fused = run_batch(jobs, no_xymm=0, no_xyzmm=1, xymm_z=order)
+        for i, (job, f, s_) in enumerate(zip(jobs, fused, separate)):
+            tag = "job %d %s %s %s->%s r=%s xymm_z=%d" % (i, job[0], job[1].shape, job[1].dtype, np.dtype(job[2]),
+                                                         job[4], order)
+            assert same_bits_or_value(f, s_), tag + " fused != separate"
+            assert same_bits_or_value(f, oracle_job(*job)), tag + " fused != oracle"
 
 
 @pytest.mark.parametrize("seed", [7, 8])
@@ -218,7 +220,8 @@ def test_xymm_partial_chunks_and_lane_layout(gpu):
             radius = (1 + i % 8, 1 + (3 * i) % 8, 1 + i % 5)
             jobs.append((kind, src, st, (0.0, 0.0, 0.0), radius))
     separate = run_batch(jobs, no_xy=1, no_xymm=1, no_xyg=1, no_xyzmm=1, no_zmm=1)
-    for knobs in ({"batch_xyseg": 8}, {"batch_xyseg": 1000}):
+    for knobs in ({"batch_xyseg": 8, "xymm_z": 0}, {"batch_xyseg": 1000, "xymm_z": 0}, {"batch_xyseg": 8, "xymm_z": 1},
+                  {"batch_xyseg": 1000, "xymm_z": 1}):
         reg = run_batch(jobs, no_xy=1, no_xymm=0, no_xyg=1, no_xyzmm=1, **knobs)
         for i, (job, r, s_) in enumerate(zip(jobs, reg, separate)):
             tag = "job %d %s %s r=%s %s" % (i, job[0], job[1].shape, job[4], knobs)
