@@ -339,7 +339,13 @@ __device__ __forceinline__ void median_27_pk8_slide_body(const MedParams& p, uin
     }
   };
   auto put = [&](int slot) {
-    auto v = [&](TS x) -> uint32_t { return (uint32_t)((int)gather_value<TS, TD>(x, p.wrap) + kBias); };
+    // the value as the reference stores it, biased to an unsigned key; for
+    // an integer source of the dst type that value is the pixel itself
+    // (both of gather_value's casts are the identity): no float round trip
+    auto v = [&](TS x) -> uint32_t {
+      if constexpr (std::is_same<TS, TD>::value && std::is_integral<TS>::value) return (uint32_t)((int)x + kBias);
+      else return (uint32_t)((int)gather_value<TS, TD>(x, p.wrap) + kBias);
+    };
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
       const uint32_t v0 = v(raw[r][0]), v1 = v(raw[r][1]), v2 = v(raw[r][2]);
@@ -416,7 +422,13 @@ __device__ __forceinline__ void median_9_pk8_slide_body(const MedParams& p, uint
     raw[3] = t == 255 ? row[xc[3]] : TS(0);
   };
   auto put = [&](int slot) {
-    auto v = [&](TS x) -> uint32_t { return (uint32_t)((int)gather_value<TS, TD>(x, p.wrap) + kBias); };
+    // the value as the reference stores it, biased to an unsigned key; for
+    // an integer source of the dst type that value is the pixel itself
+    // (both of gather_value's casts are the identity): no float round trip
+    auto v = [&](TS x) -> uint32_t {
+      if constexpr (std::is_same<TS, TD>::value && std::is_integral<TS>::value) return (uint32_t)((int)x + kBias);
+      else return (uint32_t)((int)gather_value<TS, TD>(x, p.wrap) + kBias);
+    };
     const uint32_t v0 = v(raw[0]), v1 = v(raw[1]), v2 = v(raw[2]);
     pe[slot][t] = v0 | (v1 << 16);
     po[slot][t] = v1 | (v2 << 16);
