@@ -567,7 +567,7 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
       uint32_t blocks = 0;
       size_t lds = 0;
       const int64_t per_row = (p.L + 7) / 8;
-      if (axis_is_x && xtier_tier(p.c) > 0 && xtier_pair(ps.src.dtype, ps.dst.dtype) &&
+      if (axis_is_x && xtier_tier(p.c) > 0 && xtier_pair(ps.src.dtype, ps.dst.dtype) && p.L >= 8 &&
           p.outer * per_row < (int64_t(1) << 31) - 64 && tuning(TUNE_NO_XTIER) == 0) {
         // register windows over per-row 8-pixel chunks, one class per tier
         // whatever each job's row length and c
@@ -579,7 +579,8 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
         p.vec_ld = ((p.L * ss) % 16 == 0) && ((uintptr_t)p.src % 16 == 0);
         p.vec_st = ((p.L * ds) % 16 == 0) && ((uintptr_t)p.dst % 16 == 0);
         if (ps.op == OP_GAUSS) set_tier_weights(p, key.tier);
-        const int64_t units = (p.ntasks + 63) / 64;
+        const int owned = 64 - 2 * ((key.tier + 7) / 8);  // chunks a wave owns (xt_owned)
+        const int64_t units = (p.ntasks + owned - 1) / owned;
         const int64_t waves = (units + kXTierUnits - 1) / kXTierUnits;
         blocks = (uint32_t)((waves + 3) / 4);
       } else if (axis_is_x && p.L <= 256 && p.c <= 256 && tuning(TUNE_NO_XPACK) == 0) {
