@@ -532,20 +532,22 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
         singles.push_back({s, ps});
         continue;
       }
-      if (ps.dim == 2 && (ps.op == OP_MIN || ps.op == OP_MAX) && ps.src.dtype == ps.dst.dtype &&
+      if (((ps.dim == 2 && tuning(TUNE_NO_ZMM) == 0) || (ps.dim == 1 && tuning(TUNE_NO_YMM) == 0)) &&
+          (ps.op == OP_MIN || ps.op == OP_MAX) && ps.src.dtype == ps.dst.dtype &&
           (ps.src.dtype == CLK_USHORT || ps.src.dtype == CLK_SHORT) && p.c >= 1 && p.c <= kXYMMMaxC &&
-          V < (int64_t(1) << 29) && ps.dst.width >= 8 && tuning(TUNE_NO_ZMM) == 0) {
-        // 16-bit minimum/maximum along z: the register min/max class walking
-        // planes (8-pixel chunks, packed pairs).  Off by default: on the
-        // uneven batch it made the call 3.27 -> 3.48 ms against the y/z ring
-        // classes.
+          V < (int64_t(1) << 29) && ps.dst.width >= 8) {
+        // 16-bit minimum/maximum along z (or y): the register min/max class
+        // walking planes (rows), 8-pixel chunks, packed pairs, no x filter.
+        // Off by default: on the uneven batch it made the call 3.27 -> 3.48
+        // ms against the y/z ring classes as a z pass.
+        const bool zw = ps.dim == 2;
         XYParams q;
         memset(&q, 0, sizeof(q));
         q.src = ps.src.ptr;
         q.dst = ps.dst.ptr;
         q.W = (int32_t)ps.dst.width;
-        q.H = (int32_t)ps.dst.depth;   // the walk
-        q.D = (int32_t)ps.dst.height;  // its outer slices
+        q.H = (int32_t)(zw ? ps.dst.depth : ps.dst.height);  // the walk
+        q.D = (int32_t)(zw ? ps.dst.height : ps.dst.depth);  // its outer slices
         q.cx = 0;
         q.cy = p.c;
         q.ng = (q.W + 7) / 8;
@@ -555,8 +557,8 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
         if (zseg > hk) zseg = hk;
         q.seg = zseg;
         q.nseg = (q.H + zseg - 1) / zseg;
-        q.sy = ps.dst.width * ps.dst.height;
-        q.sz = ps.dst.width;
+        q.sy = zw ? ps.dst.width * ps.dst.height : ps.dst.width;
+        q.sz = zw ? ps.dst.width : ps.dst.width * ps.dst.height;
         ClassWork& w = classes[ClassKey{K_XYMM, ps.op, ps.src.dtype, ps.dst.dtype, p.c <= 5 ? 5 : 8, 0}];
         w.add_params(&q, sizeof(XYParams));
         const uint64_t waves = (uint64_t)((q.ngt + 60) / 61) * (uint64_t)q.nseg;  // 61 chunks per wave

@@ -38,7 +38,8 @@ enum TuneKey : int {
   TUNE_YDYN_SKIP = 27,       // 1 = batched Gaussian/mean y/z passes of finite data keep the tap-skipping body
   TUNE_BATCH_YPAD = 28,      // 1 (default) = batched y passes pad each row's columns to whole waves
   TUNE_XYMM_Z = 29,          // 1 (default) = batched 16-bit min/max fuse x with z (then y) instead of x with y
-  TUNE_COUNT = 30
+  TUNE_NO_YMM = 30,          // 1 (default) = batched 16-bit min/max y passes keep the y/z ring classes
+  TUNE_COUNT = 31
 };
 
 int32_t tuning(TuneKey k);

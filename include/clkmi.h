@@ -354,7 +354,9 @@ int32_t clk_temporaries_held(void* stream, int64_t* stream_bytes, int64_t* pool_
  * waves, so no wave spans the end of one plane's row and the start of the
  * next; 0: columns packed), "xymm_z" (1, the default: batched 16-bit
  * minimum/maximum fuse x with z, a wave's chunks in consecutive rows of
- * one plane, then run y; 0: x with y, then z; same results),
+ * one plane, then run y; 0: x with y, then z; same results), "no_ymm"
+ * (1, the default: batched 16-bit minimum/maximum y passes keep the y/z
+ * ring classes; 0: they run the register min/max kernel over rows),
  * "sphere_direct" (ellipsoid filters: 0 = auto, 1 = per-voxel
  * gathers, 2 = LDS row tiles without z sliding), and
  * "autotune": 1 = the first single-axis launch of each signature (pixel
