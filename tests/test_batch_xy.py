@@ -143,8 +143,9 @@ def test_xy_float_specials(gpu):
 @pytest.mark.parametrize("seed", [5, 6])
 def test_xymm_register_class(gpu, seed):
     """Minimum/maximum of uint16/int16 with x and y radii 1..8 (the register
-    class, fusing x with y or, by default, with z; with the source type out
-    its z pass runs the same kernel over planes) and 9 (just past it), on rows with partial last chunks, waves
+    class, fusing x with y or, by default, with z; the last pass in the y/z
+    ring classes or in the same kernel without an x filter) and 9 (just past
+    it), on rows with partial last chunks, waves
     spanning many planes, the type extremes (key flip of int16), heights
     below one step and over several segments: equal to separate passes and
     the oracle."""
@@ -162,11 +163,13 @@ def test_xymm_register_class(gpu, seed):
         radius = (int(rng.integers(1, 10)), int(rng.integers(1, 10)), int(rng.integers(1, 4)))
         jobs.append((kind, src, dt, (0.0, 0.0, 0.0), radius))
     separate = run_batch(jobs, no_xymm=1, no_xy=1, no_xyg=1, no_xyzmm=1, no_zmm=1)
-    for order in (0, 1):  # x with y, then z; x with z, then y
-        fused = run_batch(jobs, no_xymm=0, no_xyzmm=1, xymm_z=order)
+    # x with y, then z; x with z, then y; each with its last pass in the
+    # y/z ring classes or in the register kernel (no x filter)
+    for knobs in ({"xymm_z": 0}, {"xymm_z": 1}, {"xymm_z": 0, "no_zmm": 0}, {"xymm_z": 1, "no_ymm": 0}):
+        fused = run_batch(jobs, no_xymm=0, no_xyzmm=1, **knobs)
         for i, (job, f, s_) in enumerate(zip(jobs, fused, separate)):
-            tag = "job %d %s %s %s->%s r=%s xymm_z=%d" % (i, job[0], job[1].shape, job[1].dtype, np.dtype(job[2]),
-                                                         job[4], order)
+            tag = "job %d %s %s %s->%s r=%s %s" % (i, job[0], job[1].shape, job[1].dtype, np.dtype(job[2]),
+                                                  job[4], knobs)
             assert same_bits_or_value(f, s_), tag + " fused != separate"
             assert same_bits_or_value(f, oracle_job(*job)), tag + " fused != oracle"
 
