@@ -619,11 +619,14 @@ int32_t run_batch(hipStream_t st, const clk_filter_job* jobs, int32_t njobs, std
         if (ps.op == OP_GAUSS || key.fin) set_ydyn_weights(p, ps.op, key.tier, key.fin != 0);
         p.ncolgroups = p.inner / key.cols;
         p.ntasks = 0;
-        if (tuning(TUNE_BATCH_YPAD) != 0 && p.outer > 1 && p.ncolgroups % 64 != 0) {
-          // y passes: each row's column groups padded to whole waves, so no
-          // wave spans the end of one plane's row and the start of the next
+        const int ypad = tuning(TUNE_BATCH_YPAD);  // lanes: 0 (packed), 16, 32 or 64
+        if (ypad > 0 && p.outer > 1 && p.ncolgroups % ypad != 0) {
+          // y passes: each row's column groups padded to a whole number of
+          // ypad lanes, so a wave's loads hold at most 64 / ypad row pieces,
+          // each starting its row (packed, a wave's columns ran from the end
+          // of one plane's row into the next plane's at any lane)
           p.ntasks = p.ncolgroups;
-          p.ncolgroups = (p.ncolgroups + 63) / 64 * 64;
+          p.ncolgroups = (p.ncolgroups + ypad - 1) / ypad * ypad;
         }
         p.ncols_total = p.outer * p.ncolgroups;
         const int64_t nblk = (p.ncols_total + 255) / 256;

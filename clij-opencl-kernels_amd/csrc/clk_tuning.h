@@ -36,7 +36,7 @@ enum TuneKey : int {
   TUNE_BATCH_ZSEG = 25,      // whole-3D batch pass: planes per block (0 = 64)
   TUNE_NO_ZMM = 26,          // 1 (default) = batched 16-bit minimum/maximum z passes keep the y/z ring classes
   TUNE_YDYN_SKIP = 27,       // 1 = batched Gaussian/mean y/z passes of finite data keep the tap-skipping body
-  TUNE_BATCH_YPAD = 28,      // 1 (default) = batched y passes pad each row's columns to whole waves
+  TUNE_BATCH_YPAD = 28,      // batched y passes pad each row's columns to 16, 32 or 64 lanes (0 = packed)
   TUNE_XYMM_Z = 29,          // 1 (default) = batched 16-bit min/max fuse x with z (then y) instead of x with y
   TUNE_NO_YMM = 30,          // 1 (default) = batched 16-bit min/max y passes keep the y/z ring classes
   TUNE_COUNT = 31

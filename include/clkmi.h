@@ -350,9 +350,9 @@ int32_t clk_temporaries_held(void* stream, int64_t* stream_bytes, int64_t* pool_
  * integer images skip the taps outside each job's half width by branches,
  * as float-image jobs do; 0, the default: they run every tap of the tier
  * with weight 0 outside it, same results, no branches), "batch_ypad"
- * (1, the default: batched y passes pad each row's columns to whole
- * waves, so no wave spans the end of one plane's row and the start of the
- * next; 0: columns packed), "xymm_z" (1, the default: batched 16-bit
+ * (batched y passes pad each row's columns to 16, 32 or 64 lanes, the
+ * default 64: whole waves, so no wave spans the end of one plane's row and
+ * the start of the next; 0: columns packed), "xymm_z" (1, the default: batched 16-bit
  * minimum/maximum fuse x with z, a wave's chunks in consecutive rows of
  * one plane, then run y; 0: x with y, then z; same results), "no_ymm"
  * (1, the default: batched 16-bit minimum/maximum y passes keep the y/z

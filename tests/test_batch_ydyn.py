@@ -6,7 +6,7 @@ from integer images (every tap of the tier, weight 0 outside the job's half
 width; their float temporaries are finite) and the tap-skipping one for
 float images (which may hold NaN and infinities) and for minimum/maximum.
 Every job must give the same bits from both bodies, at 1, 2 and 4 columns
-per lane, with y-pass rows padded to whole waves or packed, over segments
+per lane, with y-pass rows padded to whole waves, to 16 lanes or packed, over segments
 shorter than one step and several per column, and the oracle chain's
 values.  Half widths cover 0..16 on y and z (tiers 5, 8
 and 16, every c inside each)."""
@@ -97,6 +97,7 @@ def check(jobs):
         for ycols in (0, 1):
             legs[(skip, ycols)] = run_batch(jobs, ydyn_skip=skip, batch_ycols=ycols, batch_seg=16)
     legs["unpadded"] = run_batch(jobs, batch_ypad=0, batch_seg=16)
+    legs["pad16"] = run_batch(jobs, batch_ypad=16, batch_seg=16)
     want = legs[(1, 1)]
     for i, job in enumerate(jobs):
         tag = "job %d %s %s %s->%s sigma=%s r=%s" % (i, job[0], job[1].shape, job[1].dtype, np.dtype(job[2]),
