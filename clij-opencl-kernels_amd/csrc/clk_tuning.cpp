@@ -18,8 +18,9 @@ const char* kNames[TUNE_COUNT] = {"strided_rounds", "x_sub", "batch_seg",
                                   "autotune", "xreg_units", "x_dynamic", "x_tasks", "sphere_direct",
                                   "no_xpack", "yring", "no_xtier", "no_xy", "batch_xyseg",
                                   "batch_serial", "batch_ycols", "no_xymm", "no_xyg", "hist8_rows",
-                                  "no_xyzmm", "batch_zseg", "no_zmm", "ydyn_skip", "batch_ypad", "xymm_z", "no_ymm"};
-const int32_t kDefaults[TUNE_COUNT] = {16, 0, 32, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 1, 1, 0, 1, 0, 1, 0, 1, 0, 64, 1, 1};
+                                  "no_xyzmm", "batch_zseg", "no_zmm", "ydyn_skip", "batch_ypad", "xymm_z", "no_ymm",
+                                  "yring_deep"};
+const int32_t kDefaults[TUNE_COUNT] = {16, 0, 32, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 1, 1, 0, 1, 0, 1, 0, 1, 0, 64, 1, 1, 0};
 std::atomic<int32_t> g_val[TUNE_COUNT];
 std::once_flag g_once;
 
@@ -101,6 +102,7 @@ extern "C" int32_t clk_set_tuning(const char* key, int32_t value) {
   if (k == clk::TUNE_BATCH_YPAD && value != 0 && value != 16 && value != 32 && value != 64) return CLK_INVALID_VALUE;
   if (k == clk::TUNE_XYMM_Z && value != 0 && value != 1) return CLK_INVALID_VALUE;
   if (k == clk::TUNE_NO_YMM && value != 0 && value != 1) return CLK_INVALID_VALUE;
+  if (k == clk::TUNE_YRING_DEEP && value != 0 && value != 1) return CLK_INVALID_VALUE;
   if (k == clk::TUNE_NO_ZMM && value != 0 && value != 1) return CLK_INVALID_VALUE;
   if (k == clk::TUNE_BATCH_XYSEG && (value < 0 || value > (1 << 20))) return CLK_INVALID_VALUE;
   if (k == clk::TUNE_XREG_UNITS && value != 0 && value != 1 && value != 2 && value != 4)

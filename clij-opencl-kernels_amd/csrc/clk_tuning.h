@@ -39,7 +39,8 @@ enum TuneKey : int {
   TUNE_BATCH_YPAD = 28,      // batched y passes pad each row's columns to 16, 32 or 64 lanes (0 = packed)
   TUNE_XYMM_Z = 29,          // 1 (default) = batched 16-bit min/max fuse x with z (then y) instead of x with y
   TUNE_NO_YMM = 30,          // 1 (default) = batched 16-bit min/max y passes keep the y/z ring classes
-  TUNE_COUNT = 31
+  TUNE_YRING_DEEP = 31,      // 1 = the lockstep ring y/z pass loads 4 or 8 steps ahead (smaller steps)
+  TUNE_COUNT = 32
 };
 
 int32_t tuning(TuneKey k);

@@ -549,15 +549,16 @@ int32_t launch_ystatic_cc(hipStream_t st, SepParams& p, std::string* err) {
 
 // The lockstep ring pass (float -> float): K per column width so that the
 // ring and a step's outputs fit the 128 registers a 1024-thread block
-// allows.  Segments: about rounds/8 blocks per resident slot (one block per
-// CU fits), i.e. whole axes for the y passes of large volumes by default.
-template <int OP, class TS, class TD, int C, int COLS>
-int32_t launch_yring_cc(hipStream_t st, SepParams& p, std::string* err) {
-  constexpr int K = COLS == 1 ? 16 : (COLS == 2 ? (C <= 6 ? 8 : 4) : 4);
+// allows, loads one step ahead; with the yring_deep knob, K = 4 (one
+// column) or 2 and loads 8 or 4 steps ahead (twice the rows in flight).
+// Segments: about rounds/8 blocks per resident slot (one block per CU
+// fits), i.e. whole axes for the y passes of large volumes by default.
+template <int OP, class TS, class TD, int C, int COLS, int K, int D>
+int32_t launch_yring_kd(hipStream_t st, SepParams& p, std::string* err) {
   p.ncolgroups = p.inner / COLS;
   p.ncols_total = p.outer * p.ncolgroups;
   const int64_t nblk = (p.ncols_total + kYRingThreads - 1) / kYRingThreads;
-  const auto kern = sep_yring_kernel<OP, TS, TD, C, COLS, K>;
+  const auto kern = sep_yring_kernel<OP, TS, TD, C, COLS, K, D>;
   int64_t target = (int64_t)resident_blocks((const void*)kern, 0, kYRingThreads) * strided_rounds() / 8;
   plan_segments<K>(p, nblk, target > 0 ? target : 1);
   const int64_t blocks = nblk * p.nseg;
@@ -568,6 +569,12 @@ int32_t launch_yring_cc(hipStream_t st, SepParams& p, std::string* err) {
   p.order = tuning(TUNE_STRIDED_ORDER);
   hipLaunchKernelGGL(kern, dim3((uint32_t)blocks), dim3(kYRingThreads), 0, st, p);
   return 0;
+}
+
+template <int OP, class TS, class TD, int C, int COLS>
+int32_t launch_yring_cc(hipStream_t st, SepParams& p, std::string* err) {
+  if (tuning(TUNE_YRING_DEEP)) return launch_yring_kd<OP, TS, TD, C, COLS, COLS == 1 ? 4 : 2, COLS == 1 ? 8 : 4>(st, p, err);
+  return launch_yring_kd<OP, TS, TD, C, COLS, COLS == 1 ? 16 : (COLS == 2 ? (C <= 6 ? 8 : 4) : 4), 1>(st, p, err);
 }
 
 // The lockstep ring pass for float -> float when the yring knob asks for it

@@ -239,7 +239,7 @@ struct TuneSig {
 };
 
 struct Candidate {
-  TuneSetting set[4];
+  TuneSetting set[5];
   int n;
 };
 
@@ -256,8 +256,9 @@ const Candidate kXCands[] = {
     {{{TUNE_XREG_MAXC, 16}, {TUNE_XREG_UNITS, 4}}, 2},
     {{{TUNE_XREG_MAXC, 16}, {TUNE_XREG_UNITS, 2}}, 2},
 };
-// y / z: coordinate descent over blocks per slot, block order and column
-// width (then blocks per slot again, as the best count depends on order)
+// y / z: coordinate descent over blocks per slot, block order, column
+// width and the ring pass's step shape (then blocks per slot again, as the
+// best count depends on order)
 struct TuneDim {
   TuneKey key;
   int n;
@@ -268,6 +269,7 @@ const TuneDim kYZDims[] = {
     {TUNE_STRIDED_ROUNDS, 6, {2, 4, 8, 16, 32, 64}},
     {TUNE_STRIDED_ORDER, 4, {0, 1, 2, 3}},
     {TUNE_YSTATIC_COLS, 3, {0, 1, 2}},
+    {TUNE_YRING_DEEP, 2, {0, 1}},  // float -> float only: loads 1 or 4-8 steps ahead
     {TUNE_STRIDED_ROUNDS, 6, {2, 4, 8, 16, 32, 64}},
 };
 constexpr int64_t kTuneMinVoxels = int64_t(1) << 22;
@@ -275,13 +277,13 @@ constexpr int64_t kTuneMinVoxels = int64_t(1) << 22;
 std::mutex g_tune_mu;
 std::map<TuneSig, Candidate> g_tuned;
 
-// Times every candidate in three rounds (the first warms up) and sets *bi
+// Times every candidate in four rounds (the first warms up) and sets *bi
 // to the one with the smallest of its timed launches; rc != 0 when a launch
 // failed.
 int32_t time_candidates(int op, hipStream_t st, const clk_image& dst, const clk_image& src, int32_t dim,
                         int32_t N, float s, SepParams& p, const Candidate* cands, int n, int* bi,
                         std::string* err) {
-  constexpr int kTimed = 2;
+  constexpr int kTimed = 3;
   std::vector<hipEvent_t> ev(2 * n * kTimed, nullptr);
   for (auto& e : ev)
     if (hipEventCreate(&e) != hipSuccess) {
@@ -329,15 +331,20 @@ int32_t tune_signature(int op, hipStream_t st, const clk_image& dst, const clk_i
     if (rc == 0) *best = kXCands[bi];
     return rc;
   }
-  // current point: rounds, order, cols, ring (the launcher's defaults)
+  // current point: rounds, order, cols, ring, ring steps (the launcher's defaults)
   const bool float_pair = src.dtype == CLK_FLOAT && dst.dtype == CLK_FLOAT;
-  Candidate cur{{{TUNE_STRIDED_ROUNDS, 16}, {TUNE_STRIDED_ORDER, 0}, {TUNE_YSTATIC_COLS, 0}, {TUNE_YRING, 0}},
-                4};
+  Candidate cur{{{TUNE_STRIDED_ROUNDS, 16},
+                 {TUNE_STRIDED_ORDER, 0},
+                 {TUNE_YSTATIC_COLS, 0},
+                 {TUNE_YRING, 0},
+                 {TUNE_YRING_DEEP, 0}},
+                5};
   for (const TuneDim& d : kYZDims) {
-    if (d.key == TUNE_YRING && !float_pair) continue;  // no ring pass for other pixel types
+    // no ring pass for other pixel types
+    if ((d.key == TUNE_YRING || d.key == TUNE_YRING_DEEP) && !float_pair) continue;
     Candidate c[6];
-    const int slot = d.key == TUNE_STRIDED_ROUNDS ? 0
-                     : (d.key == TUNE_STRIDED_ORDER ? 1 : (d.key == TUNE_YSTATIC_COLS ? 2 : 3));
+    int slot = 0;
+    while (cur.set[slot].key != d.key) ++slot;
     for (int i = 0; i < d.n; ++i) {
       c[i] = cur;
       c[i].set[slot].value = d.vals[i];
@@ -394,7 +401,7 @@ int32_t autotune_load(const char* path) {
     const char* q = line + used;
     char key[64];
     int val = 0, adv = 0;
-    while (c.n < 4 && sscanf(q, " %63[a-z_]=%d%n", key, &val, &adv) == 2) {
+    while (c.n < (int)(sizeof(c.set) / sizeof(c.set[0])) && sscanf(q, " %63[a-z_]=%d%n", key, &val, &adv) == 2) {
       int k = 0;
       while (k < TUNE_COUNT && strcmp(tuning_name(k), key) != 0) ++k;
       if (k == TUNE_COUNT) {

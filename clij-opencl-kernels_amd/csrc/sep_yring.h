@@ -17,6 +17,10 @@
 //     to 0.79 ms.  The gain of the z pass depends on where the two buffers
 //     lie in memory (0.77-0.89 ms against 0.85), so the autotuner decides
 //     there.
+// A step's loads are for the step D steps ahead (D = 1: the next one), so
+// a lane keeps D*K rows of COLS columns in flight while the ring grows by
+// only (D-1)*K rows; a smaller K with a larger D keeps more bytes in
+// flight for the same registers.
 // Lanes beyond the last column stay in the loop (the barrier needs them),
 // load a valid column and store nothing.
 #pragma once
@@ -25,12 +29,12 @@ namespace clk {
 
 constexpr int kYRingThreads = 1024;
 
-template <int OP, class TS, class TD, int C, int COLS, int K>
+template <int OP, class TS, class TD, int C, int COLS, int K, int D>
 __global__ __launch_bounds__(kYRingThreads) void sep_yring_kernel(const SepParams p) {
   using VT = xr_vt<OP, TS>;
   constexpr bool kFloatSrc = !Pix<TS>::kInt;
   constexpr int N = 2 * C + 1;
-  constexpr int U = (2 * K + 2 * C + K - 1) / K;  // window (K + 2C rows) + next step's K rows
+  constexpr int U = ((D + 1) * K + 2 * C + K - 1) / K;  // window (K + 2C rows) + D steps' new rows
   constexpr int RS = U * K;
   uint32_t bid = blockIdx.x;
   if (p.order & 2) {
@@ -70,7 +74,7 @@ __global__ __launch_bounds__(kYRingThreads) void sep_yring_kernel(const SepParam
     load_vt_stream<TS, VT, COLS>(reinterpret_cast<const TS*>(ls0 + (int64_t)pos * ss + lso), dst);
   };
 #pragma unroll
-  for (int i = 0; i < K + 2 * C; ++i) ldrow(i, ring[i]);
+  for (int i = 0; i < D * K + 2 * C; ++i) ldrow(i, ring[i]);
 
   TD outs[K][COLS];
   // output q of a step whose window starts at slot u*K (static)
@@ -106,11 +110,12 @@ __global__ __launch_bounds__(kYRingThreads) void sep_yring_kernel(const SepParam
       const int p0 = a0 + (s + u) * K;
       if (p0 >= a1) return;  // block-uniform: the segment is the block's
       __syncthreads();
-      if (p0 + K < a1) {
-        // the next step's new rows (relative (s+u)*K + K + 2C + i) go to
-        // slots this step does not read
+      if (p0 + D * K < a1) {
+        // the new rows of step s+u+D (relative (s+u+D)*K + 2C + i) go to
+        // slots neither this step nor the loads in flight use
 #pragma unroll
-        for (int i = 0; i < K; ++i) ldrow((s + u) * K + K + 2 * C + i, ring[(u * K + K + 2 * C + i) % RS]);
+        for (int i = 0; i < K; ++i)
+          ldrow((s + u + D) * K + 2 * C + i, ring[((u + D) * K + 2 * C + i) % RS]);
 #pragma unroll
         for (int q = 0; q < K; ++q) compute(u, q);
 #pragma unroll

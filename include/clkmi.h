@@ -357,7 +357,9 @@ int32_t clk_temporaries_held(void* stream, int64_t* stream_bytes, int64_t* pool_
  * one plane, then run y; 0: x with y, then z; same results), "no_ymm"
  * (1, the default: batched 16-bit minimum/maximum y passes keep the y/z
  * ring classes; 0: they run the register min/max kernel over rows),
- * "sphere_direct" (ellipsoid filters: 0 = auto, 1 = per-voxel
+ * "yring_deep" (1 = the lockstep ring y/z pass takes steps of 4 rows (one
+ * column per lane) or 2 and loads 8 or 4 steps ahead; 0, the default: 16, 8
+ * or 4 rows, one step ahead), "sphere_direct" (ellipsoid filters: 0 = auto, 1 = per-voxel
  * gathers, 2 = LDS row tiles without z sliding), and
  * "autotune": 1 = the first single-axis launch of each signature (pixel
  * types, axis, N, volume size; volumes of 2^22 voxels and more) times a set

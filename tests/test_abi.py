@@ -94,7 +94,8 @@ def test_tuning_knobs_validate_values():
     import clkmi
     for key, good, bad in [("autotune", 1, 2), ("strided_order", 3, 4), ("ystatic_cols", 4, 3),
                            ("xreg_maxc", 16, 17), ("strided_rounds", 64, 0), ("batch_seg", 8, 7),
-                           ("xreg_units", 4, 3), ("x_tasks", 2, 3), ("sphere_direct", 2, 3), ("yring", 2, 3)]:
+                           ("xreg_units", 4, 3), ("x_tasks", 2, 3), ("sphere_direct", 2, 3), ("yring", 2, 3),
+                           ("yring_deep", 1, 2)]:
         old = clkmi.get_tuning(key)
         try:
             clkmi.set_tuning(key, good)
@@ -113,13 +114,17 @@ def test_autotune_cache_round_trip(tmp_path):
     import clkmi
     src = tmp_path / "shapes.txt"
     src.write_text("# comment\n0 3 4 1 25 1024 1024 512 : strided_rounds=16 strided_order=1 ystatic_cols=2\n"
-                   "2 3 3 0 7 1024 1024 512 : xreg_units=4\n")
+                   "2 3 3 0 7 1024 1024 512 : xreg_units=4\n"
+                   "0 4 4 2 13 1024 1024 512 : strided_rounds=8 strided_order=2 ystatic_cols=0 yring=2 "
+                   "yring_deep=1\n")
     clkmi.autotune_load(src)
     out = tmp_path / "out.txt"
     clkmi.autotune_save(out)
     lines = [l for l in out.read_text().splitlines() if not l.startswith("#")]
     assert "0 3 4 1 25 1024 1024 512 : strided_rounds=16 strided_order=1 ystatic_cols=2" in lines
     assert "2 3 3 0 7 1024 1024 512 : xreg_units=4" in lines
+    assert ("0 4 4 2 13 1024 1024 512 : strided_rounds=8 strided_order=2 ystatic_cols=0 yring=2 yring_deep=1"
+            in lines)
     bad = tmp_path / "bad.txt"
     bad.write_text("0 3 4 1 25 1024 1024 512 : no_such_knob=1\n")
     with pytest.raises(clkmi.ClkError) as e:

@@ -901,6 +901,9 @@ RING_SETTINGS = [
     (("yring", 2), ("ystatic_cols", 2), ("strided_order", 1)),
     (("yring", 2), ("ystatic_cols", 4), ("strided_rounds", 64), ("strided_order", 3)),
     (("yring", 2), ("ystatic_cols", 0), ("strided_rounds", 16), ("strided_order", 2)),
+    (("yring", 2), ("yring_deep", 1), ("ystatic_cols", 1), ("strided_order", 1)),
+    (("yring", 2), ("yring_deep", 1), ("ystatic_cols", 2)),
+    (("yring", 2), ("yring_deep", 1), ("ystatic_cols", 4), ("strided_rounds", 64), ("strided_order", 3)),
 ]
 
 
