@@ -486,6 +486,26 @@ float clko_div_uniform(float a, float b, float r) {
   return q1;
 }
 
+/* How many numerators a of the binade [1, 2) (all 2^23 floats) give
+ * clko_div_uniform(a, b, RN(1/b)) != a / b.  Scaling a or b by a power of
+ * two scales q, the residual and the result exactly (away from underflow
+ * and overflow, which the range guard sends to the IEEE divide), so one
+ * binade of a checks every guarded numerator for this divisor. */
+int64_t clko_div_uniform_binade_mismatches(float b) {
+  const float r = 1.0f / b;
+  int64_t bad = 0;
+#pragma omp parallel for reduction(+ : bad) schedule(static)
+  for (int32_t m = 0; m < (1 << 23); ++m) {
+    union { uint32_t u; float f; } v;
+    v.u = 0x3f800000u | (uint32_t)m;
+    const float a = v.f;
+    const float got = clko_div_uniform(a, b, r);
+    const float want = a / b;
+    if (memcmp(&got, &want, sizeof(float)) != 0) ++bad;
+  }
+  return bad;
+}
+
 int clko_num_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

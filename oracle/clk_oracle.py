@@ -90,6 +90,8 @@ def _load(path):
     L.clko_convert.restype = f32
     L.clko_div_uniform.argtypes = [f32, f32, f32]
     L.clko_div_uniform.restype = f32
+    L.clko_div_uniform_binade_mismatches.argtypes = [f32]
+    L.clko_div_uniform_binade_mismatches.restype = i64
     L.clko_num_threads.argtypes = []
     L.clko_num_threads.restype = i32
     return L

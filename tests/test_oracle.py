@@ -358,6 +358,29 @@ def test_div_uniform_equals_ieee_divide():
             assert (np.isnan(got) and np.isnan(want)) or got.tobytes() == want.tobytes(), (a, b)
 
 
+def test_div_uniform_whole_binade_of_numerators():
+    """Every float numerator of a binade against the IEEE divide, for the
+    Gaussian weight sums of sigma 0.5..4 (N = int(8 sigma) made odd, the
+    host chain's rule), the mean's tap counts 1..33 and divisors whose
+    mantissas lie next to 1 and 2 (all 2^23 numerators each; the formula is
+    invariant under power-of-two scaling of either operand)."""
+    L = O.lib()
+    bs = [np.float32(n) for n in range(1, 34)]
+    for sigma in (0.5, 0.8, 1.0, 1.5, 2.0, 2.5, 3.0, 3.7, 4.0):
+        N = int(8 * sigma)
+        N += 1 - N % 2
+        c = (N - 1) // 2
+        n = np.float32(-2) * np.float32(sigma) * np.float32(sigma)
+        h = np.float32(0)
+        for v in range(-c, c + 1):
+            h = np.float32(h + L.clko_expf(np.float32(v * v) / n))
+        bs.append(h)
+    one = np.float32(1).view(np.uint32)
+    bs += [np.uint32(one + k).view(np.float32) for k in (1, 2, 3, 0x7FFFFF, 0x7FFFFE, 0x400000, 0x2AAAAB)]
+    for b in bs:
+        assert L.clko_div_uniform_binade_mismatches(float(b)) == 0, b
+
+
 # ---- the 2D siblings -----------------------------------------------------------
 
 def _specials_2d(dt, shape, seed):
