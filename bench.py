@@ -168,7 +168,7 @@ def suite_c2():
 
 
 def cpu_baseline(src_host, nthreads):
-    """The oracle's -O3 x86-64-v3 build (bit-identical to the checker) on
+    """The oracle's -O3 x86-64-v3 build (bit-identical to the -O2 test build) on
     this host's cores, on the benchmark's own uint16 volume at full size:
     the same four 3D filters as three single-axis passes each."""
     import clk_oracle as O

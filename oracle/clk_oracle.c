@@ -3,7 +3,7 @@
  * clEsperanto/clij-opencl-kernels and their five 2D siblings, voxel by
  * voxel, in the reference's own operation order.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/,
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle: only tests/,
  * __graft_entry__.smoke() and the cpu_baseline leg of bench.py may load it.
  * The product library (libclkmi) never links, calls or falls back to it.
  *

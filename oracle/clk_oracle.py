@@ -1,7 +1,7 @@
 """numpy front end of the CPU restatement in clk_oracle.c, plus an FP64
 reference of the Gaussian pass.
 
-TEST INFRASTRUCTURE ONLY: the checker.  Only tests/, __graft_entry__.smoke()
+TEST INFRASTRUCTURE ONLY: the oracle.  Only tests/, __graft_entry__.smoke()
 and the cpu_baseline leg of bench.py import this module; the product library
 never calls it.
 

@@ -605,9 +605,9 @@ def test_maximum_z_projection_equals_numpy(src_t, dst_t):
 
 
 @pytest.mark.parametrize("op", [O.OP_GAUSS, O.OP_MEAN, O.OP_MIN, O.OP_MAX])
-def test_fast_build_gives_the_checker_bits(op):
+def test_fast_build_gives_the_test_build_bits(op):
     """The -O3 x86-64-v3 build that bench.py times computes exactly what the
-    checker build computes (no contraction, no fast-math), specials
+    test build computes (no contraction, no fast-math), specials
     included: the same bits everywhere except that a NaN may carry another
     sign or payload."""
     rng = np.random.default_rng(11 + op)
