@@ -239,6 +239,39 @@ __device__ __forceinline__ void median_regs_body(const MedParams& p, uint32_t bx
   for (int dx = -MX; dx <= MX; ++dx) xs[dx + MX] = clamp64(x + dx, 0, p.W - 1);
   uint32_t key[NS];
   uint32_t lo = 0xffffffffu, hi = 0u;
+  if (p.ex == MX && p.ey == MY && p.ez == MZ && p.W * p.H * p.D * (int64_t)sizeof(TS) < (int64_t(1) << 31)) {
+    // The box fills the tier: every slot is read, with no branch between
+    // the loads, through a buffer resource whose row offset (one row y of
+    // plane z per block) is a scalar operand and whose lane offsets are
+    // the NX clamped columns: no address arithmetic per slot.
+    const __amdgpu_buffer_rsrc_t r = byte_rsrc(gen(src), p.W * p.H * p.D * (int64_t)sizeof(TS));
+    uint32_t xo[NX];
+#pragma unroll
+    for (int i = 0; i < NX; ++i) xo[i] = (uint32_t)xs[i] * (uint32_t)sizeof(TS);
+#pragma unroll
+    for (int dz = -MZ; dz <= MZ; ++dz) {
+      const int64_t zz = clamp64(z + dz, 0, p.D - 1);
+#pragma unroll
+      for (int dy = -MY; dy <= MY; ++dy) {
+        const int64_t yy = clamp64(y + dy, 0, p.H - 1);
+        const uint32_t ro = (uint32_t)((zz * p.H + yy) * p.W * (int64_t)sizeof(TS));
+#pragma unroll
+        for (int dx = -MX; dx <= MX; ++dx) {
+          TS v;
+          if constexpr (sizeof(TS) == 1)
+            v = __builtin_bit_cast(TS, __builtin_amdgcn_raw_buffer_load_b8(r, xo[dx + MX], ro, 0));
+          else if constexpr (sizeof(TS) == 2)
+            v = __builtin_bit_cast(TS, __builtin_amdgcn_raw_buffer_load_b16(r, xo[dx + MX], ro, 0));
+          else
+            v = __builtin_bit_cast(TS, __builtin_amdgcn_raw_buffer_load_b32(r, xo[dx + MX], ro, 0));
+          const uint32_t k = value_key<TD>(gather_value<TS, TD>(v, p.wrap));
+          lo = k < lo ? k : lo;
+          hi = k > hi ? k : hi;
+          key[((dz + MZ) * NY + dy + MY) * NX + dx + MX] = k;
+        }
+      }
+    }
+  } else {
 #pragma unroll
   for (int dz = -MZ; dz <= MZ; ++dz) {
     const int64_t zz = clamp64(z + dz, 0, p.D - 1);
@@ -258,6 +291,7 @@ __device__ __forceinline__ void median_regs_body(const MedParams& p, uint32_t bx
         key[((dz + MZ) * NY + dy + MY) * NX + dx + MX] = k;
       }
     }
+  }
   }
   CLK_GLOBAL TD* const out = gmem(static_cast<TD*>(p.dst)) + (z * p.H + y) * p.W + x;
   // A box that fills the tier has its rank at a compile-time position, so
