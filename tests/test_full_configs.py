@@ -12,7 +12,9 @@
   every job, and the oracle on 40 jobs including the largest volume;
 - served chains (Gaussian (2,2,1) -> maximum r=2 -> median r=1) on cubes of
   edge 64..768: each result is the same alone and in the mix, and equals
-  the oracle chain on the cubes of edge <= 300.
+  the oracle chain on the cubes of edge <= 300;
+- a 2 GiB uint16 volume (1024^3) through the register-tier median's 64-bit
+  addressed gather, on crops against the oracle.
 """
 import numpy as np
 import pytest
@@ -189,3 +191,33 @@ def test_c5_served_cubes_alone_and_mixed(gpu):
         _assert_same(a.cpu().numpy().view(np.uint16), cur, "edge %d vs oracle" % L)
     del vols, alone
     torch.cuda.empty_cache()
+
+
+def test_median_register_tier_volume_of_2gib(gpu):
+    """A uint16 volume of exactly 2 GiB (1024^3) is beyond the 32-bit byte
+    offsets of the register tier's buffer-resource gather and takes its
+    64-bit addressed gather instead: the 5x5x5 median (the network) and the
+    5x5x1 one, checked on crops at corners, edges and the centre against the
+    oracle run on the same crops with their halo."""
+    import torch
+    import clkmi
+    D = H = W = 1024
+    g = torch.Generator(device="cuda")
+    g.manual_seed(7)
+    src = torch.randint(-32768, 32768, (D, H, W), dtype=torch.int16, device="cuda", generator=g)
+    dst = torch.empty_like(src)
+    n = (8, 8, 96)  # output crop (z, y, x)
+    corners = ((0, 0, 0), (D - 8, H - 8, W - 96), (500, 511, 700), (0, H - 8, 0), (D - 8, 0, W - 96), (3, 2, 1))
+    for Ns in ((5, 5, 5), (5, 5, 1)):
+        clkmi.median_box_3d(clkmi.Image(dst.data_ptr(), 3, W, H, D), clkmi.Image(src.data_ptr(), 3, W, H, D), *Ns)
+        torch.cuda.synchronize()
+        halo = ((Ns[2] - 1) // 2, (Ns[1] - 1) // 2, (Ns[0] - 1) // 2)  # (z, y, x)
+        for c in corners:
+            lo = [max(c[i] - halo[i], 0) for i in range(3)]
+            hi = [min(c[i] + n[i] + halo[i], (D, H, W)[i]) for i in range(3)]
+            crop = src[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]].contiguous().cpu().numpy().view(np.uint16)
+            want = O.median_box_3d(crop, np.uint16, *Ns)
+            o = [c[i] - lo[i] for i in range(3)]
+            want = want[o[0]:o[0] + n[0], o[1]:o[1] + n[1], o[2]:o[2] + n[2]]
+            got = dst[c[0]:c[0] + n[0], c[1]:c[1] + n[1], c[2]:c[2] + n[2]].contiguous().cpu().numpy().view(np.uint16)
+            _assert_same(got, want, "median %s at %s" % (Ns, c))
